@@ -1,0 +1,225 @@
+"""ctypes binding of libarcwarden_hip.so (the C ABI declared in include/aw.h).
+
+The library is loaded lazily by :func:`lib`.  If it has not been built, or the
+HIP runtime cannot be loaded, :class:`LibraryMissing` is raised -- there is no
+CPU fallback for any operation.  ``torch`` is imported first when available so
+that the process keeps a single HIP runtime (torch's bundled copy and the
+system copy share SONAMEs; whichever loads first serves both).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libarcwarden_hip.so"
+
+AW_OK = 0
+AW_DEPOSIT_GLOBAL, AW_DEPOSIT_LDS_GRID, AW_DEPOSIT_BINNED, AW_DEPOSIT_AUTO = 0, 1, 2, 3
+AW_MODEL_ES, AW_MODEL_DARWIN = 0, 1
+
+
+class LibraryMissing(RuntimeError):
+    """libarcwarden_hip.so is not built or cannot be loaded."""
+
+
+class AwError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[aw status {code}] {msg}")
+        self.code = code
+
+
+class AwGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("dy", C.c_double),
+                ("Lx", C.c_double), ("Ly", C.c_double)]
+
+
+class AwRunParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("n0", C.c_double), ("vth", C.c_double), ("vd", C.c_double),
+        ("wpe", C.c_double), ("wce", C.c_double), ("B0", C.c_float * 3), ("ppc", C.c_int32),
+        ("nsteps", C.c_int64), ("dump_every", C.c_int64), ("norm", C.c_int32),
+        ("eps0", C.c_double), ("qm", C.c_double), ("c", C.c_double), ("ndc", C.c_int32),
+        ("jfilter", C.c_int32), ("tile_sort", C.c_int32), ("bnd_x", C.c_int32),
+        ("bnd_nd", C.c_int32), ("bnd_numax", C.c_double), ("deltaf", C.c_int32),
+        ("df_tpar", C.c_double), ("df_tperp", C.c_double), ("rel", C.c_int32),
+        ("cold_nc", C.c_double), ("b0_prof", C.c_int32), ("b0_a", C.c_double),
+        ("b0_xc", C.c_double), ("b0_lre", C.c_double), ("b0_sref", C.c_double),
+        ("b0_c", C.c_double * 6), ("df_dist", C.c_int32), ("df_rho", C.c_double),
+        ("df_kappa", C.c_double), ("df_taud", C.c_double), ("df_wprec", C.c_int32),
+        ("ant_amp", C.c_double), ("ant_x0", C.c_double), ("ant_sigma", C.c_double),
+        ("ant_w0", C.c_double), ("ant_trmp", C.c_double), ("ant_toff", C.c_double),
+        ("pump", C.c_int32), ("pump_ex", C.c_double), ("pump_ey", C.c_double),
+        ("pump_ez", C.c_double), ("pump_k0", C.c_double), ("pump_w0", C.c_double),
+        ("pump_trmp", C.c_double), ("pump_toff", C.c_double), ("weight", C.c_double),
+        ("ax", C.c_double), ("ay", C.c_double), ("k_filter", C.c_double),
+        ("rng_seed", C.c_uint64), ("perturb_amp", C.c_double), ("perturb_kx", C.c_int32),
+        ("two_stream", C.c_int32), ("bump_on_tail", C.c_int32), ("beam_frac", C.c_double),
+        ("beam_vd", C.c_double), ("beam_vth", C.c_double), ("noisy_load", C.c_int32),
+    ]
+
+
+class AwSpecies(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("density", C.c_double), ("ppc", C.c_int32),
+                ("uth", C.c_double * 3), ("ufl", C.c_double * 3), ("deltaf", C.c_int32),
+                ("dist", C.c_int32), ("lc_rho", C.c_double), ("lc_kappa", C.c_double),
+                ("taud", C.c_double), ("wdnoise", C.c_double)]
+
+
+class AwParticles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("ux", C.c_void_p), ("uy", C.c_void_p),
+                ("uz", C.c_void_p), ("w", C.c_void_p), ("n", C.c_int64),
+                ("w_uniform", C.c_double)]
+
+
+class AwFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "ELx", "ELy", "ETx", "ETy", "ETz")]
+
+
+class AwSources(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ("rho", "Jx", "Jy", "Jz", "dcux", "dcuy", "dcuz", "amu0", "amu1", "amu2",
+                 "amu3")]
+
+
+class AwDiagSample(C.Structure):
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("ke", C.c_double),
+                ("ee", C.c_double), ("total", C.c_double), ("charge", C.c_double),
+                ("max_e", C.c_double)]
+
+
+class AwSimConfig(C.Structure):
+    _fields_ = [("has_b0", C.c_int32), ("deposit_policy", C.c_int32),
+                ("field_model", C.c_int32), ("tile_x", C.c_int32), ("tile_y", C.c_int32),
+                ("sort_every", C.c_int32), ("stray_frac", C.c_double), ("n_modes", C.c_int32),
+                ("mode_amp", C.c_double * 32), ("legacy_load", C.c_int32),
+                ("fixed_scale", C.c_int32)]
+
+
+P = C.c_void_p
+PP = C.POINTER
+_SIGS = {
+    "aw_last_error": (None, [C.c_char_p, C.c_size_t]),
+    "aw_version": (C.c_char_p, []),
+    "aw_run_params_defaults": (None, [PP(AwRunParams)]),
+    "aw_species_defaults": (None, [PP(AwSpecies)]),
+    "aw_ctx_create": (C.c_int32, [PP(AwGrid), C.c_int32, C.c_int32, PP(P)]),
+    "aw_ctx_destroy": (C.c_int32, [P]),
+    "aw_ctx_set_fixed_scale": (C.c_int32, [P, C.c_int32]),
+    "aw_ctx_tiles": (C.c_int32, [P, PP(C.c_int32), PP(C.c_int32), PP(C.c_int32),
+                                 PP(C.c_int32)]),
+    "aw_cell_index": (C.c_int32, [P, PP(AwParticles), P, P]),
+    "aw_bin_tiles": (C.c_int32, [P, PP(AwParticles), P, P, P]),
+    "aw_sort_cells": (C.c_int32, [P, PP(AwParticles), P, P, P]),
+    "aw_sort_by_tile": (C.c_int32, [P, PP(AwParticles), PP(AwParticles), P, P]),
+    "aw_migrate": (C.c_int32, [P, PP(AwParticles), P, P]),
+    "aw_load_species": (C.c_int32, [P, PP(AwSpecies), C.c_int32, PP(AwRunParams),
+                                    PP(C.c_double), C.c_int32, PP(AwParticles),
+                                    PP(C.c_double), P]),
+    "aw_load_legacy": (C.c_int32, [P, PP(AwRunParams), PP(AwParticles), P]),
+    "aw_deposit_rho": (C.c_int32, [P, PP(AwParticles), C.c_double, C.c_int32, P, P]),
+    "aw_push_es": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32, P]),
+    "aw_half_step_back": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32,
+                                      P]),
+    "aw_push_deposit_es": (C.c_int32, [P, PP(AwParticles), P, P, P, PP(AwRunParams),
+                                       C.c_int32, P, PP(C.c_int64), P]),
+    "aw_fft_r2c": (C.c_int32, [P, P, P, P]),
+    "aw_fft_c2r": (C.c_int32, [P, P, P, P]),
+    "aw_solve_es": (C.c_int32, [P, P, P, P, C.c_double, P]),
+    "aw_reduce_energies": (C.c_int32, [P, PP(AwParticles), P, P, P, C.c_double,
+                                       PP(AwDiagSample), P]),
+    "aw_sim_create": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(AwSpecies), C.c_int32,
+                                  PP(AwSimConfig), PP(P)]),
+    "aw_sim_destroy": (C.c_int32, [P]),
+    "aw_sim_init": (C.c_int32, [P]),
+    "aw_sim_step": (C.c_int32, [P, C.c_int64]),
+    "aw_sim_run": (C.c_int32, [P, C.c_int64]),
+    "aw_sim_synchronize": (C.c_int32, [P]),
+    "aw_sim_stream": (C.c_int32, [P, PP(P)]),
+    "aw_sim_grid": (C.c_int32, [P, PP(AwGrid)]),
+    "aw_sim_particles": (C.c_int32, [P, PP(AwParticles)]),
+    "aw_sim_fields": (C.c_int32, [P, PP(AwFields)]),
+    "aw_sim_sources": (C.c_int32, [P, PP(AwSources)]),
+    "aw_sim_history": (C.c_int32, [P, PP(AwDiagSample), C.c_int64, PP(C.c_int64)]),
+    "aw_sim_diag_now": (C.c_int32, [P, C.c_int64, PP(AwDiagSample)]),
+    "aw_sim_request_sort": (C.c_int32, [P]),
+    "aw_sim_set_profiling": (C.c_int32, [P, C.c_int32]),
+    "aw_sim_kernel_time": (C.c_int32, [P, PP(C.c_double), PP(C.c_int64), PP(C.c_int64)]),
+}
+
+_LIB = None
+
+
+def exported_symbols() -> list[str]:
+    """Names this binding expects the library to export (all of include/aw.h)."""
+    return sorted(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Loads libarcwarden_hip.so once; raises LibraryMissing if unavailable."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise LibraryMissing(
+            f"{LIB_PATH} is not built; run `python -m arcwarden_amd.build` first")
+    try:  # one HIP runtime per process: let torch's copy load first if present
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    try:
+        L = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | os.RTLD_GLOBAL)
+    except OSError as e:
+        raise LibraryMissing(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(2048)
+    lib().aw_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != AW_OK:
+        raise AwError(status, last_error())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
+
+
+def run_params(**kw) -> AwRunParams:
+    rp = AwRunParams()
+    lib().aw_run_params_defaults(C.byref(rp))
+    for k, v in kw.items():
+        if k == "B0":
+            for i in range(3):
+                rp.B0[i] = float(v[i])
+        else:
+            setattr(rp, k, v)
+    return rp
+
+
+def species(name="electrons", density=1.0, ppc=0, uth=(0, 0, 0), ufl=(0, 0, 0)) -> AwSpecies:
+    sp = AwSpecies()
+    lib().aw_species_defaults(C.byref(sp))
+    sp.name = name.encode()[:31]
+    sp.density = density
+    sp.ppc = ppc
+    for i in range(3):
+        sp.uth[i] = uth[i]
+        sp.ufl[i] = ufl[i]
+    return sp
+
+
+def grid(nx, ny, Lx, Ly) -> AwGrid:
+    """Grid(nx, ny, Lx, Ly) as the reference's constructor (grid.hpp:53-56)."""
+    return AwGrid(nx, ny, Lx / nx, Ly / ny, Lx, Ly)

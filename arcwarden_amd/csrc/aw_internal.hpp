@@ -1,0 +1,339 @@
+// aw_internal.hpp -- shared host/device internals of libarcwarden_hip.
+//
+// Host side: error plumbing (AwError -> aw_status at the C ABI), RAII device
+// buffers, the aw_ctx definition, launch helpers.  Device side: the inline
+// arithmetic every kernel shares (wrap, CIC, Boris, fixed-point quantiser).
+// The FP32 sequences here are the ones the CPU oracle restates with explicit
+// fmaf; the library is compiled with -ffp-contract=off so nothing else fuses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "aw.h"
+
+namespace aw {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+struct AwError : std::runtime_error {
+    aw_status code;
+    AwError(aw_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+const std::string& last_error();
+
+[[noreturn]] inline void fail(aw_status code, const std::string& what, const char* file,
+                              int line) {
+    throw AwError(code, std::string(file) + ":" + std::to_string(line) + ": " + what);
+}
+
+#define AW_REQUIRE(cond, msg)                                                          \
+    do {                                                                               \
+        if (!(cond)) ::aw::fail(AW_ERR_INVALID, std::string(msg), __FILE__, __LINE__); \
+    } while (0)
+
+#define AW_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            ::aw::fail(e_ == hipErrorOutOfMemory ? AW_ERR_NOMEM : AW_ERR_HIP,          \
+                       std::string(#expr) + ": " + hipGetErrorName(e_) + " (" +        \
+                           hipGetErrorString(e_) + ")",                                \
+                       __FILE__, __LINE__);                                            \
+    } while (0)
+
+#define AW_FFT(expr)                                                                   \
+    do {                                                                               \
+        rocfft_status e_ = (expr);                                                     \
+        if (e_ != rocfft_status_success)                                               \
+            ::aw::fail(AW_ERR_FFT, std::string(#expr) + ": rocfft status " +           \
+                                       std::to_string((int)e_),                        \
+                       __FILE__, __LINE__);                                            \
+    } while (0)
+
+#define AW_LAUNCH_CHECK() AW_HIP(hipGetLastError())
+
+// Runs body, converting exceptions to a status + thread-local message.
+template <class F>
+aw_status guard(const char* fn, F&& body) {
+    try {
+        body();
+        return AW_OK;
+    } catch (const AwError& e) {
+        set_last_error(std::string(fn) + ": " + e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error(std::string(fn) + ": host allocation failed");
+        return AW_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(std::string(fn) + ": " + e.what());
+        return AW_ERR_INVALID;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+
+template <class T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = std::exchange(o.p_, nullptr);
+            n_ = std::exchange(o.n_, 0);
+        }
+        return *this;
+    }
+    // Grows to at least n elements (contents not preserved).
+    void reserve(size_t n) {
+        if (n > n_) resize(n);
+    }
+    void resize(size_t n) {
+        release();
+        if (n) AW_HIP(hipMalloc(&p_, n * sizeof(T)));
+        n_ = n;
+    }
+    void release() noexcept {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    void zero(hipStream_t s) {
+        if (n_) AW_HIP(hipMemsetAsync(p_, 0, n_ * sizeof(T), s));
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    void swap(DevBuf& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// uniform-B0 Boris constants (host-computed, identical formula to the oracle)
+// ---------------------------------------------------------------------------
+
+struct BorisConsts {
+    float qmh, t[3], s[3];
+};
+
+inline BorisConsts boris_make(double qm, double dt_eff, const float B0[3], bool has_b0) {
+    BorisConsts b{};
+    const double qmh = qm * 0.5 * dt_eff;
+    double t[3], t2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        t[k] = has_b0 ? qmh * (double)B0[k] : 0.0;
+        t2 += t[k] * t[k];
+    }
+    const double sf = 2.0 / (1.0 + t2);
+    b.qmh = (float)qmh;
+    for (int k = 0; k < 3; ++k) {
+        b.t[k] = (float)t[k];
+        b.s[k] = (float)(sf * t[k]);
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+struct SortScratch {
+    DevBuf<uint32_t> keyA, keyB;
+    DevBuf<int32_t> valA, valB;
+    DevBuf<uint32_t> table, sums;
+    DevBuf<int32_t> bin_off, bin_perm;  // tile bins of the last binned deposit
+};
+
+}  // namespace aw
+
+struct aw_ctx {
+    aw_grid g{};
+    int tx = 16, ty = 16, ntx = 0, nty = 0, ntiles = 0;
+    int apron = 2;
+    int fix_log2 = 40;
+    int device = 0, num_cu = 0;
+    size_t lds_per_block = 0;
+    // rocFFT
+    rocfft_plan r2c = nullptr, c2r = nullptr, c2r2 = nullptr;
+    rocfft_plan r2c3 = nullptr, c2r3 = nullptr;
+    rocfft_execution_info info = nullptr;
+    aw::DevBuf<unsigned char> fft_work;
+    // workspace
+    aw::DevBuf<float> ktmp;                  // complex scratch, 2 floats per mode
+    aw::DevBuf<unsigned long long> acc;      // fixed-point grid accumulator
+    aw::DevBuf<double> red;                  // reduction partials
+    aw::SortScratch sort;
+    ~aw_ctx();
+};
+
+namespace aw {
+
+inline int64_t ncell(const aw_ctx& c) { return (int64_t)c.g.nx * c.g.ny; }
+inline int64_t nkmodes(const aw_ctx& c) { return (int64_t)(c.g.nx / 2 + 1) * c.g.ny; }
+inline hipStream_t S(aw_stream s) { return (hipStream_t)s; }
+
+void fft_exec(aw_ctx& c, rocfft_plan plan, void* in, void* out, hipStream_t s);
+
+// Sort / key machinery (particles.hip)
+void compute_tile_keys(aw_ctx& c, const aw_particles& p, uint32_t* key, bool cell_key,
+                       hipStream_t s);
+// Stable LSD radix sort of (key, index) pairs; perm receives the permutation,
+// off (nkeys+1) the key offsets.  keys are consumed.
+void stable_sort_keys(aw_ctx& c, int64_t n, uint32_t nkeys, int32_t* perm, int32_t* off,
+                      hipStream_t s);
+void permute_particles(const aw_particles& src, const aw_particles& dst, const int32_t* perm,
+                       hipStream_t s);
+
+// Deposit helpers (deposit.hip)
+void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
+                      bool zero_acc, hipStream_t s);
+void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
+                       const int32_t* sorted_off, hipStream_t s);
+
+// Spectral (spectral.hip)
+void solve_es_contig(aw_ctx& c, const float* rho, float* E2, double eps0, hipStream_t s);
+
+}  // namespace aw
+
+// ===========================================================================
+// device helpers
+// ===========================================================================
+
+namespace aw::dev {
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    if (i >= n) i -= n;
+    if (i < 0) i += n;
+    return i;
+}
+
+__device__ __forceinline__ int wrapfar(int i, int n) {
+    i %= n;
+    if (i < 0) i += n;
+    return i;
+}
+
+// Wrapped CIC pair (i, i+1) valid for ANY int i (a NaN / inf position gives
+// an arbitrary but in-range index instead of an out-of-bounds address).
+__device__ __forceinline__ void wrap_pair(int i, int n, int& a, int& b) {
+    a = wrapfar(i, n);
+    b = (a + 1 == n) ? 0 : a + 1;
+}
+
+// Tile-local LDS coordinate test: l in [0, lim] as an unsigned compare, so a
+// garbage negative index fails and takes the (safe) global path.
+__device__ __forceinline__ bool in_tile(int l, int lim) {
+    return (unsigned)l <= (unsigned)lim;
+}
+
+// Periodic wrap with the reference migrate semantics; identical to the
+// oracle's wrap_coord (one-period fast paths are exact rewrites of fmodf).
+__device__ __forceinline__ float wrap_coord(float v, float n) {
+    if (v >= n) {
+        v = (v < 2.0f * n) ? v - n : fmodf(v, n);
+    } else if (v < 0.0f) {
+        if (!(v > -n)) v = fmodf(v, n);
+        if (v < 0.0f) v += n;
+    }
+    if (v >= n) v = 0.0f;
+    return v;
+}
+
+__device__ __forceinline__ int cell_of(float x, float y, int nx, int ny) {
+    int ci = (int)floorf(x), cj = (int)floorf(y);
+    if (ci >= nx) ci = nx - 1;
+    if (cj >= ny) cj = ny - 1;
+    if (ci < 0) ci = 0;
+    if (cj < 0) cj = 0;
+    return cj * nx + ci;
+}
+
+struct CicF {
+    int i0, j0;
+    float w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ CicF cic_f(float x, float y) {
+    CicF s;
+    s.i0 = (int)floorf(x);
+    s.j0 = (int)floorf(y);
+    const float fx = x - (float)s.i0, fy = y - (float)s.j0;
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    s.w0 = gx * gy;
+    s.w1 = fx * gy;
+    s.w2 = gx * fy;
+    s.w3 = fx * fy;
+    return s;
+}
+
+// gather as the oracle's fma chain: w3*f3 + (w2*f2 + (w1*f1 + w0*f0))
+__device__ __forceinline__ float gather4(const CicF& s, float f0, float f1, float f2, float f3) {
+    return fmaf(s.w3, f3, fmaf(s.w2, f2, fmaf(s.w1, f1, s.w0 * f0)));
+}
+
+// boris_velocity_update, ES form (no Ez kick)
+__device__ __forceinline__ void boris_es(const BorisConsts& b, bool has_b0, float ex, float ey,
+                                         float& ux, float& uy, float& uz) {
+    ux = fmaf(b.qmh, ex, ux);
+    uy = fmaf(b.qmh, ey, uy);
+    if (has_b0) {
+        const float upx = fmaf(uy, b.t[2], fmaf(-uz, b.t[1], ux));
+        const float upy = fmaf(uz, b.t[0], fmaf(-ux, b.t[2], uy));
+        const float upz = fmaf(ux, b.t[1], fmaf(-uy, b.t[0], uz));
+        ux = fmaf(upy, b.s[2], fmaf(-upz, b.s[1], ux));
+        uy = fmaf(upz, b.s[0], fmaf(-upx, b.s[2], uy));
+        uz = fmaf(upx, b.s[1], fmaf(-upy, b.s[0], uz));
+    }
+    ux = fmaf(b.qmh, ex, ux);
+    uy = fmaf(b.qmh, ey, uy);
+}
+
+// CIC weights in double (exact products of 24-bit factors) and the
+// fixed-point quantiser: q = rint(coef * W * 2^S).
+struct CicD {
+    int i0, j0;
+    double w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ CicD cic_d(float x, float y) {
+    CicD s;
+    s.i0 = (int)floorf(x);
+    s.j0 = (int)floorf(y);
+    const double fx = (double)x - (double)s.i0, fy = (double)y - (double)s.j0;
+    const double gx = 1.0 - fx, gy = 1.0 - fy;
+    s.w0 = gx * gy;
+    s.w1 = fx * gy;
+    s.w2 = gx * fy;
+    s.w3 = fx * fy;
+    return s;
+}
+
+__device__ __forceinline__ unsigned long long fixq(double coef, double w, double scale) {
+    return (unsigned long long)(long long)rint(coef * w * scale);
+}
+
+}  // namespace aw::dev

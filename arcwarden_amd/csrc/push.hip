@@ -1,0 +1,281 @@
+// push.hip -- electrostatic Boris push kernels.
+//
+//   push_es_kernel        Pusher<Cfg>::boris (include/pic/pusher.hpp:112-129)
+//   half_back_kernel      Pusher<Cfg>::half_step_back (:133-145)
+//   push_deposit_kernel   the MI355X hot path: one pass per particle that
+//                         gathers E^n from an LDS field tile, pushes, moves,
+//                         wraps and deposits rho(x^{n+1}) into an LDS tile
+//                         (fixed-point, flushed once per workgroup).  It does
+//                         the work of boris_push_tiled_kernel (:154-202) +
+//                         particle_migrate_kernel (particles.hpp:293-315) +
+//                         the next step's deposit_charge_tiled_kernel
+//                         (depositor.hpp:112-153) for 40 B of particle traffic.
+#include "aw_internal.hpp"
+
+namespace aw {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
+
+struct PushArgs {
+    float *x, *y, *ux, *uy, *uz;
+    int n, nx, ny;
+    const float* Ex;
+    const float* Ey;
+    BorisConsts bc;
+    float cx, cy;  // dt/dx, dt/dy in cell units
+    int has_b0, move;
+};
+
+__device__ __forceinline__ void gather_global(const float* __restrict__ Ex,
+                                              const float* __restrict__ Ey, const dev::CicF& s,
+                                              int nx, int ny, float& ex, float& ey) {
+    int i0, i1, j0, j1;
+    dev::wrap_pair(s.i0, nx, i0, i1);
+    dev::wrap_pair(s.j0, ny, j0, j1);
+    const int c0 = j0 * nx + i0, c1 = j0 * nx + i1, c2 = j1 * nx + i0, c3 = j1 * nx + i1;
+    ex = dev::gather4(s, Ex[c0], Ex[c1], Ex[c2], Ex[c3]);
+    ey = dev::gather4(s, Ey[c0], Ey[c1], Ey[c2], Ey[c3]);
+}
+
+__global__ void __launch_bounds__(kThreads) push_es_kernel(PushArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float x = a.x[i], y = a.y[i];
+    float ux = a.ux[i], uy = a.uy[i], uz = a.uz[i];
+    const dev::CicF s = dev::cic_f(x, y);
+    float ex, ey;
+    gather_global(a.Ex, a.Ey, s, a.nx, a.ny, ex, ey);
+    dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
+    a.ux[i] = ux;
+    a.uy[i] = uy;
+    a.uz[i] = uz;
+    if (a.move) {
+        a.x[i] = fmaf(ux, a.cx, x);
+        a.y[i] = fmaf(uy, a.cy, y);
+    }
+}
+
+struct FusedArgs {
+    float *x, *y, *ux, *uy, *uz;
+    const float* w;  // nullptr: uniform weight
+    const int32_t* off;
+    int ntx, tx, ty, apron, W, H, bpt;
+    int nx, ny;
+    const float* Ex;
+    const float* Ey;
+    unsigned long long* acc;
+    BorisConsts bc;
+    float cx, cy;
+    int has_b0;
+    double qm, area, coef_u, scale;
+    unsigned long long* strays;
+    double* ke_part;  // per-workgroup sum of w*u^2 (nullptr: skip)
+};
+
+__device__ __forceinline__ int local_index(int i, int org, int n) {
+    int l = i - org;
+    if (l < 0) l += n;
+    else if (l >= n) l -= n;
+    return l;
+}
+
+template <bool PER_W, bool KE>
+__global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
+    extern __shared__ unsigned long long smem[];
+    const int W = a.W, H = a.H, R = W * H;
+    unsigned long long* s_rho = smem;
+    float* s_ex = reinterpret_cast<float*>(smem + R);
+    float* s_ey = s_ex + R;
+
+    const int tile = blockIdx.x / a.bpt, part = blockIdx.x % a.bpt;
+    const int orgx = (tile % a.ntx) * a.tx - a.apron;
+    const int orgy = (tile / a.ntx) * a.ty - a.apron;
+    for (int c = threadIdx.x; c < R; c += kThreads) {
+        const int gx = dev::wrapfar(orgx + c % W, a.nx);
+        const int gy = dev::wrapfar(orgy + c / W, a.ny);
+        s_ex[c] = a.Ex[gy * a.nx + gx];
+        s_ey[c] = a.Ey[gy * a.nx + gx];
+        s_rho[c] = 0ull;
+    }
+    __syncthreads();
+
+    const int beg = a.off[tile], end = a.off[tile + 1];
+    const int chunk = (end - beg + a.bpt - 1) / a.bpt;
+    const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
+    const float fnx = (float)a.nx, fny = (float)a.ny;
+    unsigned int nstray = 0;
+    double ke = 0.0;
+
+    for (int k = r0 + (int)threadIdx.x; k < r1; k += kThreads) {
+        const float x = a.x[k], y = a.y[k];
+        float ux = a.ux[k], uy = a.uy[k], uz = a.uz[k];
+
+        // gather E at x^n (LDS tile, global fallback for strays)
+        const dev::CicF s = dev::cic_f(x, y);
+        const int li = local_index(s.i0, orgx, a.nx), lj = local_index(s.j0, orgy, a.ny);
+        float ex, ey;
+        if (dev::in_tile(li, W - 2) && dev::in_tile(lj, H - 2)) {
+            const int b = lj * W + li;
+            ex = dev::gather4(s, s_ex[b], s_ex[b + 1], s_ex[b + W], s_ex[b + W + 1]);
+            ey = dev::gather4(s, s_ey[b], s_ey[b + 1], s_ey[b + W], s_ey[b + W + 1]);
+        } else {
+            gather_global(a.Ex, a.Ey, s, a.nx, a.ny, ex, ey);
+            ++nstray;
+        }
+
+        // Boris (uniform B0), move, periodic wrap
+        dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
+        const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), fnx);
+        const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), fny);
+        a.x[k] = xn;
+        a.y[k] = yn;
+        a.ux[k] = ux;
+        a.uy[k] = uy;
+        a.uz[k] = uz;
+
+        double coef;
+        if constexpr (PER_W) {
+            const float wk = a.w[k];
+            coef = a.qm * (double)wk / a.area;
+            if constexpr (KE) ke += (double)wk * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
+        } else {
+            coef = a.coef_u;
+            if constexpr (KE) ke += (double)ux * ux + (double)uy * uy + (double)uz * uz;
+        }
+
+        // deposit rho(x^{n+1})
+        const dev::CicD d = dev::cic_d(xn, yn);
+        const int mi = local_index(d.i0, orgx, a.nx), mj = local_index(d.j0, orgy, a.ny);
+        const unsigned long long q0 = dev::fixq(coef, d.w0, a.scale);
+        const unsigned long long q1 = dev::fixq(coef, d.w1, a.scale);
+        const unsigned long long q2 = dev::fixq(coef, d.w2, a.scale);
+        const unsigned long long q3 = dev::fixq(coef, d.w3, a.scale);
+        if (dev::in_tile(mi, W - 2) && dev::in_tile(mj, H - 2)) {
+            const int b = mj * W + mi;
+            atomicAdd(&s_rho[b], q0);
+            atomicAdd(&s_rho[b + 1], q1);
+            atomicAdd(&s_rho[b + W], q2);
+            atomicAdd(&s_rho[b + W + 1], q3);
+        } else {
+            int i0, i1, j0, j1;
+            dev::wrap_pair(d.i0, a.nx, i0, i1);
+            dev::wrap_pair(d.j0, a.ny, j0, j1);
+            atomicAdd(&a.acc[j0 * a.nx + i0], q0);
+            atomicAdd(&a.acc[j0 * a.nx + i1], q1);
+            atomicAdd(&a.acc[j1 * a.nx + i0], q2);
+            atomicAdd(&a.acc[j1 * a.nx + i1], q3);
+            ++nstray;
+        }
+    }
+
+    // per-workgroup stray count and kinetic-energy partial
+    __shared__ unsigned int s_stray;
+    __shared__ double s_ke[kThreads / 64];
+    if (threadIdx.x == 0) s_stray = 0;
+    __syncthreads();
+    if (nstray) atomicAdd(&s_stray, nstray);
+    if constexpr (KE) {
+        for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
+        if ((threadIdx.x & 63) == 0) s_ke[threadIdx.x >> 6] = ke;
+    }
+    __syncthreads();
+
+    // flush the LDS charge tile (every cell once)
+    for (int c = threadIdx.x; c < R; c += kThreads) {
+        const unsigned long long v = s_rho[c];
+        if (v) {
+            const int gx = dev::wrapfar(orgx + c % W, a.nx);
+            const int gy = dev::wrapfar(orgy + c / W, a.ny);
+            atomicAdd(&a.acc[gy * a.nx + gx], v);
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (s_stray && a.strays) atomicAdd(a.strays, (unsigned long long)s_stray);
+        if constexpr (KE) {
+            double t = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) t += s_ke[w];
+            a.ke_part[blockIdx.x] = t;
+        }
+    }
+}
+
+}  // namespace
+
+int tile_bpt(const aw_ctx& c, int64_t n);
+
+void push_es(aw_ctx& c, const aw_particles& p, const float* Ex, const float* Ey,
+             const aw_run_params& rp, bool has_b0, bool half_back, hipStream_t s) {
+    if (p.n == 0) return;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    PushArgs a{};
+    a.x = p.x;
+    a.y = p.y;
+    a.ux = p.ux;
+    a.uy = p.uy;
+    a.uz = p.uz;
+    a.n = (int)p.n;
+    a.nx = c.g.nx;
+    a.ny = c.g.ny;
+    a.Ex = Ex;
+    a.Ey = Ey;
+    a.bc = boris_make(rp.qm, half_back ? -0.5 * rp.dt : rp.dt, rp.B0, has_b0);
+    a.cx = (float)(rp.dt / c.g.dx);
+    a.cy = (float)(rp.dt / c.g.dy);
+    a.has_b0 = has_b0 ? 1 : 0;
+    a.move = half_back ? 0 : 1;
+    push_es_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a);
+    AW_LAUNCH_CHECK();
+}
+
+// Returns the number of workgroups launched (size of ke_part).
+int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
+                    const float* Ey, const aw_run_params& rp, bool has_b0,
+                    unsigned long long* strays, double* ke_part, int bpt, hipStream_t s) {
+    if (p.n == 0) return 0;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    FusedArgs a{};
+    a.x = p.x;
+    a.y = p.y;
+    a.ux = p.ux;
+    a.uy = p.uy;
+    a.uz = p.uz;
+    a.w = p.w;
+    a.off = off;
+    a.ntx = c.ntx;
+    a.tx = c.tx;
+    a.ty = c.ty;
+    a.apron = c.apron;
+    a.W = c.tx + 2 * c.apron + 1;
+    a.H = c.ty + 2 * c.apron + 1;
+    a.bpt = bpt > 0 ? bpt : tile_bpt(c, p.n);
+    a.nx = c.g.nx;
+    a.ny = c.g.ny;
+    a.Ex = Ex;
+    a.Ey = Ey;
+    a.acc = c.acc.get();
+    a.bc = boris_make(rp.qm, rp.dt, rp.B0, has_b0);
+    a.cx = (float)(rp.dt / c.g.dx);
+    a.cy = (float)(rp.dt / c.g.dy);
+    a.has_b0 = has_b0 ? 1 : 0;
+    a.qm = rp.qm;
+    a.area = c.g.dx * c.g.dy;
+    a.coef_u = rp.qm * p.w_uniform / a.area;
+    a.scale = std::ldexp(1.0, c.fix_log2);
+    a.strays = strays;
+    a.ke_part = ke_part;
+    const size_t sh = (size_t)a.W * a.H * (8 + 4 + 4);
+    const int nblk = c.ntiles * a.bpt;
+    const bool pw = p.w != nullptr, ke = ke_part != nullptr;
+    if (pw && ke) push_deposit_kernel<true, true><<<nblk, kThreads, sh, s>>>(a);
+    else if (pw) push_deposit_kernel<true, false><<<nblk, kThreads, sh, s>>>(a);
+    else if (ke) push_deposit_kernel<false, true><<<nblk, kThreads, sh, s>>>(a);
+    else push_deposit_kernel<false, false><<<nblk, kThreads, sh, s>>>(a);
+    AW_LAUNCH_CHECK();
+    return nblk;
+}
+
+}  // namespace aw

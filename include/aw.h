@@ -1,0 +1,319 @@
+/*
+ * aw.h -- C ABI of the MI355X-native ArcWarden 2D3V PIC hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an aw_status (0 = success).  On failure aw_last_error() returns a
+ * message naming the call, the file:line and the HIP / rocFFT error, the
+ * same information the reference's CudaError carries
+ * (include/pic/cuda_utils.hpp:29-77).
+ *
+ * Device memory passed in aw_particles / aw_fields / float* arguments is
+ * CALLER-owned (hipMalloc'd or torch-allocated), as in the reference, whose
+ * kernels receive non-owning DeviceViews (include/pic/device_array.hpp:25-40).
+ * Every launch is stream-ordered on the aw_stream argument (a hipStream_t;
+ * NULL = the legacy default stream).  An aw_ctx owns per-grid resources
+ * (rocFFT plans, k-space workspace, sort scratch) -- the reference's
+ * SpectralEngine + Particles scratch (include/pic/spectral.hpp:217-260,
+ * include/pic/particles.hpp:460-469).  Contexts hold no process-wide state,
+ * so several grids (ensembles) can live side by side.
+ *
+ * Conventions (reference include/pic/grid.hpp:14-29, particles.hpp:12-16):
+ *   positions x,y in cell units, x in [0,nx), y in [0,ny);
+ *   fields row-major [ny][nx], x contiguous;
+ *   k-space: interleaved complex64 [ny][nx/2+1];
+ *   u = gamma*v with gamma == 1.
+ *
+ * Charge / current accumulation is deterministic: every per-particle
+ * contribution is quantised to a 2^-S fixed-point integer (S chosen per
+ * context, aw_ctx_set_fixed_scale) and summed with integer atomics, so the
+ * result does not depend on thread order and all deposit policies return
+ * identical bits.
+ */
+#ifndef AW_H
+#define AW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t aw_status;
+#define AW_OK 0
+#define AW_ERR_INVALID 1     /* bad argument / shape */
+#define AW_ERR_HIP 2         /* HIP runtime error */
+#define AW_ERR_FFT 3         /* rocFFT error */
+#define AW_ERR_NOMEM 4       /* device allocation failed */
+#define AW_ERR_UNSUPPORTED 5 /* option not implemented for this path */
+#define AW_ERR_STATE 6       /* call out of order (e.g. step before init) */
+
+typedef void* aw_stream; /* hipStream_t */
+
+/* ---- plain data (mirrors of the reference's POD types) ------------------ */
+
+/* Grid -- include/pic/grid.hpp:44-57 (nx,ny,dx,dy,Lx,Ly). */
+typedef struct aw_grid {
+    int32_t nx, ny;
+    double dx, dy, Lx, Ly;
+} aw_grid;
+
+/* RunParams -- include/pic/config.hpp:127-313, member for member (bool as
+ * int32).  Only the electrostatic / Darwin / pump / loader members are read
+ * by this library; the Yee-branch members are carried so decks round-trip. */
+typedef struct aw_run_params {
+    double dt, n0, vth, vd, wpe, wce;
+    float B0[3];
+    int32_t ppc;
+    int64_t nsteps, dump_every;
+    int32_t norm;
+    double eps0, qm, c;
+    int32_t ndc;
+    int32_t jfilter, tile_sort, bnd_x, bnd_nd;
+    double bnd_numax;
+    int32_t deltaf;
+    double df_tpar, df_tperp;
+    int32_t rel;
+    double cold_nc;
+    int32_t b0_prof;
+    double b0_a, b0_xc, b0_lre, b0_sref, b0_c[6];
+    int32_t df_dist;
+    double df_rho, df_kappa, df_taud;
+    int32_t df_wprec;
+    double ant_amp, ant_x0, ant_sigma, ant_w0, ant_trmp, ant_toff;
+    int32_t pump;
+    double pump_ex, pump_ey, pump_ez, pump_k0, pump_w0, pump_trmp, pump_toff;
+    double weight, ax, ay, k_filter;
+    uint64_t rng_seed;
+    double perturb_amp;
+    int32_t perturb_kx;
+    int32_t two_stream, bump_on_tail;
+    double beam_frac, beam_vd, beam_vth;
+    int32_t noisy_load;
+} aw_run_params;
+
+/* Species -- include/pic/species.hpp:20-45. */
+typedef struct aw_species {
+    char name[32];
+    double density;
+    int32_t ppc;
+    double uth[3], ufl[3];
+    int32_t deltaf, dist;
+    double lc_rho, lc_kappa, taud, wdnoise;
+} aw_species;
+
+/* Particle SoA (include/pic/particles.hpp:47-57): caller-owned device
+ * arrays.  w == NULL means every particle carries weight w_uniform. */
+typedef struct aw_particles {
+    float *x, *y, *ux, *uy, *uz;
+    float* w;
+    int64_t n;
+    double w_uniform;
+} aw_particles;
+
+/* Real-space fields (include/pic/fields.hpp:197-200 + Darwin members). */
+typedef struct aw_fields {
+    float *Ex, *Ey, *Ez, *Bx, *By, *Bz;
+    float *ELx, *ELy, *ETx, *ETy, *ETz;
+} aw_fields;
+
+/* Real-space sources (include/pic/sources.hpp:106-113). */
+typedef struct aw_sources {
+    float* rho;
+    float *Jx, *Jy, *Jz;
+    float *dcux, *dcuy, *dcuz;
+    float *amu0, *amu1, *amu2, *amu3;
+} aw_sources;
+
+/* DiagSample -- include/pic/diagnostics.hpp:36-44. */
+typedef struct aw_diag_sample {
+    int64_t step;
+    double time, ke, ee, total, charge, max_e;
+} aw_diag_sample;
+
+/* Deposit policies (include/pic/config.hpp:58-66). */
+#define AW_DEPOSIT_GLOBAL 0    /* AtomicGlobalDeposit */
+#define AW_DEPOSIT_LDS_GRID 1  /* SharedTileDeposit: whole grid in LDS */
+#define AW_DEPOSIT_BINNED 2    /* TiledBinnedDeposit: tile + halo in LDS */
+#define AW_DEPOSIT_AUTO 3      /* pick per grid (the reference's charge()) */
+
+/* Field models (include/pic/config.hpp:74). */
+#define AW_MODEL_ES 0
+#define AW_MODEL_DARWIN 1
+
+/* ---- errors -------------------------------------------------------------- */
+
+/* Copies the calling thread's last error message (NUL-terminated). */
+void aw_last_error(char* buf, size_t len);
+/* Library version string, e.g. "arcwarden-amd 0.1 gfx950". */
+const char* aw_version(void);
+
+/* Default-constructed RunParams / Species: the member initialisers of
+ * include/pic/config.hpp:127-313 and include/pic/species.hpp:20-45. */
+void aw_run_params_defaults(aw_run_params* rp);
+void aw_species_defaults(aw_species* sp);
+
+/* ---- context ------------------------------------------------------------- */
+
+typedef struct aw_ctx aw_ctx;
+
+/* Creates a context for one grid: rocFFT R2C/C2R plans of shape (ny,nx)
+ * (include/pic/fft.hpp:26-35), k-space workspace, tile geometry tx*ty
+ * (0 -> 16x16, clamped to the grid).  Replaces SpectralEngine(g)
+ * (include/pic/spectral.hpp:217-227). */
+aw_status aw_ctx_create(const aw_grid* g, int32_t tile_x, int32_t tile_y, aw_ctx** out);
+aw_status aw_ctx_destroy(aw_ctx* ctx);
+/* Fixed-point exponent S for rho (contributions quantised to 2^-S);
+ * J / amu / dcu use S minus a per-quantity headroom.  Default S = 40. */
+aw_status aw_ctx_set_fixed_scale(aw_ctx* ctx, int32_t scale_log2);
+aw_status aw_ctx_tiles(const aw_ctx* ctx, int32_t* ntiles, int32_t* ntx, int32_t* tx,
+                       int32_t* ty);
+
+/* ---- integer layer: cells, tile bins, sort -------------------------------- */
+
+/* cell[i] = floor(y)*nx + floor(x), clamped (include/pic/particles.hpp:310-314). */
+aw_status aw_cell_index(aw_ctx* ctx, const aw_particles* p, int32_t* cell, aw_stream s);
+/* Tile binning without moving particles -- Particles::build_tile_bins
+ * (include/pic/particles.hpp:632-671): off[ntiles+1], idx[n].  The order
+ * inside a tile is STABLE (ascending particle index), unlike the
+ * reference's atomic-order slots (:149-152). */
+aw_status aw_bin_tiles(aw_ctx* ctx, const aw_particles* p, int32_t* off, int32_t* idx,
+                       aw_stream s);
+/* Stable sort of particles by tile-major cell key (tile*tx*ty + in-tile
+ * cell): perm[n] (perm[k] = index of k-th particle), off[ncellkeys+1]. */
+aw_status aw_sort_cells(aw_ctx* ctx, const aw_particles* p, int32_t* perm, int32_t* off,
+                        aw_stream s);
+/* Physical tile sort -- Particles::sort_by_tile (include/pic/particles.hpp:
+ * 678-726): stable by tile, SoA moved from src into dst (distinct buffers,
+ * same n), tile offsets into off[ntiles+1]. */
+aw_status aw_sort_by_tile(aw_ctx* ctx, const aw_particles* src, aw_particles* dst,
+                          int32_t* off, aw_stream s);
+/* particle_migrate_kernel (include/pic/particles.hpp:293-315): periodic wrap;
+ * cell may be NULL. */
+aw_status aw_migrate(aw_ctx* ctx, aw_particles* p, int32_t* cell, aw_stream s);
+
+/* ---- loaders (setup) ------------------------------------------------------ */
+
+/* Particles::initialize(species...) (include/pic/particles.hpp:533-559):
+ * species k fills [base_k, base_k + ppc_k*nx*ny) of p (p->n must equal the
+ * total).  Particle (cell c, slot s) of species k is drawn exactly as the
+ * reference draws index base+c*ppc+s and stored at base + s*nx*ny + c.
+ * mode_amp[m-1] (physical length) adds x += sum_m A_m sin(2 pi m x/Lx)
+ * (config extension; n_modes may be 0). weights_out (nspecies doubles)
+ * receives density*dx*dy/ppc per species. */
+aw_status aw_load_species(aw_ctx* ctx, const aw_species* sp, int32_t nspecies,
+                          const aw_run_params* rp, const double* mode_amp,
+                          int32_t n_modes, aw_particles* p, double* weights_out,
+                          aw_stream s);
+/* Particles::initialize(rp) single-species legacy load with perturb_amp,
+ * two_stream and bump_on_tail (include/pic/particles.hpp:213-290,521-527). */
+aw_status aw_load_legacy(aw_ctx* ctx, const aw_run_params* rp, aw_particles* p,
+                         aw_stream s);
+
+/* ---- deposits ---------------------------------------------------------------- */
+
+/* Depositor<Cfg>::charge (include/pic/depositor.hpp:34-153,440-496):
+ * rho[c] += sum_p qm*w_p/(dx*dy) * W_cic.  Accumulates into rho (caller
+ * zeroes it, as in the reference).  policy = AW_DEPOSIT_*. */
+aw_status aw_deposit_rho(aw_ctx* ctx, const aw_particles* p, double qm, int32_t policy,
+                         float* rho, aw_stream s);
+
+/* ---- push ---------------------------------------------------------------------- */
+
+/* Pusher<Cfg>::boris (include/pic/pusher.hpp:112-129,265-275): CIC gather of
+ * (Ex,Ey), Boris with uniform rp.B0 when has_b0, x += u*dt/dx.  No wrap
+ * (call aw_migrate), as in the reference. */
+aw_status aw_push_es(aw_ctx* ctx, aw_particles* p, const float* Ex, const float* Ey,
+                     const aw_run_params* rp, int32_t has_b0, aw_stream s);
+/* Pusher<Cfg>::half_step_back (include/pic/pusher.hpp:133-145,321-330). */
+aw_status aw_half_step_back(aw_ctx* ctx, aw_particles* p, const float* Ex,
+                            const float* Ey, const aw_run_params* rp, int32_t has_b0,
+                            aw_stream s);
+/* The MI355X hot path: one pass per particle that gathers E^n from an LDS
+ * field tile, pushes (Boris, uniform B0), moves, wraps, and deposits
+ * rho(x^{n+1}) into an LDS tile flushed once per workgroup.  Particles must
+ * be tile-sorted (off from aw_sort_by_tile); particles that left their
+ * tile's apron since the sort fall back to global gathers/atomics.
+ * rho_next += rho(x^{n+1}).  Replaces the reference's
+ * boris_push_tiled_kernel + particle_migrate_kernel + next step's
+ * deposit_charge_tiled_kernel (include/pic/pusher.hpp:154-202;
+ * particles.hpp:293-315; depositor.hpp:112-153).  strays (may be NULL)
+ * receives the number of fallback particles. */
+aw_status aw_push_deposit_es(aw_ctx* ctx, aw_particles* p, const int32_t* off,
+                             const float* Ex, const float* Ey, const aw_run_params* rp,
+                             int32_t has_b0, float* rho_next, int64_t* strays,
+                             aw_stream s);
+
+/* ---- spectral -------------------------------------------------------------------- */
+
+/* SpectralEngine::r2c / c2r (include/pic/spectral.hpp:228-243): unnormalised
+ * R2C; C2R applies 1/(nx*ny) and may overwrite its input. */
+aw_status aw_fft_r2c(aw_ctx* ctx, const float* in, void* out_k, aw_stream s);
+aw_status aw_fft_c2r(aw_ctx* ctx, void* in_k, float* out, aw_stream s);
+/* ElectrostaticSpectralSolver::solve (include/pic/solver_es.hpp:85-106):
+ * rho -> E = -grad phi, phi_k = rho_k/(eps0 k^2), DC and Nyquist zeroed. */
+aw_status aw_solve_es(aw_ctx* ctx, const float* rho, float* Ex, float* Ey, double eps0,
+                      aw_stream s);
+
+/* ---- diagnostics ------------------------------------------------------------------- */
+
+/* Diagnostics::compute (include/pic/diagnostics.hpp:58-108) as an on-device
+ * FP64 reduction: KE = 1/2 sum w u^2, EE = 1/2 eps0 sum |E|^2 dx dy,
+ * charge = sum rho dx dy, max|E|.  rho may be NULL (charge = 0).  Blocks
+ * until the result is on the host. */
+aw_status aw_reduce_energies(aw_ctx* ctx, const aw_particles* p, const float* Ex,
+                             const float* Ey, const float* rho, double eps0,
+                             aw_diag_sample* out, aw_stream s);
+
+/* ---- simulation (Simulation<Cfg>, include/pic/simulation.hpp:38-140) ------------- */
+
+typedef struct aw_sim aw_sim;
+
+/* SimConfig (include/pic/config.hpp:81-98) as run-time values plus the
+ * MI355X execution knobs.  Zero-initialise and set what you need. */
+typedef struct aw_sim_config {
+    int32_t has_b0;         /* SimConfig::has_b0 */
+    int32_t deposit_policy; /* AW_DEPOSIT_* used for init deposits */
+    int32_t field_model;    /* AW_MODEL_ES | AW_MODEL_DARWIN */
+    int32_t tile_x, tile_y; /* 0 -> 16 */
+    int32_t sort_every;     /* re-sort cadence in steps (0 -> adaptive only) */
+    double stray_frac;      /* re-sort when strays exceed this fraction (0 -> 0.02) */
+    int32_t n_modes;        /* x-displacement modes applied at load */
+    double mode_amp[32];    /* physical amplitude of mode m = 1..n_modes */
+    int32_t legacy_load;    /* 1: Particles::initialize(rp) path */
+    int32_t fixed_scale;    /* 0 -> chosen from the species densities */
+} aw_sim_config;
+
+aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
+                        int32_t nspecies, const aw_sim_config* cfg, aw_sim** out);
+aw_status aw_sim_destroy(aw_sim* sim);
+/* Simulation::init (:56-77): load, t=0 field, leapfrog half step back. */
+aw_status aw_sim_init(aw_sim* sim);
+/* Simulation::step (:80-93); records a DiagSample when rp.dump_every says so
+ * (dump_every <= 0: every step), as Diagnostics::maybe_compute (:111-120). */
+aw_status aw_sim_step(aw_sim* sim, int64_t n);
+/* Simulation::run (:95-98): steps 0..nsteps-1 then synchronises. */
+aw_status aw_sim_run(aw_sim* sim, int64_t nsteps);
+aw_status aw_sim_synchronize(aw_sim* sim);
+aw_status aw_sim_stream(aw_sim* sim, aw_stream* out);
+aw_status aw_sim_grid(aw_sim* sim, aw_grid* out);
+/* Current particle buffers (they swap on re-sort; query after each step). */
+aw_status aw_sim_particles(aw_sim* sim, aw_particles* out);
+aw_status aw_sim_fields(aw_sim* sim, aw_fields* out);
+aw_status aw_sim_sources(aw_sim* sim, aw_sources* out);
+/* Diagnostics::history (:128): copies up to cap samples, returns count. */
+aw_status aw_sim_history(aw_sim* sim, aw_diag_sample* out, int64_t cap, int64_t* count);
+/* Diagnostics::compute now (blocking). */
+aw_status aw_sim_diag_now(aw_sim* sim, int64_t step, aw_diag_sample* out);
+/* Forces a re-sort before the next step (tests; normally cadence/adaptive). */
+aw_status aw_sim_request_sort(aw_sim* sim);
+/* Event timing of the dominant particle kernel, recorded on the sim stream
+ * while enabled: total ms, launches, sorts performed. */
+aw_status aw_sim_set_profiling(aw_sim* sim, int32_t on);
+aw_status aw_sim_kernel_time(aw_sim* sim, double* total_ms, int64_t* launches,
+                             int64_t* sorts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AW_H */

@@ -1,0 +1,145 @@
+/*
+ * pic_oracle.h -- CPU restatement of ArcWarden's 2D3V PIC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry point and the bench's cpu_baseline leg load it.  The product path
+ * (libarcwarden_hip.so) never links, loads or calls it.
+ *
+ * Two flavours of every particle operation:
+ *   *_f64  -- the reference's mathematics evaluated in double precision from
+ *             the same FP32 inputs (the FP64 oracle for the 1e-6 gates);
+ *   *_f32  -- the exact FP32 operation sequence the HIP kernels execute
+ *             (explicit fmaf, no contraction), so integer outputs (cells,
+ *             tile keys, permutations) and particle state can be compared
+ *             bit for bit against a scalar host run.
+ *
+ * Conventions follow the reference: positions in cell units, row-major
+ * [ny][nx] fields with x fastest (include/pic/grid.hpp:14-29), R2C half
+ * spectrum [ny][nx/2+1] (include/pic/grid.hpp:20-27).
+ */
+#ifndef PIC_ORACLE_H
+#define PIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t nx, ny;
+    double  dx, dy;
+} orc_grid;
+
+/* uniform-B0 Boris parameters, precomputed in double and rounded once */
+typedef struct {
+    double qm, dt;
+    float  B0[3];
+    int32_t has_b0;
+} orc_push_params;
+
+/* ---- integer layer: cells, tiles, stable counting sort ------------------ */
+int32_t orc_cell_of(const orc_grid* g, float x, float y);
+void orc_cell_index(const orc_grid* g, const float* x, const float* y, int64_t n,
+                    int32_t* cell);
+void orc_tile_ids(const orc_grid* g, int32_t tx, int32_t ty, const float* x,
+                  const float* y, int64_t n, uint32_t* tile);
+void orc_cell_keys(const orc_grid* g, int32_t tx, int32_t ty, const float* x,
+                   const float* y, int64_t n, uint32_t* key);
+void orc_counting_sort(const uint32_t* key, int64_t n, uint32_t nkeys,
+                       int32_t* perm, int32_t* off);
+void orc_wrap_f32(const orc_grid* g, float* x, float* y, int64_t n);
+void orc_migrate_f32(const orc_grid* g, float* x, float* y, int32_t* cell, int64_t n);
+
+/* ---- deposit ------------------------------------------------------------ */
+void orc_deposit_rho_f64(const orc_grid* g, const float* x, const float* y,
+                         const float* w, double w_uniform, int64_t n, double qm,
+                         double* rho);
+void orc_deposit_rho_fix(const orc_grid* g, const float* x, const float* y,
+                         const float* w, double w_uniform, int64_t n, double qm,
+                         int32_t scale_log2, int64_t* acc);
+void orc_fix_to_float(const int64_t* acc, int64_t ncell, int32_t scale_log2, float* out);
+void orc_deposit_rho_j_f64(const orc_grid* g, const float* x, const float* y,
+                           const float* ux, const float* uy, const float* uz,
+                           const float* w, double w_uniform, int64_t n, double qm,
+                           double* rho, double* jx, double* jy, double* jz);
+void orc_deposit_amu_f64(const orc_grid* g, const float* x, const float* y,
+                         const float* ux, const float* uy, const float* uz,
+                         const float* w, double w_uniform, int64_t n, double qm,
+                         double* amu0, double* amu1, double* amu2, double* amu3);
+void orc_deposit_dcu_f64(const orc_grid* g, const float* x, const float* y,
+                         const float* ux, const float* uy, const float* uz,
+                         const float* w, double w_uniform, int64_t n, double qm,
+                         const float* Ex, const float* Ey, const float* Ez,
+                         const float* Bx, const float* By, const float* Bz,
+                         double* dcx, double* dcy, double* dcz);
+
+/* ---- push --------------------------------------------------------------- */
+void orc_push_es_f32(const orc_grid* g, const orc_push_params* pp, float* x, float* y,
+                     float* ux, float* uy, float* uz, int64_t n, const float* Ex,
+                     const float* Ey, int32_t wrap);
+void orc_push_es_f64(const orc_grid* g, const orc_push_params* pp, double* x, double* y,
+                     double* ux, double* uy, double* uz, int64_t n, const double* Ex,
+                     const double* Ey, int32_t wrap);
+void orc_half_back_f32(const orc_grid* g, const orc_push_params* pp, const float* x,
+                       const float* y, float* ux, float* uy, float* uz, int64_t n,
+                       const float* Ex, const float* Ey);
+void orc_push_em_f64(const orc_grid* g, double qm, double dt, double* x, double* y,
+                     double* ux, double* uy, double* uz, int64_t n,
+                     const double* Ex, const double* Ey, const double* Ez,
+                     const double* Bx, const double* By, const double* Bz, int32_t wrap);
+
+/* ---- spectral (double) --------------------------------------------------- */
+void orc_fft2_r2c(int32_t nx, int32_t ny, const double* in, double* out_re_im);
+void orc_fft2_c2r(int32_t nx, int32_t ny, const double* in_re_im, double* out);
+void orc_dft2_r2c_brute(int32_t nx, int32_t ny, const double* in, double* out_re_im);
+void orc_solve_es_f64(const orc_grid* g, double eps0, const double* rho, double* Ex,
+                      double* Ey);
+void orc_solve_darwin_b_f64(const orc_grid* g, double eps0, double c, const double* jx,
+                            const double* jy, const double* jz, double* bx, double* by,
+                            double* bz);
+void orc_solve_darwin_et_f64(const orc_grid* g, double eps0, double c, double n0,
+                             const double* dcx, const double* dcy, const double* dcz,
+                             const double* a0, const double* a1, const double* a2,
+                             const double* a3, double* etx, double* ety, double* etz);
+
+/* ---- Esirkepov (charge-conserving current) ------------------------------ */
+void orc_esirkepov_f64(const orc_grid* g, const float* x0, const float* y0,
+                       const float* x1, const float* y1, const float* uz,
+                       const float* w, double w_uniform, int64_t n, double qm, double dt,
+                       double* jx, double* jy, double* jz);
+void orc_rho_nodes_f64(const orc_grid* g, const float* x, const float* y, const float* w,
+                       double w_uniform, int64_t n, double qm, double* rho);
+
+/* ---- loaders -------------------------------------------------------------- */
+double orc_radical_inverse(uint32_t i, uint32_t base);
+uint32_t orc_hash_u32(uint32_t v);
+double orc_rng_uniform(int64_t t, uint32_t stream, uint64_t seed);
+double orc_erfinv(double y);
+float orc_quiet_u(int64_t t, uint32_t base, double uth, double ufl);
+typedef struct {
+    int32_t ppc, index;
+    int64_t base, count;
+    double  weight, uth[3], ufl[3];
+    int32_t noisy;
+    uint64_t seed;
+} orc_species_init;
+void orc_load_species(const orc_grid* g, const orc_species_init* sp, float* x, float* y,
+                      float* ux, float* uy, float* uz);
+
+/* ---- diagnostics (double) -------------------------------------------------- */
+void orc_energies(const orc_grid* g, const float* ux, const float* uy, const float* uz,
+                  const float* w, double w_uniform, int64_t n, const float* Ex,
+                  const float* Ey, const float* rho, double eps0, double* out5);
+
+/* ---- whole ES step, same-algorithm FP32 path (CPU baseline) --------------- */
+int32_t orc_es_step_f32(const orc_grid* g, const orc_push_params* pp, double eps0,
+                        float* x, float* y, float* ux, float* uy, float* uz, int64_t n,
+                        double w_uniform, int32_t scale_log2, float* rho, float* Ex,
+                        float* Ey, int32_t nthreads);
+int32_t orc_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

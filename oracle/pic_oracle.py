@@ -1,0 +1,250 @@
+"""numpy binding of liboracle_pic.so -- the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
+bench's cpu_baseline leg, as the checker.  The product library never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_pic.so"
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("dy", C.c_double)]
+
+
+class OrcPush(C.Structure):
+    _fields_ = [("qm", C.c_double), ("dt", C.c_double), ("B0", C.c_float * 3),
+                ("has_b0", C.c_int32)]
+
+
+class OrcSpecies(C.Structure):
+    _fields_ = [("ppc", C.c_int32), ("index", C.c_int32), ("base", C.c_int64),
+                ("count", C.c_int64), ("weight", C.c_double), ("uth", C.c_double * 3),
+                ("ufl", C.c_double * 3), ("noisy", C.c_int32), ("seed", C.c_uint64)]
+
+
+_L = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        if not LIB.exists():
+            build()
+        _L = C.CDLL(str(LIB))
+    return _L
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def grid(nx, ny, dx, dy) -> OrcGrid:
+    return OrcGrid(nx, ny, dx, dy)
+
+
+def push_params(qm, dt, B0=(0.0, 0.0, 0.0), has_b0=True) -> OrcPush:
+    pp = OrcPush(qm, dt)
+    for i in range(3):
+        pp.B0[i] = B0[i]
+    pp.has_b0 = int(has_b0)
+    return pp
+
+
+# ---- integer layer -------------------------------------------------------------
+
+def cell_index(g, x, y):
+    x, y = _f(x), _f(y)
+    out = np.empty(x.size, np.int32)
+    lib().orc_cell_index(C.byref(g), _p(x), _p(y), C.c_int64(x.size), _p(out))
+    return out
+
+
+def tile_ids(g, tx, ty, x, y):
+    x, y = _f(x), _f(y)
+    out = np.empty(x.size, np.uint32)
+    lib().orc_tile_ids(C.byref(g), tx, ty, _p(x), _p(y), C.c_int64(x.size), _p(out))
+    return out
+
+
+def cell_keys(g, tx, ty, x, y):
+    x, y = _f(x), _f(y)
+    out = np.empty(x.size, np.uint32)
+    lib().orc_cell_keys(C.byref(g), tx, ty, _p(x), _p(y), C.c_int64(x.size), _p(out))
+    return out
+
+
+def counting_sort(keys, nkeys):
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    perm = np.empty(k.size, np.int32)
+    off = np.empty(nkeys + 1, np.int32)
+    lib().orc_counting_sort(_p(k), C.c_int64(k.size), C.c_uint32(nkeys), _p(perm), _p(off))
+    return perm, off
+
+
+def migrate(g, x, y):
+    x, y = _f(x).copy(), _f(y).copy()
+    cell = np.empty(x.size, np.int32)
+    lib().orc_migrate_f32(C.byref(g), _p(x), _p(y), _p(cell), C.c_int64(x.size))
+    return x, y, cell
+
+
+# ---- deposits ------------------------------------------------------------------
+
+def deposit_rho_f64(g, x, y, qm, w=None, w_uniform=0.0):
+    x, y = _f(x), _f(y)
+    w = _f(w) if w is not None else None
+    rho = np.zeros(g.nx * g.ny, np.float64)
+    lib().orc_deposit_rho_f64(C.byref(g), _p(x), _p(y), _p(w), C.c_double(w_uniform),
+                              C.c_int64(x.size), C.c_double(qm), _p(rho))
+    return rho.reshape(g.ny, g.nx)
+
+
+def deposit_rho_fix(g, x, y, qm, scale_log2, w=None, w_uniform=0.0):
+    x, y = _f(x), _f(y)
+    w = _f(w) if w is not None else None
+    acc = np.zeros(g.nx * g.ny, np.int64)
+    lib().orc_deposit_rho_fix(C.byref(g), _p(x), _p(y), _p(w), C.c_double(w_uniform),
+                              C.c_int64(x.size), C.c_double(qm), C.c_int32(scale_log2), _p(acc))
+    out = np.empty(g.nx * g.ny, np.float32)
+    lib().orc_fix_to_float(_p(acc), C.c_int64(acc.size), C.c_int32(scale_log2), _p(out))
+    return out.reshape(g.ny, g.nx), acc.reshape(g.ny, g.nx)
+
+
+# ---- push ------------------------------------------------------------------------
+
+def push_es_f32(g, pp, state, Ex, Ey, wrap):
+    s = {k: _f(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    Ex, Ey = _f(Ex).ravel(), _f(Ey).ravel()
+    lib().orc_push_es_f32(C.byref(g), C.byref(pp), *(_p(s[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                          C.c_int64(s["x"].size), _p(Ex), _p(Ey), C.c_int32(int(wrap)))
+    return s
+
+
+def push_es_f64(g, pp, state, Ex, Ey, wrap):
+    s = {k: _d(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    Ex, Ey = _d(Ex).ravel(), _d(Ey).ravel()
+    lib().orc_push_es_f64(C.byref(g), C.byref(pp), *(_p(s[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                          C.c_int64(s["x"].size), _p(Ex), _p(Ey), C.c_int32(int(wrap)))
+    return s
+
+
+def half_back_f32(g, pp, state, Ex, Ey):
+    s = {k: _f(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    Ex, Ey = _f(Ex).ravel(), _f(Ey).ravel()
+    lib().orc_half_back_f32(C.byref(g), C.byref(pp), _p(s["x"]), _p(s["y"]), _p(s["ux"]),
+                            _p(s["uy"]), _p(s["uz"]), C.c_int64(s["x"].size), _p(Ex), _p(Ey))
+    return s
+
+
+# ---- spectral ----------------------------------------------------------------------
+
+def fft2_r2c(field):
+    f = _d(field)
+    ny, nx = f.shape
+    out = np.empty((ny, nx // 2 + 1, 2), np.float64)
+    lib().orc_fft2_r2c(nx, ny, _p(f), _p(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def fft2_c2r(spec, nx):
+    ny = spec.shape[0]
+    s = np.ascontiguousarray(np.stack([spec.real, spec.imag], axis=-1), dtype=np.float64)
+    out = np.empty((ny, nx), np.float64)
+    lib().orc_fft2_c2r(nx, ny, _p(s), _p(out))
+    return out
+
+
+def dft2_brute(field):
+    f = _d(field)
+    ny, nx = f.shape
+    out = np.empty((ny, nx // 2 + 1, 2), np.float64)
+    lib().orc_dft2_r2c_brute(nx, ny, _p(f), _p(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def solve_es_f64(g, rho, eps0=1.0):
+    r = _d(rho)
+    ex = np.empty((g.ny, g.nx), np.float64)
+    ey = np.empty((g.ny, g.nx), np.float64)
+    lib().orc_solve_es_f64(C.byref(g), C.c_double(eps0), _p(r), _p(ex), _p(ey))
+    return ex, ey
+
+
+# ---- loaders / diagnostics ---------------------------------------------------------
+
+def erfinv(y):
+    lib().orc_erfinv.restype = C.c_double
+    return lib().orc_erfinv(C.c_double(y))
+
+
+def quiet_u(t, base, uth=1.0, ufl=0.0):
+    lib().orc_quiet_u.restype = C.c_float
+    lib().orc_quiet_u.argtypes = [C.c_int64, C.c_uint32, C.c_double, C.c_double]
+    return lib().orc_quiet_u(t, base, uth, ufl)
+
+
+def load_species(g, specs, noisy, seed):
+    """specs: list of (density, ppc, uth3, ufl3).  Same draw + rank-major storage as
+    the library's loader."""
+    nc = g.nx * g.ny
+    n = sum(p * nc for _, p, _, _ in specs)
+    arrs = {k: np.empty(n, np.float32) for k in ("x", "y", "ux", "uy", "uz")}
+    base, weights = 0, []
+    for idx, (dens, ppc, uth, ufl) in enumerate(specs):
+        si = OrcSpecies(ppc, idx, base, ppc * nc, dens * g.dx * g.dy / ppc)
+        for i in range(3):
+            si.uth[i], si.ufl[i] = uth[i], ufl[i]
+        si.noisy, si.seed = int(noisy), seed
+        lib().orc_load_species(C.byref(g), C.byref(si), *(_p(arrs[k]) for k in
+                                                           ("x", "y", "ux", "uy", "uz")))
+        weights.append(si.weight)
+        base += ppc * nc
+    return arrs, weights
+
+
+def energies(g, state, Ex, Ey, rho, w_uniform, eps0=1.0, w=None):
+    out = np.empty(5, np.float64)
+    ux, uy, uz = _f(state["ux"]), _f(state["uy"]), _f(state["uz"])
+    w = _f(w) if w is not None else None
+    lib().orc_energies(C.byref(g), _p(ux), _p(uy), _p(uz), _p(w), C.c_double(w_uniform),
+                       C.c_int64(ux.size), _p(_f(Ex).ravel()), _p(_f(Ey).ravel()),
+                       _p(_f(rho).ravel()) if rho is not None else None, C.c_double(eps0),
+                       _p(out))
+    return dict(zip(("ke", "ee", "total", "charge", "max_e"), out))
+
+
+def es_step_f32(g, pp, state, w_uniform, scale_log2, rho, eps0=1.0, nthreads=1):
+    """One FP32 same-algorithm ES step in place on `state` (dict of float32 arrays)
+    and `rho` (float32 [ny,nx]); returns (Ex, Ey)."""
+    ex = np.empty((g.ny, g.nx), np.float32)
+    ey = np.empty((g.ny, g.nx), np.float32)
+    lib().orc_es_step_f32(C.byref(g), C.byref(pp), C.c_double(eps0),
+                          *(_p(state[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                          C.c_int64(state["x"].size), C.c_double(w_uniform),
+                          C.c_int32(scale_log2), _p(rho), _p(ex), _p(ey), C.c_int32(nthreads))
+    return ex, ey
+
+
+def max_threads() -> int:
+    return lib().orc_max_threads()

@@ -1,0 +1,186 @@
+"""GPU Simulation tests: one step against the FP64 oracle, diagnostics, the
+reference's physics gates (tighter tolerances), and size-independent
+invariants at the 512^2 x 256 ppc workload."""
+import math
+
+import numpy as np
+import pytest
+
+from arcwarden_amd import _lib as L
+from arcwarden_amd.sim import SimConfig, Simulation
+
+import physics
+from gpu_util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * math.pi
+
+
+def periodic_rel_l2(a, b, n):
+    d = (np.asarray(a, np.float64) - np.asarray(b, np.float64) + 0.5 * n) % n - 0.5 * n
+    return np.linalg.norm(d) / np.linalg.norm(np.asarray(b, np.float64))
+
+
+def config1_sim(sort_every=20, stray_frac=0.0, **kw):
+    """Plain ES two-stream: 64x64, dx=0.5, beams +-0.2, vth=0.01, 64 ppc,
+    x-displacement 1e-3 sin(k1 x) (physical units), dt=0.1."""
+    g = L.grid(64, 64, 32.0, 32.0)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260627,
+                      dump_every=1, nsteps=200)
+    sp = [L.species("beam_plus", 0.5, 32, (0.01, 0.01, 0.01), (0.2, 0, 0)),
+          L.species("beam_minus", 0.5, 32, (0.01, 0.01, 0.01), (-0.2, 0, 0))]
+    cfg = SimConfig(has_b0=True, mode_amp=[1e-3], sort_every=sort_every, stray_frac=stray_frac,
+                    **kw)
+    return Simulation(g, rp, sp, cfg)
+
+
+def test_one_step_matches_fp64_oracle(gpu, oracle):
+    sim = config1_sim(sort_every=0, stray_frac=1.0)
+    sim.init()
+    s0 = sim.particles()
+    w = s0["w_uniform"]
+    sim.step(0)
+    s1 = sim.particles()
+    rho0 = sim.rho()
+    og = oracle.grid(64, 64, 0.5, 0.5)
+    rho64 = oracle.deposit_rho_f64(og, s0["x"], s0["y"], -1.0, w_uniform=w)
+    assert rel_l2(rho0, rho64) < 1e-6
+    ex64, ey64 = oracle.solve_es_f64(og, rho64)
+    f = sim.fields()
+    assert rel_l2(f["Ex"], ex64) < 1e-5 and rel_l2(f["Ey"], ey64) < 1e-5
+    pp = oracle.push_params(-1.0, 0.1, (0, 0, 0), True)
+    ref = oracle.push_es_f64(og, pp, s0, ex64, ey64, True)
+    assert periodic_rel_l2(s1["x"], ref["x"], 64) < 1e-6
+    assert periodic_rel_l2(s1["y"], ref["y"], 64) < 1e-6
+    for k in ("ux", "uy", "uz"):
+        assert rel_l2(s1[k], ref[k]) < 1e-6, k
+    # rho(x^1) feeds step 1
+    sim.step(1)
+    rho1 = sim.rho()
+    rho1_64 = oracle.deposit_rho_f64(og, s1["x"], s1["y"], -1.0, w_uniform=w)
+    assert rel_l2(rho1, rho1_64) < 1e-6
+    sim.close()
+
+
+def test_fused_step_is_bit_exact_against_same_algorithm_host_run(gpu, oracle):
+    """Given the GPU's field, the host's FP32 step reproduces state and rho bit for bit."""
+    sim = config1_sim(sort_every=0, stray_frac=1.0)
+    sim.init()
+    s0 = sim.particles()
+    sim.step(0)
+    f = sim.fields()
+    s1 = sim.particles()
+    og = oracle.grid(64, 64, 0.5, 0.5)
+    pp = oracle.push_params(-1.0, 0.1, (0, 0, 0), True)
+    ref = oracle.push_es_f32(og, pp, s0, f["Ex"], f["Ey"], True)
+    for k in ("x", "y", "ux", "uy", "uz"):
+        assert np.array_equal(s1[k], ref[k]), k
+    sim.step(1)
+    S = 40  # default for unit density
+    ref_rho, _ = oracle.deposit_rho_fix(og, ref["x"], ref["y"], -1.0, S, w_uniform=s0["w_uniform"])
+    assert np.array_equal(sim.rho(), ref_rho)
+    sim.close()
+
+
+def test_diagnostics_match_oracle(gpu, oracle):
+    sim = config1_sim()
+    sim.init()
+    for n in range(5):
+        sim.step(n)
+    h = sim.history()
+    assert [d["step"] for d in h] == list(range(5))
+    s = sim.particles()
+    f = sim.fields()
+    e = oracle.energies(oracle.grid(64, 64, 0.5, 0.5), s, f["Ex"], f["Ey"], sim.rho(),
+                        s["w_uniform"])
+    last = h[-1]
+    assert last["ke"] == pytest.approx(e["ke"], rel=1e-12)
+    assert last["ee"] == pytest.approx(e["ee"], rel=1e-12)
+    assert last["charge"] == pytest.approx(e["charge"], rel=1e-12, abs=1e-9)
+    assert last["max_e"] == pytest.approx(e["max_e"], rel=1e-12)
+    # total charge = qm * n0 * area, constant
+    for d in h:
+        assert d["charge"] == pytest.approx(-1.0 * 32.0 * 32.0, rel=1e-6)
+    sim.close()
+
+
+def test_resort_keeps_results_identical(gpu):
+    """Re-sorting only permutes particles: rho and the diagnostics do not change."""
+    a = config1_sim(sort_every=0, stray_frac=1.0)
+    b = config1_sim(sort_every=1)
+    a.init()
+    b.init()
+    for n in range(6):
+        a.step(n)
+        b.step(n)
+    assert np.array_equal(a.rho(), b.rho())
+    ha, hb = a.history(), b.history()
+    for x, y in zip(ha, hb):
+        assert x["ee"] == y["ee"] and x["charge"] == y["charge"]
+        assert x["ke"] == pytest.approx(y["ke"], rel=1e-12)
+    assert b.kernel_time()[2] >= 6
+    a.close()
+    b.close()
+
+
+def test_langmuir_frequency_within_1pct(gpu):
+    """tests/test_langmuir.cu setup; gate tightened from 5% to 1%."""
+    g = L.grid(64, 8, TWO_PI, TWO_PI)
+    rp = L.run_params(n0=1.0, qm=-1.0, eps0=1.0, vth=0.0, ppc=64, dt=0.05, nsteps=600,
+                      dump_every=1, perturb_amp=0.3, perturb_kx=1)
+    rp.weight = rp.n0 * g.dx * g.dy / rp.ppc
+    sim = Simulation(g, rp, None, SimConfig(legacy_load=True))
+    sim.init()
+    sim.run(600)
+    h = sim.history()
+    t = np.array([d["time"] for d in h])
+    ee = np.array([d["ee"] for d in h])
+    ke = np.array([d["ke"] for d in h])
+    tot = np.array([d["total"] for d in h])
+    om = physics.langmuir_omega_from_ee(t, ee)
+    assert abs(om - physics.langmuir_leapfrog_omega(0.05)) < 0.01
+    assert np.corrcoef(ke, ee)[0, 1] < -0.7
+    assert (tot.max() - tot.min()) / tot.mean() < 0.2
+    sim.close()
+
+
+def test_two_stream_growth_within_5pct(gpu):
+    """tests/test_two_stream.cu setup; gamma pinned to the cold root 1/(2 sqrt 2)."""
+    kv = math.sqrt(3.0 / 8.0)
+    Lx = TWO_PI / kv
+    g = L.grid(64, 4, Lx, 1.0)
+    rp = L.run_params(n0=1.0, qm=-1.0, eps0=1.0, vth=0.0, vd=1.0, two_stream=1, ppc=64,
+                      dt=0.05, nsteps=600, dump_every=1, perturb_amp=0.005, perturb_kx=1)
+    rp.weight = rp.n0 * g.dx * g.dy / rp.ppc
+    sim = Simulation(g, rp, None, SimConfig(legacy_load=True))
+    sim.init()
+    sim.run(600)
+    h = sim.history()
+    gamma = physics.growth_rate(np.array([d["time"] for d in h]), np.array([d["ee"] for d in h]))
+    theory = physics.two_stream_growth(TWO_PI / Lx, 1.0)
+    assert abs(gamma - theory) / theory < 0.05, (gamma, theory)
+    s = sim.particles()
+    assert np.mean(np.abs(s["ux"]) < 0.5) > 0.10
+    sim.close()
+
+
+def test_config2_invariants(gpu):
+    """512^2, 256 ppc (67,108,864 electrons), B0 = 0.5 z: charge conservation,
+    kinetic energy of the Maxwellian, no strays right after the load sort."""
+    g = L.grid(512, 512, 256.0, 256.0)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260628,
+                      B0=(0.0, 0.0, 0.5), dump_every=1)
+    sp = [L.species("electrons", 1.0, 256, (0.05, 0.05, 0.05), (0, 0, 0))]
+    sim = Simulation(g, rp, sp, SimConfig(has_b0=True, sort_every=20))
+    sim.init()
+    for n in range(5):
+        sim.step(n)
+    h = sim.history()
+    for d in h:
+        assert d["charge"] == pytest.approx(-65536.0, rel=1e-6)
+        assert math.isfinite(d["ee"]) and d["ee"] > 0
+    ke_expect = 0.5 * 65536.0 * 3 * 0.05 ** 2
+    assert h[-1]["ke"] == pytest.approx(ke_expect, rel=0.01)
+    assert abs(h[-1]["total"] - h[0]["total"]) / h[0]["total"] < 1e-3
+    sim.close()
