@@ -411,7 +411,8 @@ aw_status aw_sort_by_tile(aw_ctx* ctx, const aw_particles* src, aw_particles* ds
         check_particles(dst);
         AW_REQUIRE(src->n == dst->n, "src and dst particle counts differ");
         AW_REQUIRE((src->w == nullptr) == (dst->w == nullptr), "w present in only one of src/dst");
-        AW_REQUIRE(src->x != dst->x, "aw_sort_by_tile needs distinct src and dst buffers");
+        AW_REQUIRE(src->n == 0 || src->x != dst->x,
+                   "aw_sort_by_tile needs distinct src and dst buffers");
         ctx->sort.bin_perm.reserve((size_t)std::max<int64_t>(1, src->n));
         sort_tiles(*ctx, *src, *dst, ctx->sort.bin_perm.get(), off, S(s));
     });

@@ -81,7 +81,18 @@ def main():
     ap.add_argument("--sort-every", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy (torch clone) so that a "
+                         "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a "
+                         "kernel of known traffic")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        for _ in range(3):
+            dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the ensemble and particle-decomposition sharding is described in DESIGN.md)",
