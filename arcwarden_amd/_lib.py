@@ -95,7 +95,8 @@ class AwSimConfig(C.Structure):
                 ("field_model", C.c_int32), ("tile_x", C.c_int32), ("tile_y", C.c_int32),
                 ("sort_every", C.c_int32), ("stray_frac", C.c_double), ("n_modes", C.c_int32),
                 ("mode_amp", C.c_double * 32), ("legacy_load", C.c_int32),
-                ("fixed_scale", C.c_int32)]
+                ("fixed_scale", C.c_int32), ("blocks_per_tile", C.c_int32),
+                ("autotune", C.c_int32)]
 
 
 P = C.c_void_p

@@ -207,10 +207,13 @@ void stable_sort_keys(aw_ctx& c, int64_t n, uint32_t nkeys, int32_t* perm, int32
                       hipStream_t s);
 void permute_particles(const aw_particles& src, const aw_particles& dst, const int32_t* perm,
                        hipStream_t s);
+bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* off,
+                       int32_t* perm, hipStream_t s);
 
 // Deposit helpers (deposit.hip)
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
-                      bool zero_acc, hipStream_t s);
+                      bool zero_acc, hipStream_t s, bool assign = false,
+                      unsigned long long* counter = nullptr);
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
                        const int32_t* sorted_off, hipStream_t s);
 

@@ -59,13 +59,14 @@ void check_particles(const aw_particles* p) {
         AW_REQUIRE(p->x && p->y && p->ux && p->uy && p->uz, "particle arrays must be non-NULL");
 }
 
-void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s) {
-    if (assign) AW_HIP(hipMemsetAsync(out, 0, (size_t)ncell(c) * sizeof(float), s));
-    fix_to_float_add(c.acc.get(), out, ncell(c), c.fix_log2, true, s);
+void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s,
+                  unsigned long long* counter = nullptr) {
+    fix_to_float_add(c.acc.get(), out, ncell(c), c.fix_log2, true, s, assign, counter);
 }
 
 void sort_tiles(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* perm,
                 int32_t* off, hipStream_t s) {
+    if (tile_sort_onepass(c, src, dst, off, nullptr, s)) return;
     c.sort.keyA.reserve((size_t)std::max<int64_t>(1, src.n));
     compute_tile_keys(c, src, c.sort.keyA.get(), false, s);
     stable_sort_keys(c, src.n, (uint32_t)c.ntiles, perm, off, s);
@@ -198,6 +199,7 @@ struct aw_sim {
         fix_to_float(*ctx, rho.get(), true, s);
         solve_es_contig(*ctx, rho.get(), E.get(), rp.eps0, s);
         push_es(*ctx, particles(), Ex(), Ey(), rp, cfg.has_b0 != 0, true, s);
+        strays.zero(s);  // afterwards cleared by each step's rho conversion
         fields_ready = true;
         initialized = true;
         AW_HIP(hipStreamSynchronize(s));
@@ -208,13 +210,12 @@ struct aw_sim {
         const hipStream_t s = stream;
         poll_strays();
         if (!fields_ready) {
-            fix_to_float(*ctx, rho.get(), true, s);
+            fix_to_float(*ctx, rho.get(), true, s, strays.get());
             solve_es_contig(*ctx, rho.get(), E.get(), rp.eps0, s);
         }
         fields_ready = false;
         if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) do_sort();
         const bool dump = rp.dump_every <= 0 || (nstep % rp.dump_every) == 0;
-        AW_HIP(hipMemsetAsync(strays.get(), 0, sizeof(unsigned long long), s));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
             AW_HIP(hipEventCreate(&e0));
@@ -598,7 +599,7 @@ aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         sim->rho.resize((size_t)nc);
         sim->E.resize((size_t)2 * nc);
         sim->strays.resize(1);
-        sim->bpt = tile_bpt(*sim->ctx, sim->n);
+        sim->bpt = cfg->blocks_per_tile > 0 ? cfg->blocks_per_tile : tile_bpt(*sim->ctx, sim->n);
         sim->ke_part.resize((size_t)sim->ctx->ntiles * sim->bpt + 1);
         sim->grid_part.resize((size_t)3 * 512 + 3);
         sim->diag_out.resize((size_t)4 * sim->diag_cap);

@@ -117,23 +117,28 @@ deposit_tiles_kernel(DepArgs a, TileGeom t, unsigned long long* __restrict__ acc
     }
 }
 
+// out = (assign ? 0 : out) + acc * 2^-S; optionally clears acc for the next
+// accumulation and one 64-bit counter (saves separate memset launches).
 __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
-                                    int64_t n, double inv, int zero_acc) {
+                                    int64_t n, double inv, int zero_acc, int assign,
+                                    unsigned long long* __restrict__ counter) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0 && counter) *counter = 0ull;
     if (c >= n) return;
     const long long v = (long long)acc[c];
-    out[c] += (float)((double)v * inv);
+    const float add = (float)((double)v * inv);
+    out[c] = assign ? add : out[c] + add;
     if (zero_acc) acc[c] = 0ull;
 }
 
 }  // namespace
 
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
-                      bool zero_acc, hipStream_t s) {
+                      bool zero_acc, hipStream_t s, bool assign, unsigned long long* counter) {
     if (n == 0) return;
     fix_to_float_kernel<<<blocks_for(n), kThreads, 0, s>>>(
         const_cast<unsigned long long*>(acc), out, n, std::ldexp(1.0, -scale_log2),
-        zero_acc ? 1 : 0);
+        zero_acc ? 1 : 0, assign ? 1 : 0, counter);
     AW_LAUNCH_CHECK();
 }
 

@@ -186,6 +186,122 @@ __global__ void key_offsets_kernel(const uint32_t* __restrict__ skeys, int n, ui
     for (int64_t t = prev + 1; t <= cur; ++t) off[t] = i;
 }
 
+// ---- one-pass stable tile sort that moves the SoA ------------------------------------
+//
+// For particles that are already nearly tile-sorted (the previous sort plus a
+// few steps of drift) a single stable counting sort by tile id beats the LSD
+// radix + permute path: one histogram pass over (x, y), one scan of the
+// chunk x tile count table, and one scatter that ranks each particle inside
+// its chunk with wavefront ballots and writes it straight to its final slot.
+
+constexpr int kTsRounds = 128;                   // 256 x 128 = 32768 particles / chunk
+constexpr int kTsChunk = kThreads * kTsRounds;
+
+__device__ __forceinline__ int tile_id(float x, float y, int nx, int ny, int tx, int ty,
+                                       int ntx) {
+    const int c = dev::cell_of(x, y, nx, ny);
+    return ((c / nx) / ty) * ntx + (c % nx) / tx;
+}
+
+__global__ void __launch_bounds__(kThreads)
+ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, int nx, int ny,
+               int tx, int ty, int ntx, int ntiles, int nchunks, uint32_t* __restrict__ table) {
+    extern __shared__ uint32_t h[];
+    for (int t = threadIdx.x; t < ntiles; t += kThreads) h[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTsChunk, end = min(n, base + kTsChunk);
+    const int lane = threadIdx.x & 63;
+    for (int i0 = base; i0 < end; i0 += kThreads) {
+        const int i = i0 + (int)threadIdx.x;
+        const bool valid = i < end;
+        const int t = valid ? tile_id(x[i], y[i], nx, ny, tx, ty, ntx) : -1;
+        // aggregate equal tiles across the wave: one LDS atomic per distinct tile
+        unsigned long long todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lt = __shfl(t, leader, 64);
+            const unsigned long long m = __ballot(valid && t == lt) & todo;
+            if (lane == leader) atomicAdd(&h[lt], (uint32_t)__popcll(m));
+            todo &= ~m;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += kThreads)
+        table[(size_t)t * nchunks + blockIdx.x] = h[t];
+}
+
+struct TsMove {
+    const float *x, *y, *ux, *uy, *uz, *w;
+    float *xo, *yo, *uxo, *uyo, *uzo, *wo;
+    int32_t* perm;  // optional: perm[pos] = source index
+};
+
+__global__ void __launch_bounds__(kThreads)
+ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int ntiles,
+                  int nchunks, int tbits, const uint32_t* __restrict__ table) {
+    extern __shared__ uint32_t sh[];
+    uint32_t* base = sh;                         // ntiles running offsets
+    uint32_t* wst = sh + ntiles;                 // [4][ntiles] (round << 16 | count)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < ntiles; t += kThreads) {
+        base[t] = table[(size_t)t * nchunks + blockIdx.x];
+        for (int w = 0; w < kThreads / 64; ++w) wst[w * ntiles + t] = 0xffffffffu;
+    }
+    __syncthreads();
+    const int cbase = blockIdx.x * kTsChunk;
+    const int rounds = min(kTsRounds, (n - cbase + kThreads - 1) / kThreads);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < rounds; ++r) {
+        const int i = cbase + r * kThreads + tid;
+        const bool valid = i < n;
+        float x = 0.f, y = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, w = 0.f;
+        int t = 0;
+        if (valid) {
+            x = m.x[i];
+            y = m.y[i];
+            ux = m.ux[i];
+            uy = m.uy[i];
+            uz = m.uz[i];
+            if (m.w) w = m.w[i];
+            t = tile_id(x, y, nx, ny, tx, ty, ntx);
+        }
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < tbits; ++b) {
+            const bool bit = (t >> b) & 1;
+            const unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        if (valid && rank == 0) wst[wave * ntiles + t] = ((uint32_t)r << 16) | cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[t] + rank;
+            for (int w2 = 0; w2 < wave; ++w2) {
+                const uint32_t s = wst[w2 * ntiles + t];
+                if ((s >> 16) == (uint32_t)r) pos += s & 0xffffu;
+            }
+            m.xo[pos] = x;
+            m.yo[pos] = y;
+            m.uxo[pos] = ux;
+            m.uyo[pos] = uy;
+            m.uzo[pos] = uz;
+            if (m.wo) m.wo[pos] = w;
+            if (m.perm) m.perm[pos] = i;
+        }
+        __syncthreads();
+        if (valid && rank == 0) atomicAdd(&base[t], cnt);
+        __syncthreads();
+    }
+}
+
+__global__ void ts_offsets_kernel(const uint32_t* __restrict__ table, int nchunks, int ntiles,
+                                  int n, int32_t* __restrict__ off) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) off[t] = (int32_t)table[(size_t)t * nchunks];
+    if (t == ntiles) off[t] = n;
+}
+
 // ---- SoA permutation ---------------------------------------------------------------
 
 __global__ void permute_kernel(const float* __restrict__ x, const float* __restrict__ y,
@@ -412,6 +528,37 @@ void stable_sort_keys(aw_ctx& c, int64_t n, uint32_t nkeys, int32_t* perm, int32
         key_offsets_kernel<<<blocks_for((int64_t)nn + 1), kThreads, 0, s>>>(kin, nn, nkeys, off);
         AW_LAUNCH_CHECK();
     }
+}
+
+// Stable tile sort src -> dst in one pass (see ts_* kernels).  Returns false
+// when the per-tile LDS tables do not fit (caller uses the radix path).
+bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* off,
+                       int32_t* perm, hipStream_t s) {
+    const size_t lds = (size_t)c.ntiles * 4 * (1 + kThreads / 64);
+    if (lds > c.lds_per_block) return false;
+    AW_REQUIRE(src.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    const int n = (int)src.n;
+    if (n == 0) {
+        AW_HIP(hipMemsetAsync(off, 0, ((size_t)c.ntiles + 1) * sizeof(int32_t), s));
+        return true;
+    }
+    const int nchunks = (n + kTsChunk - 1) / kTsChunk;
+    int tbits = 0;
+    while ((1 << tbits) < c.ntiles) ++tbits;
+    c.sort.table.reserve((size_t)c.ntiles * nchunks);
+    ts_hist_kernel<<<nchunks, kThreads, (size_t)c.ntiles * 4, s>>>(
+        src.x, src.y, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx, c.ntiles, nchunks, c.sort.table.get());
+    AW_LAUNCH_CHECK();
+    exclusive_scan(c, c.sort.table.get(), (int64_t)c.ntiles * nchunks, s);
+    ts_offsets_kernel<<<blocks_for((int64_t)c.ntiles + 1), kThreads, 0, s>>>(
+        c.sort.table.get(), nchunks, c.ntiles, n, off);
+    AW_LAUNCH_CHECK();
+    TsMove m{src.x, src.y, src.ux, src.uy, src.uz, src.w,
+             dst.x, dst.y, dst.ux, dst.uy, dst.uz, dst.w, perm};
+    ts_scatter_kernel<<<nchunks, kThreads, lds, s>>>(m, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx,
+                                                     c.ntiles, nchunks, tbits, c.sort.table.get());
+    AW_LAUNCH_CHECK();
+    return true;
 }
 
 void permute_particles(const aw_particles& src, const aw_particles& dst, const int32_t* perm,

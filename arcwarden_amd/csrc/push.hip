@@ -83,93 +83,121 @@ __device__ __forceinline__ int local_index(int i, int org, int n) {
     return l;
 }
 
+struct TileFrame {
+    unsigned long long* s_rho;
+    const float2* s_e;  // (Ex, Ey) interleaved
+    int W, H, orgx, orgy;
+};
+
+// Everything after the loads for one particle: gather E^n, Boris, move,
+// wrap, store, deposit rho(x^{n+1}).
+template <bool PER_W, bool KE>
+__device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f, int k, float x,
+                                         float y, float ux, float uy, float uz, float wk,
+                                         unsigned int& nstray, double& ke) {
+    const dev::CicF s = dev::cic_f(x, y);
+    const int li = local_index(s.i0, f.orgx, a.nx), lj = local_index(s.j0, f.orgy, a.ny);
+    float ex, ey;
+    if (dev::in_tile(li, f.W - 2) && dev::in_tile(lj, f.H - 2)) {
+        const int b = lj * f.W + li;
+        const float2 e0 = f.s_e[b], e1 = f.s_e[b + 1], e2 = f.s_e[b + f.W],
+                     e3 = f.s_e[b + f.W + 1];
+        ex = dev::gather4(s, e0.x, e1.x, e2.x, e3.x);
+        ey = dev::gather4(s, e0.y, e1.y, e2.y, e3.y);
+    } else {
+        gather_global(a.Ex, a.Ey, s, a.nx, a.ny, ex, ey);
+        ++nstray;
+    }
+
+    dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
+    const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), (float)a.nx);
+    const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), (float)a.ny);
+    a.x[k] = xn;
+    a.y[k] = yn;
+    a.ux[k] = ux;
+    a.uy[k] = uy;
+    a.uz[k] = uz;
+
+    double coef;
+    if constexpr (PER_W) {
+        coef = a.qm * (double)wk / a.area;
+        if constexpr (KE) ke += (double)wk * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
+    } else {
+        coef = a.coef_u;
+        if constexpr (KE) ke += (double)ux * ux + (double)uy * uy + (double)uz * uz;
+    }
+
+    const dev::CicD d = dev::cic_d(xn, yn);
+    const int mi = local_index(d.i0, f.orgx, a.nx), mj = local_index(d.j0, f.orgy, a.ny);
+    const unsigned long long q0 = dev::fixq(coef, d.w0, a.scale);
+    const unsigned long long q1 = dev::fixq(coef, d.w1, a.scale);
+    const unsigned long long q2 = dev::fixq(coef, d.w2, a.scale);
+    const unsigned long long q3 = dev::fixq(coef, d.w3, a.scale);
+    if (dev::in_tile(mi, f.W - 2) && dev::in_tile(mj, f.H - 2)) {
+        const int b = mj * f.W + mi;
+        atomicAdd(&f.s_rho[b], q0);
+        atomicAdd(&f.s_rho[b + 1], q1);
+        atomicAdd(&f.s_rho[b + f.W], q2);
+        atomicAdd(&f.s_rho[b + f.W + 1], q3);
+    } else {
+        int i0, i1, j0, j1;
+        dev::wrap_pair(d.i0, a.nx, i0, i1);
+        dev::wrap_pair(d.j0, a.ny, j0, j1);
+        atomicAdd(&a.acc[j0 * a.nx + i0], q0);
+        atomicAdd(&a.acc[j0 * a.nx + i1], q1);
+        atomicAdd(&a.acc[j1 * a.nx + i0], q2);
+        atomicAdd(&a.acc[j1 * a.nx + i1], q3);
+        ++nstray;
+    }
+}
+
 template <bool PER_W, bool KE>
 __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
     extern __shared__ unsigned long long smem[];
     const int W = a.W, H = a.H, R = W * H;
-    unsigned long long* s_rho = smem;
-    float* s_ex = reinterpret_cast<float*>(smem + R);
-    float* s_ey = s_ex + R;
+    TileFrame f;
+    f.s_rho = smem;
+    float2* s_e = reinterpret_cast<float2*>(smem + R);
+    f.s_e = s_e;
+    f.W = W;
+    f.H = H;
 
     const int tile = blockIdx.x / a.bpt, part = blockIdx.x % a.bpt;
-    const int orgx = (tile % a.ntx) * a.tx - a.apron;
-    const int orgy = (tile / a.ntx) * a.ty - a.apron;
+    f.orgx = (tile % a.ntx) * a.tx - a.apron;
+    f.orgy = (tile / a.ntx) * a.ty - a.apron;
     for (int c = threadIdx.x; c < R; c += kThreads) {
-        const int gx = dev::wrapfar(orgx + c % W, a.nx);
-        const int gy = dev::wrapfar(orgy + c / W, a.ny);
-        s_ex[c] = a.Ex[gy * a.nx + gx];
-        s_ey[c] = a.Ey[gy * a.nx + gx];
-        s_rho[c] = 0ull;
+        const int gx = dev::wrapfar(f.orgx + c % W, a.nx);
+        const int gy = dev::wrapfar(f.orgy + c / W, a.ny);
+        s_e[c] = make_float2(a.Ex[gy * a.nx + gx], a.Ey[gy * a.nx + gx]);
+        f.s_rho[c] = 0ull;
     }
     __syncthreads();
 
     const int beg = a.off[tile], end = a.off[tile + 1];
     const int chunk = (end - beg + a.bpt - 1) / a.bpt;
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
-    const float fnx = (float)a.nx, fny = (float)a.ny;
     unsigned int nstray = 0;
     double ke = 0.0;
 
-    for (int k = r0 + (int)threadIdx.x; k < r1; k += kThreads) {
-        const float x = a.x[k], y = a.y[k];
-        float ux = a.ux[k], uy = a.uy[k], uz = a.uz[k];
-
-        // gather E at x^n (LDS tile, global fallback for strays)
-        const dev::CicF s = dev::cic_f(x, y);
-        const int li = local_index(s.i0, orgx, a.nx), lj = local_index(s.j0, orgy, a.ny);
-        float ex, ey;
-        if (dev::in_tile(li, W - 2) && dev::in_tile(lj, H - 2)) {
-            const int b = lj * W + li;
-            ex = dev::gather4(s, s_ex[b], s_ex[b + 1], s_ex[b + W], s_ex[b + W + 1]);
-            ey = dev::gather4(s, s_ey[b], s_ey[b + 1], s_ey[b + W], s_ey[b + W + 1]);
-        } else {
-            gather_global(a.Ex, a.Ey, s, a.nx, a.ny, ex, ey);
-            ++nstray;
-        }
-
-        // Boris (uniform B0), move, periodic wrap
-        dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
-        const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), fnx);
-        const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), fny);
-        a.x[k] = xn;
-        a.y[k] = yn;
-        a.ux[k] = ux;
-        a.uy[k] = uy;
-        a.uz[k] = uz;
-
-        double coef;
+    // two particles per thread per iteration: both sets of loads are issued
+    // before either is consumed
+    int k = r0 + (int)threadIdx.x;
+    for (; k + kThreads < r1; k += 2 * kThreads) {
+        const int k2 = k + kThreads;
+        const float x0 = a.x[k], y0 = a.y[k], ux0 = a.ux[k], uy0 = a.uy[k], uz0 = a.uz[k];
+        const float x1 = a.x[k2], y1 = a.y[k2], ux1 = a.ux[k2], uy1 = a.uy[k2], uz1 = a.uz[k2];
+        float w0 = 0.f, w1 = 0.f;
         if constexpr (PER_W) {
-            const float wk = a.w[k];
-            coef = a.qm * (double)wk / a.area;
-            if constexpr (KE) ke += (double)wk * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
-        } else {
-            coef = a.coef_u;
-            if constexpr (KE) ke += (double)ux * ux + (double)uy * uy + (double)uz * uz;
+            w0 = a.w[k];
+            w1 = a.w[k2];
         }
-
-        // deposit rho(x^{n+1})
-        const dev::CicD d = dev::cic_d(xn, yn);
-        const int mi = local_index(d.i0, orgx, a.nx), mj = local_index(d.j0, orgy, a.ny);
-        const unsigned long long q0 = dev::fixq(coef, d.w0, a.scale);
-        const unsigned long long q1 = dev::fixq(coef, d.w1, a.scale);
-        const unsigned long long q2 = dev::fixq(coef, d.w2, a.scale);
-        const unsigned long long q3 = dev::fixq(coef, d.w3, a.scale);
-        if (dev::in_tile(mi, W - 2) && dev::in_tile(mj, H - 2)) {
-            const int b = mj * W + mi;
-            atomicAdd(&s_rho[b], q0);
-            atomicAdd(&s_rho[b + 1], q1);
-            atomicAdd(&s_rho[b + W], q2);
-            atomicAdd(&s_rho[b + W + 1], q3);
-        } else {
-            int i0, i1, j0, j1;
-            dev::wrap_pair(d.i0, a.nx, i0, i1);
-            dev::wrap_pair(d.j0, a.ny, j0, j1);
-            atomicAdd(&a.acc[j0 * a.nx + i0], q0);
-            atomicAdd(&a.acc[j0 * a.nx + i1], q1);
-            atomicAdd(&a.acc[j1 * a.nx + i0], q2);
-            atomicAdd(&a.acc[j1 * a.nx + i1], q3);
-            ++nstray;
-        }
+        push_one<PER_W, KE>(a, f, k, x0, y0, ux0, uy0, uz0, w0, nstray, ke);
+        push_one<PER_W, KE>(a, f, k2, x1, y1, ux1, uy1, uz1, w1, nstray, ke);
+    }
+    if (k < r1) {
+        float w0 = 0.f;
+        if constexpr (PER_W) w0 = a.w[k];
+        push_one<PER_W, KE>(a, f, k, a.x[k], a.y[k], a.ux[k], a.uy[k], a.uz[k], w0, nstray, ke);
     }
 
     // per-workgroup stray count and kinetic-energy partial
@@ -186,10 +214,10 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
 
     // flush the LDS charge tile (every cell once)
     for (int c = threadIdx.x; c < R; c += kThreads) {
-        const unsigned long long v = s_rho[c];
+        const unsigned long long v = f.s_rho[c];
         if (v) {
-            const int gx = dev::wrapfar(orgx + c % W, a.nx);
-            const int gy = dev::wrapfar(orgy + c / W, a.ny);
+            const int gx = dev::wrapfar(f.orgx + c % W, a.nx);
+            const int gy = dev::wrapfar(f.orgy + c / W, a.ny);
             atomicAdd(&a.acc[gy * a.nx + gx], v);
         }
     }

@@ -70,6 +70,8 @@ class SimConfig:
     mode_amp: list = field(default_factory=list)
     legacy_load: bool = False
     fixed_scale: int = 0
+    blocks_per_tile: int = 0
+    autotune: bool = False
 
     def to_c(self) -> L.AwSimConfig:
         c = L.AwSimConfig()
@@ -84,6 +86,8 @@ class SimConfig:
             c.mode_amp[i] = a
         c.legacy_load = int(self.legacy_load)
         c.fixed_scale = self.fixed_scale
+        c.blocks_per_tile = self.blocks_per_tile
+        c.autotune = int(self.autotune)
         return c
 
 

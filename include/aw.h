@@ -282,6 +282,9 @@ typedef struct aw_sim_config {
     double mode_amp[32];    /* physical amplitude of mode m = 1..n_modes */
     int32_t legacy_load;    /* 1: Particles::initialize(rp) path */
     int32_t fixed_scale;    /* 0 -> chosen from the species densities */
+    int32_t blocks_per_tile; /* workgroups sharing one tile (0 -> heuristic) */
+    int32_t autotune;       /* 1: time candidate tile / bpt configurations online
+                               during the first steps and keep the fastest */
 } aw_sim_config;
 
 aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
