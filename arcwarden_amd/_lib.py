@@ -99,6 +99,26 @@ class AwSimConfig(C.Structure):
                 ("autotune", C.c_int32)]
 
 
+AW_DECK_MAX_SPECIES = 16
+AW_DECK_MAX_PROBES = 64
+
+
+class AwDeck(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("Lx", C.c_double), ("Ly", C.c_double),
+                ("rp", AwRunParams), ("nspecies", C.c_int32),
+                ("species", AwSpecies * AW_DECK_MAX_SPECIES), ("dump_every", C.c_int64),
+                ("outdir", C.c_char * 256), ("darwin", C.c_int32), ("yee", C.c_int32),
+                ("dx_wpe_c", C.c_double), ("c_direct", C.c_double), ("wce", C.c_double),
+                ("theta_deg", C.c_double), ("b0_direct", C.c_int32), ("b0_xc_set", C.c_int32),
+                ("bnoise", C.c_double), ("pump_enable", C.c_int32), ("pump_M", C.c_int32),
+                ("pump_w0", C.c_double), ("pump_ex0", C.c_double), ("pump_ey0", C.c_double),
+                ("pump_ez0", C.c_double), ("pump_amp", C.c_double), ("pump_trmp", C.c_double),
+                ("pump_toff", C.c_double), ("diag_enable", C.c_char * 256),
+                ("nprobes", C.c_int32), ("probes", C.c_double * AW_DECK_MAX_PROBES),
+                ("tsnap", C.c_double), ("band_lo", C.c_int32), ("band_hi", C.c_int32),
+                ("kt_stride", C.c_int32), ("n_frames", C.c_int32), ("prefix", C.c_char * 128)]
+
+
 P = C.c_void_p
 PP = C.POINTER
 _SIGS = {
@@ -121,6 +141,8 @@ _SIGS = {
                                     PP(C.c_double), P]),
     "aw_load_legacy": (C.c_int32, [P, PP(AwRunParams), PP(AwParticles), P]),
     "aw_deposit_rho": (C.c_int32, [P, PP(AwParticles), C.c_double, C.c_int32, P, P]),
+    "aw_deposit_esirkepov": (C.c_int32, [P, P, P, PP(AwParticles), C.c_double, C.c_double,
+                                         C.c_int32, P, P, P, P]),
     "aw_push_es": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32, P]),
     "aw_half_step_back": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32,
                                       P]),
@@ -129,6 +151,9 @@ _SIGS = {
     "aw_fft_r2c": (C.c_int32, [P, P, P, P]),
     "aw_fft_c2r": (C.c_int32, [P, P, P, P]),
     "aw_solve_es": (C.c_int32, [P, P, P, P, C.c_double, P]),
+    "aw_solve_darwin_el_b": (C.c_int32, [P, P, P, C.c_double, C.c_double,
+                                         PP(C.c_float), P]),
+    "aw_solve_darwin_et": (C.c_int32, [P, P, P, C.c_double, C.c_double, C.c_double, P]),
     "aw_reduce_energies": (C.c_int32, [P, PP(AwParticles), P, P, P, C.c_double,
                                        PP(AwDiagSample), P]),
     "aw_sim_create": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(AwSpecies), C.c_int32,
@@ -148,6 +173,8 @@ _SIGS = {
     "aw_sim_request_sort": (C.c_int32, [P]),
     "aw_sim_set_profiling": (C.c_int32, [P, C.c_int32]),
     "aw_sim_kernel_time": (C.c_int32, [P, PP(C.c_double), PP(C.c_int64), PP(C.c_int64)]),
+    "aw_sim_exec_config": (C.c_int32, [P, PP(C.c_int32), PP(C.c_int32), PP(C.c_int32)]),
+    "aw_deck_load": (C.c_int32, [C.c_char_p, PP(AwDeck)]),
 }
 
 _LIB = None

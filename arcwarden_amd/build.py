@@ -1,6 +1,7 @@
 """Build recipe for libarcwarden_hip.so (gfx950) -- the product library.
 
-Compiles every ``csrc/*.hip`` with hipcc for gfx950 and links them with rocFFT
+Compiles every ``csrc/*.hip`` with hipcc for gfx950 (and the host-only
+``csrc/*.cpp`` with g++) and links them with rocFFT
 into ``arcwarden_amd/libarcwarden_hip.so`` (in-tree, so it travels with the
 repository snapshot).  Usage: ``python -m arcwarden_amd.build`` or
 ``arcwarden_amd.build.build()``.
@@ -46,18 +47,25 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC)")
 
 
+# host-only sources (no device code): plain g++
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++20", "-Wall", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
 def _compile(src: Path) -> Path:
-    obj = OBJ / (src.stem + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+    obj = OBJ / (src.stem + (".o" if src.suffix == ".hip" else ".cpp.o"))
+    if src.suffix == ".hip":
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
+        raise RuntimeError(f"compile failed for {src.name}:\n{res.stdout}\n{res.stderr}")
     return obj
 
 
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if not sources:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     workers = min(len(sources), max(1, (os.cpu_count() or 4)))

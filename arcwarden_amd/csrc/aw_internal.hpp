@@ -180,12 +180,16 @@ struct aw_ctx {
     // rocFFT
     rocfft_plan r2c = nullptr, c2r = nullptr, c2r2 = nullptr;
     rocfft_plan r2c3 = nullptr, c2r3 = nullptr;
+    rocfft_plan r2c4 = nullptr, r2c7 = nullptr, c2r5 = nullptr;  // Darwin batches
     rocfft_execution_info info = nullptr;
     aw::DevBuf<unsigned char> fft_work;
     // workspace
     aw::DevBuf<float> ktmp;                  // complex scratch, 2 floats per mode
+    aw::DevBuf<float> rin;                   // R2C input copy (rocFFT may clobber it)
     aw::DevBuf<unsigned long long> acc;      // fixed-point grid accumulator
     aw::DevBuf<double> red;                  // reduction partials
+    aw::DevBuf<unsigned long long> jacc;     // Esirkepov Jx|Jy|Jz fixed-point accumulators
+    aw::DevBuf<unsigned int> jbound;         // Esirkepov scale bounds (max|w|, max|w uz|)
     aw::SortScratch sort;
     ~aw_ctx();
 };
@@ -213,7 +217,7 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
 // Deposit helpers (deposit.hip)
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign = false,
-                      unsigned long long* counter = nullptr);
+                      unsigned long long* counter = nullptr, float* out2 = nullptr);
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
                        const int32_t* sorted_off, hipStream_t s);
 
@@ -313,6 +317,62 @@ __device__ __forceinline__ void boris_es(const BorisConsts& b, bool has_b0, floa
     }
     ux = fmaf(b.qmh, ex, ux);
     uy = fmaf(b.qmh, ey, uy);
+}
+
+// boris_es for B0 along z (t[0] = t[1] = s[0] = s[1] = 0): every term the
+// general form takes from uz is an exact zero there, so this gives the same
+// ux, uy without reading uz, and uz is left unchanged.
+__device__ __forceinline__ void boris_es_bz(const BorisConsts& b, bool has_b0, float ex, float ey,
+                                            float& ux, float& uy) {
+    ux = fmaf(b.qmh, ex, ux);
+    uy = fmaf(b.qmh, ey, uy);
+    if (has_b0) {
+        const float upx = fmaf(uy, b.t[2], ux);
+        const float upy = fmaf(-ux, b.t[2], uy);
+        ux = fmaf(upy, b.s[2], ux);
+        uy = fmaf(-upx, b.s[2], uy);
+    }
+    ux = fmaf(b.qmh, ex, ux);
+    uy = fmaf(b.qmh, ey, uy);
+}
+
+// Full Boris with a per-particle (gathered) B: boris_update_full
+// (include/pic/pusher.hpp:103-109) with explicit fmaf; the oracle restates it.
+__device__ __forceinline__ void boris_full(float qmh, float ex, float ey, float ez, float bx,
+                                           float by, float bz, float& ux, float& uy, float& uz) {
+    ux = fmaf(qmh, ex, ux);
+    uy = fmaf(qmh, ey, uy);
+    uz = fmaf(qmh, ez, uz);
+    const float tx = qmh * bx, ty = qmh * by, tz = qmh * bz;
+    const float t2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
+    const float sf = 2.0f / (1.0f + t2);
+    const float sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    const float upx = fmaf(uy, tz, fmaf(-uz, ty, ux));
+    const float upy = fmaf(uz, tx, fmaf(-ux, tz, uy));
+    const float upz = fmaf(ux, ty, fmaf(-uy, tx, uz));
+    ux = fmaf(upy, sz, fmaf(-upz, sy, ux));
+    uy = fmaf(upz, sx, fmaf(-upx, sz, uy));
+    uz = fmaf(upx, sy, fmaf(-upy, sx, uz));
+    ux = fmaf(qmh, ex, ux);
+    uy = fmaf(qmh, ey, uy);
+    uz = fmaf(qmh, ez, uz);
+}
+
+// Exact rint(x) as int64 for |x| < 2^62 with FP32/int32 operations only:
+// x = hi*2^31 + rem with hi = trunc(x/2^31); rem = x - hi*2^31 is an integer
+// below 2^31 in magnitude and representable in FP32, so the fma is exact.
+__device__ __forceinline__ long long f2ll_rn(float x) {
+    const float xr = rintf(x);
+    const int hi = (int)truncf(xr * 4.656612873077393e-10f);
+    const float rem = fmaf(-(float)hi, 2147483648.0f, xr);
+    return ((long long)hi << 31) + (long long)(int)rem;
+}
+
+// FP32 fixed-point contribution: rint(v * w) as a 64-bit two's-complement
+// integer, v carrying the quantity's 2^S scale.  Every product of at least
+// 2^23 is already an integer, so the sum keeps full FP32 product precision.
+__device__ __forceinline__ unsigned long long fixq32(float v, float w) {
+    return (unsigned long long)f2ll_rn(v * w);
 }
 
 // CIC weights in double (exact products of 24-bit factors) and the

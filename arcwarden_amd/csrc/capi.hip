@@ -12,7 +12,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 
 namespace aw {
 
@@ -40,6 +43,29 @@ int reduce_partial_capacity();
 int reduce_grid_offset();
 int reduce_out_offset();
 int tile_bpt(const aw_ctx& c, int64_t n);
+bool uz_evolves(const aw_run_params& rp, bool has_b0);
+void enable_em_fft(aw_ctx& c);
+void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
+                      const float B0[3], hipStream_t s);
+void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
+                     double n0, hipStream_t s);
+void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, unsigned long long* acc8,
+                const float cq[3], int bpt, unsigned long long* strays, hipStream_t s);
+void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
+            const float* B3, unsigned long long* acc3, const float* c_dcu, int bpt,
+            unsigned long long* strays, hipStream_t s);
+int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
+            const float* B3, unsigned long long* acc8, const float cq[3], double qm, double dt,
+            int bpt, unsigned long long* strays, double* ke_part, hipStream_t s);
+void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
+                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
+                       hipStream_t s);
+void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
+                   const double* inv, hipStream_t s);
+void form_e(float* E3, const float* EL2, const float* ET3, const aw_grid& g,
+            const aw_run_params& rp, double t, hipStream_t s);
+void dcu_scale(const float* E3, const float* B3, int64_t ncell, float* part, double coef_abs,
+               double coef_signed, double umax, float* c_dcu, double* inv_dcu, hipStream_t s);
 
 namespace {
 
@@ -59,9 +85,20 @@ void check_particles(const aw_particles* p) {
         AW_REQUIRE(p->x && p->y && p->ux && p->uy && p->uz, "particle arrays must be non-NULL");
 }
 
+// also writes the R2C input copy (c.rin) so the solve needs no extra copy
 void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s,
                   unsigned long long* counter = nullptr) {
-    fix_to_float_add(c.acc.get(), out, ncell(c), c.fix_log2, true, s, assign, counter);
+    c.rin.reserve((size_t)ncell(c));
+    fix_to_float_add(c.acc.get(), out, ncell(c), c.fix_log2, true, s, assign, counter,
+                     assign ? c.rin.get() : nullptr);
+}
+
+void set_ctx_tiles(aw_ctx& c, int tile_x, int tile_y) {
+    c.tx = std::min(tile_x > 0 ? tile_x : 16, c.g.nx);
+    c.ty = std::min(tile_y > 0 ? tile_y : 16, c.g.ny);
+    c.ntx = (c.g.nx + c.tx - 1) / c.tx;
+    c.nty = (c.g.ny + c.ty - 1) / c.ty;
+    c.ntiles = c.ntx * c.nty;
 }
 
 void sort_tiles(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* perm,
@@ -118,6 +155,130 @@ struct aw_sim {
     int64_t prof_launches = 0;
     int bpt = 0;
 
+    // ---- spectral Darwin (field_model == AW_MODEL_DARWIN) ----
+    bool em = false;
+    DevBuf<unsigned long long> acc8, acc3;  // [rho|J|amu] next state, [dcu]
+    DevBuf<float> src;   // [rho|Jx|Jy|Jz] ++ [dcux|dcuy|dcuz|amu0..amu3]
+    DevBuf<float> fftin; // same layout: the R2C inputs (may be overwritten by rocFFT)
+    DevBuf<float> elb;   // [ELx|ELy|Bx|By|Bz] (B includes B0)
+    DevBuf<float> et;    // [ETx|ETy|ETz]
+    DevBuf<float> e3;    // E = E_L + E_T + pump (push)
+    DevBuf<float> ed3;   // E_L + pump (dcu gather)
+    DevBuf<double> inv;  // inverse scales: rho, J x3, amu x4, dcu x3
+    DevBuf<float> cdcu, fmax_part;
+    float cq[3] = {0.f, 0.f, 0.f};
+    double dcu_abs = 0.0, dcu_signed = 0.0, umax = 1.0;
+    bool moments_ready = false;
+
+    float* Ex() const { return em ? e3.get() : E.get(); }
+    float* Ey() const { return em ? e3.get() + ncell(*ctx) : E.get() + ncell(*ctx); }
+    float* rho_ptr() const { return em ? src.get() : rho.get(); }
+    float* B3() const { return elb.get() + 2 * ncell(*ctx); }
+
+    void setup_em() {
+        em = true;
+        const int64_t nc = ncell(*ctx);
+        enable_em_fft(*ctx);
+        acc8.resize((size_t)8 * nc);
+        acc3.resize((size_t)3 * nc);
+        src.resize((size_t)11 * nc);
+        fftin.resize((size_t)11 * nc);
+        elb.resize((size_t)5 * nc);
+        et.resize((size_t)3 * nc);
+        e3.resize((size_t)3 * nc);
+        ed3.resize((size_t)3 * nc);
+        inv.resize(11);
+        cdcu.resize(1);
+        fmax_part.resize(512);
+        // static per-quantity fixed-point scales: the per-contribution bound maps
+        // to 2^48, leaving 2^14 bound-sized contributions of headroom per cell
+        double wmax = w_uniform;
+        for (const auto& s : species)
+            wmax = std::max(wmax, s.density * g.dx * g.dy / s.ppc);
+        const double area = g.dx * g.dy;
+        umax = 2.0 * std::max(1.0, rp.c);
+        const double cabs = std::fabs(rp.qm) * wmax / area;
+        auto expo = [](double bound) {
+            return 48 - (int)std::ceil(std::log2(std::max(bound, 1e-300)));
+        };
+        const int Sr = expo(cabs), Sj = expo(cabs * umax), Sa = expo(cabs * umax * umax);
+        const double mult = per_w ? 1.0 : w_uniform;  // per-particle w multiplies in-kernel
+        cq[0] = (float)std::ldexp(rp.qm * mult / area, Sr);
+        cq[1] = (float)std::ldexp(rp.qm * mult / area, Sj);
+        cq[2] = (float)std::ldexp(rp.qm * mult / area, Sa);
+        double h[11] = {std::ldexp(1.0, -Sr), std::ldexp(1.0, -Sj), std::ldexp(1.0, -Sj),
+                        std::ldexp(1.0, -Sj), std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa),
+                        std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa), 1.0, 1.0, 1.0};
+        AW_HIP(hipMemcpy(inv.get(), h, sizeof(h), hipMemcpyHostToDevice));
+        dcu_abs = rp.qm * rp.qm * wmax / area;
+        dcu_signed = rp.qm * rp.qm * mult / area;
+        acc8.zero(stream);
+        acc3.zero(stream);
+        et.zero(stream);
+    }
+
+    void init_em() {
+        em_moments(*ctx, particles(), tile_off.get(), acc8.get(), cq, bpt, nullptr, stream);
+        moments_ready = true;
+    }
+
+    // darwin_fields (include/pic/simulation.hpp:103-119) + boris_em + migrate
+    void step_em(int64_t nstep) {
+        const hipStream_t s = stream;
+        const int64_t nc = ncell(*ctx);
+        const double t = (double)nstep * rp.dt;
+        if (!moments_ready)
+            em_moments(*ctx, particles(), tile_off.get(), acc8.get(), cq, bpt, nullptr, s);
+        moments_ready = false;
+        // rho, J -> src4; amu -> src7[3..6]; fftin holds the copies rocFFT may clobber
+        float* fin = fftin.get();
+        convert_multi(acc8.get(), src.get(), fin, nc, 4, inv.get(), s);
+        convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin + 7 * nc, nc, 4, inv.get() + 4,
+                      s);
+        solve_darwin_elb(*ctx, fin, elb.get(), rp.eps0, rp.c, rp.B0, s);
+        form_e(ed3.get(), elb.get(), nullptr, g, rp, t, s);
+        if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) do_sort();
+        dcu_scale(ed3.get(), B3(), nc, fmax_part.get(), dcu_abs, dcu_signed, umax, cdcu.get(),
+                  inv.get() + 8, s);
+        em_dcu(*ctx, particles(), tile_off.get(), ed3.get(), B3(), acc3.get(), cdcu.get(), bpt,
+               strays.get(), s);
+        convert_multi(acc3.get(), src.get() + 4 * nc, fin + 4 * nc, nc, 3, inv.get() + 8, s);
+        solve_darwin_et(*ctx, fin + 4 * nc, et.get(), rp.eps0, rp.c, rp.n0, s);
+        form_e(e3.get(), elb.get(), et.get(), g, rp, t, s);
+        const bool dump = rp.dump_every <= 0 || (nstep % rp.dump_every) == 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof) {
+            AW_HIP(hipEventCreate(&e0));
+            AW_HIP(hipEventCreate(&e1));
+            AW_HIP(hipEventRecord(e0, s));
+        }
+        const int nblk = em_push(*ctx, particles(), tile_off.get(), e3.get(), B3(), acc8.get(),
+                                 cq, rp.qm, rp.dt, bpt, strays.get(),
+                                 dump ? ke_part.get() : nullptr, s);
+        if (prof) {
+            AW_HIP(hipEventRecord(e1, s));
+            prof_ev.emplace_back(e0, e1);
+        }
+        moments_ready = true;
+        if (!stray_pending) {
+            AW_HIP(hipMemcpyAsync(stray_host, strays.get(), sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+            AW_HIP(hipEventRecord(stray_ev, s));
+            stray_pending = true;
+        }
+        AW_HIP(hipMemsetAsync(strays.get(), 0, sizeof(unsigned long long), s));
+        if (dump) record_diag(nstep, nblk);
+        ++steps_since_sort;
+    }
+
+    void record_diag(int64_t nstep, int nblk) {
+        if ((int)diag_pending.size() >= diag_cap) flush_diag();
+        const int ng = grid_partials(Ex(), Ey(), rho_ptr(), ncell(*ctx), grid_part.get(), stream);
+        finalize_diag(ke_part.get(), nblk, per_w ? 1.0 : w_uniform, grid_part.get(), ng,
+                      diag_out.get() + 4 * diag_pending.size(), stream);
+        diag_pending.push_back(nstep);
+    }
+
     aw_particles particles(int which) const {
         aw_particles p{};
         p.x = P[which][0].get();
@@ -131,8 +292,6 @@ struct aw_sim {
         return p;
     }
     aw_particles particles() const { return particles(cur); }
-    float* Ex() const { return E.get(); }
-    float* Ey() const { return E.get() + ncell(*ctx); }
 
     ~aw_sim() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -183,7 +342,92 @@ struct aw_sim {
         if ((double)*stray_host > frac * (double)n) sort_requested = true;
     }
 
+    // Tile geometry and workgroups per tile of the particle kernels.
+    void apply_exec(int tile, int nbpt) {
+        AW_HIP(hipStreamSynchronize(stream));
+        set_ctx_tiles(*ctx, tile, tile);
+        bpt = nbpt > 0 ? nbpt : tile_bpt(*ctx, n);
+        tile_off.resize((size_t)ctx->ntiles + 1);
+        ke_part.resize((size_t)ctx->ntiles * bpt + 1);
+    }
+
+    // Autotuner (ES): for tile 8/16/32 the state is loaded and initialised,
+    // then the fused push+deposit is timed (HIP events, best of 3, each
+    // trial on a copy of the particles) for 1..16 workgroups per tile; the
+    // fastest pair is kept and cached for the process per (device, grid,
+    // particle count, weights, uz invariance).  Every deposit is fixed-point
+    // and order-free, so the choice changes no result.
+    void autotune() {
+        using Key = std::tuple<int, int, int, int64_t, bool, bool>;
+        static std::mutex mu;
+        static std::map<Key, std::pair<int, int>> cache;
+        const Key key{ctx->device, g.nx, g.ny, n, per_w, uz_evolves(rp, cfg.has_b0 != 0)};
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const auto it = cache.find(key);
+            if (it != cache.end()) {
+                apply_exec(it->second.first, it->second.second);
+                return;
+            }
+        }
+        struct Events {
+            hipEvent_t a = nullptr, b = nullptr;
+            ~Events() {
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
+            }
+        } ev;
+        AW_HIP(hipEventCreate(&ev.a));
+        AW_HIP(hipEventCreate(&ev.b));
+        const hipStream_t s = stream;
+        float best = 0.f;
+        int best_tile = 0, best_bpt = 0;
+        for (const int tile : {8, 16, 32}) {
+            if (tile > g.nx || tile > g.ny) continue;
+            apply_exec(tile, 1);
+            init_state();
+            const int64_t per_tile = n / ctx->ntiles;
+            for (const int b : {1, 2, 4, 8, 16}) {
+                if (b > 1 && per_tile / b < 256) break;
+                float tmin = 0.f;
+                for (int rep = 0; rep < 3; ++rep) {
+                    const aw_particles src = particles(cur), dst = particles(cur ^ 1);
+                    const size_t bytes = (size_t)n * sizeof(float);
+                    AW_HIP(hipMemcpyAsync(dst.x, src.x, bytes, hipMemcpyDeviceToDevice, s));
+                    AW_HIP(hipMemcpyAsync(dst.y, src.y, bytes, hipMemcpyDeviceToDevice, s));
+                    AW_HIP(hipMemcpyAsync(dst.ux, src.ux, bytes, hipMemcpyDeviceToDevice, s));
+                    AW_HIP(hipMemcpyAsync(dst.uy, src.uy, bytes, hipMemcpyDeviceToDevice, s));
+                    AW_HIP(hipMemcpyAsync(dst.uz, src.uz, bytes, hipMemcpyDeviceToDevice, s));
+                    AW_HIP(hipEventRecord(ev.a, s));
+                    push_deposit_es(*ctx, dst, tile_off.get(), Ex(), Ey(), rp, cfg.has_b0 != 0,
+                                    strays.get(), nullptr, b, s);
+                    AW_HIP(hipEventRecord(ev.b, s));
+                    AW_HIP(hipEventSynchronize(ev.b));
+                    float ms = 0.f;
+                    AW_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+                    if (rep == 0 || ms < tmin) tmin = ms;
+                }
+                if (best_tile == 0 || tmin < best) {
+                    best = tmin;
+                    best_tile = tile;
+                    best_bpt = b;
+                }
+            }
+        }
+        if (best_tile == 0) return;  // grid smaller than every candidate tile
+        apply_exec(best_tile, best_bpt);
+        nsorts = 0;
+        initialized = false;
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = {best_tile, best_bpt};
+    }
+
     void init() {
+        if (cfg.autotune && !em && n > 0) autotune();
+        init_state();
+    }
+
+    void init_state() {
         const hipStream_t s = stream;
         const aw_particles p = particles(cur);
         if (cfg.legacy_load) {
@@ -194,10 +438,17 @@ struct aw_sim {
                          cfg.n_modes, p, wts.data(), s);
         }
         do_sort();
+        if (em) {  // Darwin: t=0 moments; fields are solved inside each step
+            init_em();
+            strays.zero(s);
+            initialized = true;
+            AW_HIP(hipStreamSynchronize(s));
+            return;
+        }
         ctx->acc.zero(s);
         deposit_rho_fixed(*ctx, particles(), rp.qm, AW_DEPOSIT_BINNED, tile_off.get(), s);
         fix_to_float(*ctx, rho.get(), true, s);
-        solve_es_contig(*ctx, rho.get(), E.get(), rp.eps0, s);
+        solve_es_contig(*ctx, ctx->rin.get(), E.get(), rp.eps0, s);
         push_es(*ctx, particles(), Ex(), Ey(), rp, cfg.has_b0 != 0, true, s);
         strays.zero(s);  // afterwards cleared by each step's rho conversion
         fields_ready = true;
@@ -209,9 +460,13 @@ struct aw_sim {
         if (!initialized) fail(AW_ERR_STATE, "aw_sim_step before aw_sim_init", __FILE__, __LINE__);
         const hipStream_t s = stream;
         poll_strays();
+        if (em) {
+            step_em(nstep);
+            return;
+        }
         if (!fields_ready) {
             fix_to_float(*ctx, rho.get(), true, s, strays.get());
-            solve_es_contig(*ctx, rho.get(), E.get(), rp.eps0, s);
+            solve_es_contig(*ctx, ctx->rin.get(), E.get(), rp.eps0, s);
         }
         fields_ready = false;
         if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) do_sort();
@@ -235,13 +490,7 @@ struct aw_sim {
             AW_HIP(hipEventRecord(stray_ev, s));
             stray_pending = true;
         }
-        if (dump) {
-            if ((int)diag_pending.size() >= diag_cap) flush_diag();
-            const int ng = grid_partials(Ex(), Ey(), rho.get(), ncell(*ctx), grid_part.get(), s);
-            finalize_diag(ke_part.get(), nblk, per_w ? 1.0 : w_uniform, grid_part.get(), ng,
-                          diag_out.get() + 4 * diag_pending.size(), s);
-            diag_pending.push_back(nstep);
-        }
+        if (dump) record_diag(nstep, nblk);
         ++steps_since_sort;
     }
 
@@ -271,11 +520,7 @@ aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y) {
     c->g = *g;
     if (c->g.Lx <= 0) c->g.Lx = g->nx * g->dx;
     if (c->g.Ly <= 0) c->g.Ly = g->ny * g->dy;
-    c->tx = std::min(tile_x > 0 ? tile_x : 16, g->nx);
-    c->ty = std::min(tile_y > 0 ? tile_y : 16, g->ny);
-    c->ntx = (g->nx + c->tx - 1) / c->tx;
-    c->nty = (g->ny + c->ty - 1) / c->ty;
-    c->ntiles = c->ntx * c->nty;
+    set_ctx_tiles(*c, tile_x, tile_y);
     AW_HIP(hipGetDevice(&c->device));
     AW_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device));
     int lds = 0;
@@ -458,6 +703,18 @@ aw_status aw_deposit_rho(aw_ctx* ctx, const aw_particles* p, double qm, int32_t 
     });
 }
 
+aw_status aw_deposit_esirkepov(aw_ctx* ctx, const float* x0, const float* y0,
+                               const aw_particles* p, double qm, double dt, int32_t policy,
+                               float* jx, float* jy, float* jz, aw_stream s) {
+    return guard("aw_deposit_esirkepov", [&] {
+        AW_REQUIRE(ctx && jx && jy && jz, "ctx/jx/jy/jz is NULL");
+        check_particles(p);
+        AW_REQUIRE(p->n == 0 || (x0 && y0), "x0/y0 is NULL");
+        AW_REQUIRE(policy >= AW_DEPOSIT_GLOBAL && policy <= AW_DEPOSIT_AUTO, "bad policy");
+        deposit_esirkepov(*ctx, x0, y0, *p, qm, dt, policy, jx, jy, jz, S(s));
+    });
+}
+
 aw_status aw_push_es(aw_ctx* ctx, aw_particles* p, const float* Ex, const float* Ey,
                      const aw_run_params* rp, int32_t has_b0, aw_stream s) {
     return guard("aw_push_es", [&] {
@@ -498,7 +755,11 @@ aw_status aw_push_deposit_es(aw_ctx* ctx, aw_particles* p, const int32_t* off,
 aw_status aw_fft_r2c(aw_ctx* ctx, const float* in, void* out_k, aw_stream s) {
     return guard("aw_fft_r2c", [&] {
         AW_REQUIRE(ctx && in && out_k, "NULL argument");
-        fft_exec(*ctx, ctx->r2c, const_cast<float*>(in), out_k, S(s));
+        // rocFFT may overwrite an out-of-place input; keep the caller's array intact
+        ctx->rin.reserve((size_t)ncell(*ctx));
+        AW_HIP(hipMemcpyAsync(ctx->rin.get(), in, (size_t)ncell(*ctx) * sizeof(float),
+                              hipMemcpyDeviceToDevice, S(s)));
+        fft_exec(*ctx, ctx->r2c, ctx->rin.get(), out_k, S(s));
     });
 }
 
@@ -515,6 +776,27 @@ aw_status aw_solve_es(aw_ctx* ctx, const float* rho, float* Ex, float* Ey, doubl
         AW_REQUIRE(ctx && rho && Ex && Ey, "NULL argument");
         AW_REQUIRE(eps0 > 0, "eps0 must be positive");
         solve_es(*ctx, rho, Ex, Ey, eps0, S(s));
+    });
+}
+
+aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, double eps0,
+                               double c, const float B0[3], aw_stream s) {
+    return guard("aw_solve_darwin_el_b", [&] {
+        AW_REQUIRE(ctx && src4 && out5, "NULL argument");
+        AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
+        const float zero[3] = {0.f, 0.f, 0.f};
+        enable_em_fft(*ctx);
+        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s));
+    });
+}
+
+aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double eps0,
+                             double c, double n0, aw_stream s) {
+    return guard("aw_solve_darwin_et", [&] {
+        AW_REQUIRE(ctx && src7 && out3, "NULL argument");
+        AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
+        enable_em_fft(*ctx);
+        solve_darwin_et(*ctx, src7, out3, eps0, c, n0, S(s));
     });
 }
 
@@ -551,8 +833,10 @@ aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
     return guard("aw_sim_create", [&] {
         AW_REQUIRE(out && rp && cfg, "out/rp/cfg is NULL");
         check_grid(g);
-        AW_REQUIRE(cfg->field_model == AW_MODEL_ES,
-                   "only the electrostatic field model is available in this build");
+        AW_REQUIRE(cfg->field_model == AW_MODEL_ES || cfg->field_model == AW_MODEL_DARWIN,
+                   "field_model must be AW_MODEL_ES or AW_MODEL_DARWIN");
+        if (cfg->field_model == AW_MODEL_DARWIN)
+            AW_REQUIRE(rp->c > 0 && rp->n0 >= 0, "Darwin needs rp.c > 0 and rp.n0 >= 0");
         AW_REQUIRE(rp->dt > 0, "rp.dt must be positive");
         AW_REQUIRE(rp->eps0 > 0, "rp.eps0 must be positive");
         auto sim = std::make_unique<aw_sim>();
@@ -587,7 +871,12 @@ aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         }
         AW_REQUIRE(sim->n < (int64_t)INT32_MAX, "particle count must fit int32");
         AW_HIP(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
-        sim->ctx.reset(make_ctx(&sim->g, cfg->tile_x, cfg->tile_y));
+        // default ES tile: 8x8 at >= 64 particles per cell (more, smaller
+        // workgroups; the apron overhead stays small), else 16x16
+        int tx = cfg->tile_x, ty = cfg->tile_y;
+        if (tx <= 0 && ty <= 0 && cfg->field_model != AW_MODEL_DARWIN)
+            tx = ty = sim->n >= 64 * nc ? 8 : 16;
+        sim->ctx.reset(make_ctx(&sim->g, tx, ty));
         sim->ctx->fix_log2 = cfg->fixed_scale > 0
                                  ? cfg->fixed_scale
                                  : fixed_scale_for(sim->species, *rp, cfg->legacy_load != 0);
@@ -603,6 +892,7 @@ aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         sim->ke_part.resize((size_t)sim->ctx->ntiles * sim->bpt + 1);
         sim->grid_part.resize((size_t)3 * 512 + 3);
         sim->diag_out.resize((size_t)4 * sim->diag_cap);
+        if (cfg->field_model == AW_MODEL_DARWIN) sim->setup_em();
         AW_HIP(hipHostMalloc((void**)&sim->stray_host, sizeof(unsigned long long),
                              hipHostMallocDefault));
         AW_HIP(hipEventCreateWithFlags(&sim->stray_ev, hipEventDisableTiming));
@@ -670,6 +960,18 @@ aw_status aw_sim_fields(aw_sim* sim, aw_fields* out) {
         std::memset(out, 0, sizeof(*out));
         out->Ex = sim->Ex();
         out->Ey = sim->Ey();
+        if (sim->em) {
+            const int64_t nc = ncell(*sim->ctx);
+            out->Ez = sim->e3.get() + 2 * nc;
+            out->Bx = sim->B3();
+            out->By = sim->B3() + nc;
+            out->Bz = sim->B3() + 2 * nc;
+            out->ELx = sim->elb.get();
+            out->ELy = sim->elb.get() + nc;
+            out->ETx = sim->et.get();
+            out->ETy = sim->et.get() + nc;
+            out->ETz = sim->et.get() + 2 * nc;
+        }
     });
 }
 
@@ -677,7 +979,21 @@ aw_status aw_sim_sources(aw_sim* sim, aw_sources* out) {
     return guard("aw_sim_sources", [&] {
         AW_REQUIRE(sim && out, "NULL argument");
         std::memset(out, 0, sizeof(*out));
-        out->rho = sim->rho.get();
+        out->rho = sim->rho_ptr();
+        if (sim->em) {
+            const int64_t nc = ncell(*sim->ctx);
+            float* s = sim->src.get();
+            out->Jx = s + nc;
+            out->Jy = s + 2 * nc;
+            out->Jz = s + 3 * nc;
+            out->dcux = s + 4 * nc;
+            out->dcuy = s + 5 * nc;
+            out->dcuz = s + 6 * nc;
+            out->amu0 = s + 7 * nc;
+            out->amu1 = s + 8 * nc;
+            out->amu2 = s + 9 * nc;
+            out->amu3 = s + 10 * nc;
+        }
     });
 }
 
@@ -701,7 +1017,7 @@ aw_status aw_sim_diag_now(aw_sim* sim, int64_t step, aw_diag_sample* out) {
         const int nke = p.n ? ke_partials(p, part, sim->stream) : 0;
         double* gp = part + reduce_grid_offset();
         double* o = part + reduce_out_offset();
-        const int ng = grid_partials(sim->Ex(), sim->Ey(), sim->rho.get(), ncell(c), gp,
+        const int ng = grid_partials(sim->Ex(), sim->Ey(), sim->rho_ptr(), ncell(c), gp,
                                      sim->stream);
         finalize_diag(part, nke, p.w ? 1.0 : p.w_uniform, gp, ng, o, sim->stream);
         double h[4];
@@ -742,6 +1058,16 @@ aw_status aw_sim_kernel_time(aw_sim* sim, double* total_ms, int64_t* launches, i
         if (total_ms) *total_ms = sim->prof_ms;
         if (launches) *launches = sim->prof_launches;
         if (sorts) *sorts = sim->nsorts;
+    });
+}
+
+aw_status aw_sim_exec_config(aw_sim* sim, int32_t* tile_x, int32_t* tile_y,
+                             int32_t* blocks_per_tile) {
+    return guard("aw_sim_exec_config", [&] {
+        AW_REQUIRE(sim, "sim is NULL");
+        if (tile_x) *tile_x = sim->ctx->tx;
+        if (tile_y) *tile_y = sim->ctx->ty;
+        if (blocks_per_tile) *blocks_per_tile = sim->bpt;
     });
 }
 

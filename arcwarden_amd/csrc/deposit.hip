@@ -120,36 +120,43 @@ deposit_tiles_kernel(DepArgs a, TileGeom t, unsigned long long* __restrict__ acc
 // out = (assign ? 0 : out) + acc * 2^-S; optionally clears acc for the next
 // accumulation and one 64-bit counter (saves separate memset launches).
 __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
-                                    int64_t n, double inv, int zero_acc, int assign,
-                                    unsigned long long* __restrict__ counter) {
+                                    float* __restrict__ out2, int64_t n, double inv, int zero_acc,
+                                    int assign, unsigned long long* __restrict__ counter) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0 && counter) *counter = 0ull;
     if (c >= n) return;
     const long long v = (long long)acc[c];
     const float add = (float)((double)v * inv);
-    out[c] = assign ? add : out[c] + add;
+    const float r = assign ? add : out[c] + add;
+    out[c] = r;
+    if (out2) out2[c] = r;
     if (zero_acc) acc[c] = 0ull;
 }
 
 }  // namespace
 
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
-                      bool zero_acc, hipStream_t s, bool assign, unsigned long long* counter) {
+                      bool zero_acc, hipStream_t s, bool assign, unsigned long long* counter,
+                      float* out2) {
     if (n == 0) return;
     fix_to_float_kernel<<<blocks_for(n), kThreads, 0, s>>>(
-        const_cast<unsigned long long*>(acc), out, n, std::ldexp(1.0, -scale_log2),
+        const_cast<unsigned long long*>(acc), out, out2, n, std::ldexp(1.0, -scale_log2),
         zero_acc ? 1 : 0, assign ? 1 : 0, counter);
     AW_LAUNCH_CHECK();
 }
 
 int tile_bpt(const aw_ctx& c, int64_t n) {
-    // enough workgroups to fill every CU several times, but keep >= 4 waves
-    // of particles per workgroup
-    const int target = 8 * std::max(1, c.num_cu);
-    int bpt = c.ntiles >= target ? 1 : (target + c.ntiles - 1) / c.ntiles;
-    const int64_t cap = n / std::max(1, c.ntiles) / kThreads;
-    if (bpt > cap + 1) bpt = (int)cap + 1;
-    return std::max(1, bpt);
+    // about 2048 particles (8 per lane) per workgroup: enough to amortise the
+    // tile load and flush, few enough that the grid holds many workgroups per
+    // CU and the tail of the launch stays short
+    const int64_t per_tile = n / std::max(1, c.ntiles);
+    int bpt = (int)std::clamp<int64_t>((per_tile + 2047) / 2048, 1, 16);
+    // small grids: keep >= 4 workgroups per CU while each keeps >= 1 particle
+    // per lane
+    const int64_t want = 4ll * std::max(1, c.num_cu);
+    while (bpt < 64 && (int64_t)c.ntiles * bpt < want && per_tile / (2 * bpt) >= kThreads)
+        bpt *= 2;
+    return bpt;
 }
 
 // Accumulates rho of p into c.acc (fixed point).  sorted_off != nullptr:

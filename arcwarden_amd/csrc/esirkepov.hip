@@ -1,0 +1,369 @@
+// esirkepov.hip -- charge-conserving (Esirkepov) current deposit.
+//
+// Reference: s1_shape4, esirkepov_scatter, GlobalJSink and the deposit half
+// of k_push_esirkepov / k_push_esirkepov_tiled (include/pic/yee2d.hpp:124-132,
+// 310-389, 401-469).  For a move (x0, y0) -> (x1, y1) of less than one cell the
+// 4x4 union stencil {ib-1..ib+2} x {jb-1..jb+2}, ib = floor(min(x0, x1)),
+// carries
+//   Jx(i+1/2, j) = -q w/(dt dy) * sum_{a<=i} DSx_a (Sy0_j + DSy_j/2)
+//   Jy(i, j+1/2) = -q w/(dt dx) * sum_{b<=j} DSy_b (Sx0_i + DSx_i/2)
+//   Jz(i, j)     =  q w uz/(dx dy) * (Sx0 Sy0 + (DSx Sy0 + Sx0 DSy)/2 + DSx DSy/3)
+// with S the CIC node shape max(0, 1 - |x - node|) and DS = S(x1) - S(x0).
+//
+// MI355X design: shapes and products in FP64, contributions summed as int64
+// fixed point (order-free, so every policy returns the same bits).  The
+// per-component scale comes from a bound pass over the particles: max |w| and
+// max |w uz| give each component's largest single contribution, which maps to
+// at most 2^44 (2^19 such contributions of headroom per node).  Policies:
+// global atomics, whole grid in LDS, or tiles binned by the old position with
+// a 2-node apron in LDS (moves that leave it fall back to global atomics).
+#include "aw_internal.hpp"
+
+namespace aw {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kFixBits = 44;
+constexpr int kApron = 2;
+
+inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
+
+struct EskArgs {
+    const float *x0, *y0, *x1, *y1, *uz, *w;
+    int n, nx, ny;
+    double qm, w_uniform, dt, dx, dy;
+    const unsigned int* bound;  // [0] max|w| (or 0), [1] max|w uz| (|uz| for uniform w)
+};
+
+// Scale exponents of the three components (identical in deposit and convert).
+__device__ __forceinline__ void esk_exponents(const EskArgs& a, int S[3]) {
+    const double wmax = a.w ? (double)__uint_as_float(a.bound[0]) : fabs(a.w_uniform);
+    const double wuz = a.w ? (double)__uint_as_float(a.bound[1])
+                           : fabs(a.w_uniform) * (double)__uint_as_float(a.bound[1]);
+    const double b[3] = {fabs(a.qm) * wmax / (a.dt * a.dy), fabs(a.qm) * wmax / (a.dt * a.dx),
+                         fabs(a.qm) * wuz / (a.dx * a.dy)};
+    for (int q = 0; q < 3; ++q) {
+        int e = 0;
+        (void)frexp(b[q], &e);  // b < 2^e
+        S[q] = kFixBits - e;
+    }
+}
+
+__device__ __forceinline__ unsigned long long fixd(double v) {
+    return (unsigned long long)__double2ll_rn(v);
+}
+
+__device__ __forceinline__ void shape4(double x, int base, double S[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S[k] = fmax(0.0, 1.0 - fabs(x - (double)(base - 1 + k)));
+}
+
+// One particle's move: stencil origin (ib, jb) and the FP64 shape factors.
+struct EskMove {
+    int ib, jb;
+    double Sx0[4], Sy0[4], DSx[4], DSy[4];
+    double cJx, cJy, cJz;  // fixed-point scaled coefficients
+};
+
+__device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double scale[3],
+                                            EskMove& m) {
+    const double X0 = a.x0[i], Y0 = a.y0[i];
+    double X1 = a.x1[i], Y1 = a.y1[i];
+    // minimal image: p's positions may already be wrapped
+    if (X1 - X0 > 0.5 * a.nx) X1 -= a.nx;
+    else if (X0 - X1 > 0.5 * a.nx) X1 += a.nx;
+    if (Y1 - Y0 > 0.5 * a.ny) Y1 -= a.ny;
+    else if (Y0 - Y1 > 0.5 * a.ny) Y1 += a.ny;
+    m.ib = (int)floor(fmin(X0, X1));
+    m.jb = (int)floor(fmin(Y0, Y1));
+    double Sx1[4], Sy1[4];
+    shape4(X0, m.ib, m.Sx0);
+    shape4(X1, m.ib, Sx1);
+    shape4(Y0, m.jb, m.Sy0);
+    shape4(Y1, m.jb, Sy1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        m.DSx[k] = Sx1[k] - m.Sx0[k];
+        m.DSy[k] = Sy1[k] - m.Sy0[k];
+    }
+    const double qw = a.qm * (a.w ? (double)a.w[i] : a.w_uniform);
+    m.cJx = -qw / (a.dt * a.dy) * scale[0];
+    m.cJy = -qw / (a.dt * a.dx) * scale[1];
+    m.cJz = qw * (double)a.uz[i] / (a.dx * a.dy) * scale[2];
+}
+
+// Calls add(q, k, b, v) for every non-zero fixed-point contribution v of
+// component q (0 Jx, 1 Jy, 2 Jz) at stencil node (ib-1+k, jb-1+b).
+template <class Add>
+__device__ __forceinline__ void esk_emit(const EskMove& m, Add&& add) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const double sy = m.Sy0[b] + 0.5 * m.DSy[b];
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc += m.DSx[k] * sy;
+            const unsigned long long v = fixd(m.cJx * acc);
+            if (v) add(0, k, b, v);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double sx = m.Sx0[k] + 0.5 * m.DSx[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc += m.DSy[b] * sx;
+            const unsigned long long v = fixd(m.cJy * acc);
+            if (v) add(1, k, b, v);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double wz = m.Sx0[k] * m.Sy0[b] +
+                              0.5 * (m.DSx[k] * m.Sy0[b] + m.Sx0[k] * m.DSy[b]) +
+                              m.DSx[k] * m.DSy[b] * (1.0 / 3.0);
+            const unsigned long long v = fixd(m.cJz * wz);
+            if (v) add(2, k, b, v);
+        }
+}
+
+__device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
+    int S[3];
+    esk_exponents(a, S);
+    for (int q = 0; q < 3; ++q) scale[q] = ldexp(1.0, S[q]);
+}
+
+__global__ void __launch_bounds__(kThreads) esk_zero_bound_kernel(unsigned int* bound) {
+    if (threadIdx.x < 2) bound[threadIdx.x] = 0u;
+}
+
+// max |w| and max |w uz| (max |uz| for a uniform weight) as positive-float
+// bit patterns, whose unsigned order is the value order.
+__global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, unsigned int* bound) {
+    float mw = 0.f, mwu = 0.f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += blockDim.x * gridDim.x) {
+        const float uz = fabsf(a.uz[i]);
+        if (a.w) {
+            const float w = fabsf(a.w[i]);
+            mw = fmaxf(mw, w);
+            mwu = fmaxf(mwu, w * uz);
+        } else {
+            mwu = fmaxf(mwu, uz);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mw = fmaxf(mw, __shfl_xor(mw, off, 64));
+        mwu = fmaxf(mwu, __shfl_xor(mwu, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&bound[0], __float_as_uint(mw));
+        atomicMax(&bound[1], __float_as_uint(mwu));
+    }
+}
+
+__device__ __forceinline__ void global_add(unsigned long long* acc, int nx, int ny, int64_t nc,
+                                           int q, int i, int j, unsigned long long v) {
+    atomicAdd(&acc[q * nc + (int64_t)dev::wrapfar(j, ny) * nx + dev::wrapfar(i, nx)], v);
+}
+
+__global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
+                                                              unsigned long long* __restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double scale[3];
+    esk_scales(a, scale);
+    const int64_t nc = (int64_t)a.nx * a.ny;
+    EskMove m;
+    esk_prepare(a, i, scale, m);
+    esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+        global_add(acc, a.nx, a.ny, nc, q, m.ib - 1 + k, m.jb - 1 + b, v);
+    });
+}
+
+// whole grid (3 components) privatised in LDS per resident workgroup
+__global__ void __launch_bounds__(kThreads) esk_ldsgrid_kernel(EskArgs a,
+                                                               unsigned long long* __restrict__ acc) {
+    extern __shared__ unsigned long long s_j[];
+    const int nc = a.nx * a.ny;
+    for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) s_j[c] = 0ull;
+    __syncthreads();
+    double scale[3];
+    esk_scales(a, scale);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += blockDim.x * gridDim.x) {
+        EskMove m;
+        esk_prepare(a, i, scale, m);
+        esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+            const int gi = dev::wrapfar(m.ib - 1 + k, a.nx);
+            const int gj = dev::wrapfar(m.jb - 1 + b, a.ny);
+            atomicAdd(&s_j[q * nc + gj * a.nx + gi], v);
+        });
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) {
+        const unsigned long long v = s_j[c];
+        if (v) atomicAdd(&acc[c], v);
+    }
+}
+
+struct EskTiles {
+    const int32_t* off;  // ntiles+1, tile bins of the old positions
+    const int32_t* idx;  // bin permutation
+    int ntx, tx, ty, W, H, bpt;
+};
+
+__device__ __forceinline__ int wrap_local(int l, int n) {
+    if (l < 0) l += n;
+    else if (l >= n) l -= n;
+    return l;
+}
+
+// workgroup per (tile, part): the tile's J plus a 2-node apron in LDS
+__global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles t,
+                                                             unsigned long long* __restrict__ acc) {
+    extern __shared__ unsigned long long s_j[];
+    const int R = t.W * t.H;
+    const int tile = blockIdx.x / t.bpt, part = blockIdx.x % t.bpt;
+    const int orgx = (tile % t.ntx) * t.tx - kApron, orgy = (tile / t.ntx) * t.ty - kApron;
+    for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) s_j[c] = 0ull;
+    __syncthreads();
+    double scale[3];
+    esk_scales(a, scale);
+    const int64_t nc = (int64_t)a.nx * a.ny;
+    const int beg = t.off[tile], end = t.off[tile + 1];
+    const int chunk = (end - beg + t.bpt - 1) / t.bpt;
+    const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+        const int i = t.idx[r];
+        EskMove m;
+        esk_prepare(a, i, scale, m);
+        const int li = wrap_local(m.ib - 1 - orgx, a.nx), lj = wrap_local(m.jb - 1 - orgy, a.ny);
+        if (dev::in_tile(li, t.W - 4) && dev::in_tile(lj, t.H - 4)) {
+            const int base = lj * t.W + li;
+            esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+                atomicAdd(&s_j[q * R + base + b * t.W + k], v);
+            });
+        } else {
+            esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+                global_add(acc, a.nx, a.ny, nc, q, m.ib - 1 + k, m.jb - 1 + b, v);
+            });
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) {
+        const unsigned long long v = s_j[c];
+        if (v) {
+            const int q = c / R, l = c % R;
+            const int gx = dev::wrapfar(orgx + l % t.W, a.nx);
+            const int gy = dev::wrapfar(orgy + l / t.W, a.ny);
+            atomicAdd(&acc[q * nc + (int64_t)gy * a.nx + gx], v);
+        }
+    }
+}
+
+// J_q[c] += acc_q[c] * 2^-S_q
+__global__ void __launch_bounds__(kThreads) esk_convert_kernel(EskArgs a,
+                                                               const unsigned long long* __restrict__ acc,
+                                                               float* jx, float* jy, float* jz) {
+    const int64_t nc = (int64_t)a.nx * a.ny;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int S[3];
+    esk_exponents(a, S);
+    float* out[3] = {jx, jy, jz};
+    for (int q = 0; q < 3; ++q) {
+        const long long v = (long long)acc[q * nc + c];
+        out[q][c] += (float)ldexp((double)v, -S[q]);
+    }
+}
+
+}  // namespace
+
+int tile_bpt(const aw_ctx& c, int64_t n);
+
+void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
+                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
+                       hipStream_t s) {
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    AW_REQUIRE(dt > 0, "dt must be positive");
+    const int64_t nc = ncell(c);
+    c.jacc.reserve((size_t)(3 * nc));
+    AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
+    c.jbound.reserve(2);
+    EskArgs a{};
+    a.x0 = x0;
+    a.y0 = y0;
+    a.x1 = p.x;
+    a.y1 = p.y;
+    a.uz = p.uz;
+    a.w = p.w;
+    a.n = (int)p.n;
+    a.nx = c.g.nx;
+    a.ny = c.g.ny;
+    a.qm = qm;
+    a.w_uniform = p.w_uniform;
+    a.dt = dt;
+    a.dx = c.g.dx;
+    a.dy = c.g.dy;
+    a.bound = c.jbound.get();
+    unsigned long long* acc = c.jacc.get();
+    esk_zero_bound_kernel<<<1, 64, 0, s>>>(c.jbound.get());
+    AW_LAUNCH_CHECK();
+    if (p.n > 0) {
+        esk_bound_kernel<<<std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)), kThreads, 0, s>>>(
+            a, c.jbound.get());
+        AW_LAUNCH_CHECK();
+        if (policy == AW_DEPOSIT_AUTO) {
+            if ((size_t)(3 * nc) * 8 <= c.lds_per_block && p.n >= 16 * nc)
+                policy = AW_DEPOSIT_LDS_GRID;
+            else
+                policy = (double)p.n / (double)nc >= 8.0 ? AW_DEPOSIT_BINNED : AW_DEPOSIT_GLOBAL;
+        }
+        if (policy == AW_DEPOSIT_GLOBAL) {
+            esk_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, acc);
+            AW_LAUNCH_CHECK();
+        } else if (policy == AW_DEPOSIT_LDS_GRID) {
+            const size_t sh = (size_t)(3 * nc) * 8;
+            AW_REQUIRE(sh <= c.lds_per_block, "grid does not fit LDS for AW_DEPOSIT_LDS_GRID");
+            int per_cu = 0;
+            AW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel), kThreads, sh));
+            const int nb = std::min(std::max(1, per_cu) * std::max(1, c.num_cu), blocks_for(p.n));
+            esk_ldsgrid_kernel<<<nb, kThreads, sh, s>>>(a, acc);
+            AW_LAUNCH_CHECK();
+        } else if (policy == AW_DEPOSIT_BINNED) {
+            // bin by the old position (stable), as deposit_rho_fixed does
+            aw_particles old = p;
+            old.x = const_cast<float*>(x0);
+            old.y = const_cast<float*>(y0);
+            c.sort.keyA.reserve((size_t)p.n);
+            c.sort.bin_off.reserve((size_t)c.ntiles + 1);
+            c.sort.bin_perm.reserve((size_t)p.n);
+            compute_tile_keys(c, old, c.sort.keyA.get(), false, s);
+            stable_sort_keys(c, p.n, (uint32_t)c.ntiles, c.sort.bin_perm.get(),
+                             c.sort.bin_off.get(), s);
+            EskTiles t{};
+            t.off = c.sort.bin_off.get();
+            t.idx = c.sort.bin_perm.get();
+            t.ntx = c.ntx;
+            t.tx = c.tx;
+            t.ty = c.ty;
+            t.W = c.tx + 2 * kApron;
+            t.H = c.ty + 2 * kApron;
+            t.bpt = tile_bpt(c, p.n);
+            const size_t sh = (size_t)3 * t.W * t.H * 8;
+            esk_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, acc);
+            AW_LAUNCH_CHECK();
+        } else {
+            fail(AW_ERR_INVALID, "unknown deposit policy " + std::to_string(policy), __FILE__,
+                 __LINE__);
+        }
+    }
+    esk_convert_kernel<<<blocks_for(nc), kThreads, 0, s>>>(a, acc, jx, jy, jz);
+    AW_LAUNCH_CHECK();
+}
+
+}  // namespace aw

@@ -9,7 +9,8 @@
 //                         the work of boris_push_tiled_kernel (:154-202) +
 //                         particle_migrate_kernel (particles.hpp:293-315) +
 //                         the next step's deposit_charge_tiled_kernel
-//                         (depositor.hpp:112-153) for 40 B of particle traffic.
+//                         (depositor.hpp:112-153) for 40 B of particle traffic
+//                         (32 B when B0 is along z: uz is then invariant).
 #include "aw_internal.hpp"
 
 namespace aw {
@@ -90,8 +91,10 @@ struct TileFrame {
 };
 
 // Everything after the loads for one particle: gather E^n, Boris, move,
-// wrap, store, deposit rho(x^{n+1}).
-template <bool PER_W, bool KE>
+// wrap, store, deposit rho(x^{n+1}).  UZ = false: B0 is along z (or absent),
+// so uz is a constant of the motion; it is then neither rotated nor stored,
+// and read only for the kinetic energy (KE).
+template <bool PER_W, bool KE, bool UZ>
 __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f, int k, float x,
                                          float y, float ux, float uy, float uz, float wk,
                                          unsigned int& nstray, double& ke) {
@@ -109,14 +112,15 @@ __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f,
         ++nstray;
     }
 
-    dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
+    if constexpr (UZ) dev::boris_es(a.bc, a.has_b0 != 0, ex, ey, ux, uy, uz);
+    else dev::boris_es_bz(a.bc, a.has_b0 != 0, ex, ey, ux, uy);
     const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), (float)a.nx);
     const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), (float)a.ny);
     a.x[k] = xn;
     a.y[k] = yn;
     a.ux[k] = ux;
     a.uy[k] = uy;
-    a.uz[k] = uz;
+    if constexpr (UZ) a.uz[k] = uz;
 
     double coef;
     if constexpr (PER_W) {
@@ -151,8 +155,9 @@ __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f,
     }
 }
 
-template <bool PER_W, bool KE>
+template <bool PER_W, bool KE, bool UZ>
 __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
+    constexpr bool kLoadUz = UZ || KE;
     extern __shared__ unsigned long long smem[];
     const int W = a.W, H = a.H, R = W * H;
     TileFrame f;
@@ -184,20 +189,25 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
     int k = r0 + (int)threadIdx.x;
     for (; k + kThreads < r1; k += 2 * kThreads) {
         const int k2 = k + kThreads;
-        const float x0 = a.x[k], y0 = a.y[k], ux0 = a.ux[k], uy0 = a.uy[k], uz0 = a.uz[k];
-        const float x1 = a.x[k2], y1 = a.y[k2], ux1 = a.ux[k2], uy1 = a.uy[k2], uz1 = a.uz[k2];
-        float w0 = 0.f, w1 = 0.f;
+        const float x0 = a.x[k], y0 = a.y[k], ux0 = a.ux[k], uy0 = a.uy[k];
+        const float x1 = a.x[k2], y1 = a.y[k2], ux1 = a.ux[k2], uy1 = a.uy[k2];
+        float uz0 = 0.f, uz1 = 0.f, w0 = 0.f, w1 = 0.f;
+        if constexpr (kLoadUz) {
+            uz0 = a.uz[k];
+            uz1 = a.uz[k2];
+        }
         if constexpr (PER_W) {
             w0 = a.w[k];
             w1 = a.w[k2];
         }
-        push_one<PER_W, KE>(a, f, k, x0, y0, ux0, uy0, uz0, w0, nstray, ke);
-        push_one<PER_W, KE>(a, f, k2, x1, y1, ux1, uy1, uz1, w1, nstray, ke);
+        push_one<PER_W, KE, UZ>(a, f, k, x0, y0, ux0, uy0, uz0, w0, nstray, ke);
+        push_one<PER_W, KE, UZ>(a, f, k2, x1, y1, ux1, uy1, uz1, w1, nstray, ke);
     }
     if (k < r1) {
-        float w0 = 0.f;
+        float uz0 = 0.f, w0 = 0.f;
+        if constexpr (kLoadUz) uz0 = a.uz[k];
         if constexpr (PER_W) w0 = a.w[k];
-        push_one<PER_W, KE>(a, f, k, a.x[k], a.y[k], a.ux[k], a.uy[k], a.uz[k], w0, nstray, ke);
+        push_one<PER_W, KE, UZ>(a, f, k, a.x[k], a.y[k], a.ux[k], a.uy[k], uz0, w0, nstray, ke);
     }
 
     // per-workgroup stray count and kinetic-energy partial
@@ -234,6 +244,7 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
 }  // namespace
 
 int tile_bpt(const aw_ctx& c, int64_t n);
+bool uz_evolves(const aw_run_params& rp, bool has_b0);
 
 void push_es(aw_ctx& c, const aw_particles& p, const float* Ex, const float* Ey,
              const aw_run_params& rp, bool has_b0, bool half_back, hipStream_t s) {
@@ -298,12 +309,28 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
     const size_t sh = (size_t)a.W * a.H * (8 + 4 + 4);
     const int nblk = c.ntiles * a.bpt;
     const bool pw = p.w != nullptr, ke = ke_part != nullptr;
-    if (pw && ke) push_deposit_kernel<true, true><<<nblk, kThreads, sh, s>>>(a);
-    else if (pw) push_deposit_kernel<true, false><<<nblk, kThreads, sh, s>>>(a);
-    else if (ke) push_deposit_kernel<false, true><<<nblk, kThreads, sh, s>>>(a);
-    else push_deposit_kernel<false, false><<<nblk, kThreads, sh, s>>>(a);
-    AW_LAUNCH_CHECK();
+    const bool uz = uz_evolves(rp, has_b0);
+    auto launch = [&](auto kernel) {
+        kernel<<<nblk, kThreads, sh, s>>>(a);
+        AW_LAUNCH_CHECK();
+    };
+    if (uz) {
+        if (pw && ke) launch(push_deposit_kernel<true, true, true>);
+        else if (pw) launch(push_deposit_kernel<true, false, true>);
+        else if (ke) launch(push_deposit_kernel<false, true, true>);
+        else launch(push_deposit_kernel<false, false, true>);
+    } else {
+        if (pw && ke) launch(push_deposit_kernel<true, true, false>);
+        else if (pw) launch(push_deposit_kernel<true, false, false>);
+        else if (ke) launch(push_deposit_kernel<false, true, false>);
+        else launch(push_deposit_kernel<false, false, false>);
+    }
     return nblk;
+}
+
+// uz changes in the ES push only through a B0 component in the plane
+bool uz_evolves(const aw_run_params& rp, bool has_b0) {
+    return has_b0 && (rp.B0[0] != 0.0f || rp.B0[1] != 0.0f);
 }
 
 }  // namespace aw

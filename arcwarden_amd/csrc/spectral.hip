@@ -107,8 +107,25 @@ void init_fft(aw_ctx& c) {
     AW_FFT(rocfft_execution_info_create(&c.info));
 }
 
+// Darwin batches: R2C x4 [rho|Jx|Jy|Jz], R2C x7 [dcu(3)|amu(4)], C2R x5
+// [ELx|ELy|Bx|By|Bz] (C2R x3 [ETx|ETy|ETz] is shared with the base set).
+void enable_em_fft(aw_ctx& c) {
+    if (c.r2c4) return;
+    const double inv = 1.0 / ((double)c.g.nx * (double)c.g.ny);
+    c.r2c4 = make_plan(c, true, 4, 1.0);
+    c.r2c7 = make_plan(c, true, 7, 1.0);
+    c.c2r5 = make_plan(c, false, 5, inv);
+    size_t work = c.fft_work.size();
+    for (rocfft_plan p : {c.r2c4, c.r2c7, c.c2r5}) {
+        size_t w = 0;
+        AW_FFT(rocfft_plan_get_work_buffer_size(p, &w));
+        work = std::max(work, w);
+    }
+    if (work > c.fft_work.size()) c.fft_work.resize(work);
+}
+
 void destroy_fft(aw_ctx& c) noexcept {
-    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3})
+    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7, &c.c2r5})
         if (*p) {
             rocfft_plan_destroy(*p);
             *p = nullptr;
@@ -130,17 +147,147 @@ void fft_exec(aw_ctx& c, rocfft_plan plan, void* in, void* out, hipStream_t s) {
 
 // rho -> (Ex, Ey) with E2 = [Ex | Ey] contiguous (2*nx*ny floats): one R2C,
 // one k-space kernel, one batched (2) C2R.
+// rocFFT may overwrite the input of an out-of-place transform: the R2C always
+// reads the context's copy (c.rin), filled here unless the caller wrote it.
+static float* r2c_input(aw_ctx& c, const float* rho, hipStream_t s) {
+    c.rin.reserve((size_t)ncell(c));
+    if (rho != c.rin.get())
+        AW_HIP(hipMemcpyAsync(c.rin.get(), rho, (size_t)ncell(c) * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+    return c.rin.get();
+}
+
 void solve_es_contig(aw_ctx& c, const float* rho, float* E2, double eps0, hipStream_t s) {
     const int64_t nk = nkmodes(c);
     c.ktmp.reserve((size_t)6 * nk);
     float2* rk = reinterpret_cast<float2*>(c.ktmp.get());
     float2* ek = rk + nk;  // [Ex_k | Ey_k]
-    fft_exec(c, c.r2c, const_cast<float*>(rho), rk, s);
+    fft_exec(c, c.r2c, r2c_input(c, rho, s), rk, s);
     KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
              6.28318530717958647692 / c.g.Ly};
     es_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(rk, ek, ek + nk, kg, eps0);
     AW_LAUNCH_CHECK();
     fft_exec(c, c.c2r2, ek, E2, s);
+}
+
+// ---- Darwin k-space ---------------------------------------------------------------
+
+namespace {
+
+// E_L (es_poisson_field_kernel, solver_es.hpp:39-78) and B_k = i mu0 (k x J)/k^2
+// (darwin_b_field_kernel, solver_darwin.hpp:38-75) from [rho|Jx|Jy|Jz]_k in
+// one pass.  The uniform background B0 (add_b0_kernel, pump.hpp:50-54) enters
+// as the DC mode of B: the C2R scale 1/(nx*ny) turns B0*nx*ny back into B0.
+__global__ void darwin_elb_kernel(const float2* __restrict__ src, float2* __restrict__ out,
+                                  KGeom kg, double eps0, double c, float b0x, float b0y,
+                                  float b0z) {
+    const int nk = kg.nkx * kg.ny;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    const int i = t % kg.nkx, j = t / kg.nkx;
+    double kx, ky;
+    kvec(kg, i, j, kx, ky);
+    const double k2 = kx * kx + ky * ky;
+    const bool nyq = nyquist(kg, i, j);
+    const float2 r = src[t], jx = src[nk + t], jy = src[2 * nk + t], jz = src[3 * nk + t];
+    // longitudinal E
+    const double g = k2 > 0.0 ? 1.0 / (eps0 * k2) : 0.0;
+    const double pr = (double)r.x * g, pi = (double)r.y * g;
+    float2 ex = make_float2(0.f, 0.f), ey = make_float2(0.f, 0.f);
+    if (!nyq) {
+        ex = make_float2((float)(kx * pi), (float)(-kx * pr));
+        ey = make_float2((float)(ky * pi), (float)(-ky * pr));
+    }
+    // magnetic field
+    float2 bx = make_float2(0.f, 0.f), by = bx, bz = bx;
+    if (!nyq && k2 > 0.0) {
+        const double G = 1.0 / (eps0 * c * c * k2);
+        bx = make_float2((float)(-G * ky * jz.y), (float)(G * ky * jz.x));
+        by = make_float2((float)(G * kx * jz.y), (float)(-G * kx * jz.x));
+        const double cr = kx * jy.x - ky * jx.x, ci = kx * jy.y - ky * jx.y;
+        bz = make_float2((float)(-G * ci), (float)(G * cr));
+    } else if (t == 0) {
+        const float nn = (float)kg.nx * (float)kg.ny;
+        bx = make_float2(b0x * nn, 0.f);
+        by = make_float2(b0y * nn, 0.f);
+        bz = make_float2(b0z * nn, 0.f);
+    }
+    out[t] = ex;
+    out[nk + t] = ey;
+    out[2 * nk + t] = bx;
+    out[3 * nk + t] = by;
+    out[4 * nk + t] = bz;
+}
+
+// darwin_et_kernel (solver_darwin.hpp:83-129): dJ/dt = dcu - i S(amu), transverse
+// projection, E_T = -G_et (dJ/dt)_T with G_et = 1/(eps0 c^2 k^2 + n0).
+__global__ void darwin_et_kernel(const float2* __restrict__ src, float2* __restrict__ out,
+                                 KGeom kg, double eps0, double c, double n0) {
+    const int nk = kg.nkx * kg.ny;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk) return;
+    const int i = t % kg.nkx, j = t / kg.nkx;
+    double kx, ky;
+    kvec(kg, i, j, kx, ky);
+    const double k2 = kx * kx + ky * ky;
+    float2 ex = make_float2(0.f, 0.f), ey = ex, ez = ex;
+    if (!nyquist(kg, i, j) && k2 > 0.0) {
+        const double G = 1.0 / (eps0 * c * c * k2 + n0);
+        const float2 dcx = src[t], dcy = src[nk + t], dcz = src[2 * nk + t];
+        const float2 a0 = src[3 * nk + t], a1 = src[4 * nk + t], a2 = src[5 * nk + t],
+                     a3 = src[6 * nk + t];
+        const double Sxr = kx * a0.x + ky * a1.x, Sxi = kx * a0.y + ky * a1.y;
+        const double Syr = kx * a1.x - ky * a0.x, Syi = kx * a1.y - ky * a0.y;
+        const double Szr = kx * a2.x + ky * a3.x, Szi = kx * a2.y + ky * a3.y;
+        double Jxr = dcx.x + Sxi, Jxi = dcx.y - Sxr;
+        double Jyr = dcy.x + Syi, Jyi = dcy.y - Syr;
+        const double Jzr = dcz.x + Szi, Jzi = dcz.y - Szr;
+        const double inv = 1.0 / k2;
+        const double kdr = (kx * Jxr + ky * Jyr) * inv, kdi = (kx * Jxi + ky * Jyi) * inv;
+        Jxr -= kx * kdr;
+        Jxi -= kx * kdi;
+        Jyr -= ky * kdr;
+        Jyi -= ky * kdi;
+        ex = make_float2((float)(-G * Jxr), (float)(-G * Jxi));
+        ey = make_float2((float)(-G * Jyr), (float)(-G * Jyi));
+        ez = make_float2((float)(-G * Jzr), (float)(-G * Jzi));
+    }
+    out[t] = ex;
+    out[nk + t] = ey;
+    out[2 * nk + t] = ez;
+}
+
+}  // namespace
+
+// src4 = [rho|Jx|Jy|Jz] (float, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz].
+void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
+                      const float B0[3], hipStream_t s) {
+    const int64_t nk = nkmodes(c);
+    c.ktmp.reserve((size_t)2 * 12 * nk);
+    float2* k4 = reinterpret_cast<float2*>(c.ktmp.get());
+    float2* k5 = k4 + 4 * nk;
+    fft_exec(c, c.r2c4, const_cast<float*>(src4), k4, s);
+    KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
+             6.28318530717958647692 / c.g.Ly};
+    darwin_elb_kernel<<<blocks_for(nk), kThreads, 0, s>>>(k4, k5, kg, eps0, cl, B0[0], B0[1],
+                                                         B0[2]);
+    AW_LAUNCH_CHECK();
+    fft_exec(c, c.c2r5, k5, out5, s);
+}
+
+// src7 = [dcux|dcuy|dcuz|amu0..3] -> out3 = [ETx|ETy|ETz].
+void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
+                     double n0, hipStream_t s) {
+    const int64_t nk = nkmodes(c);
+    c.ktmp.reserve((size_t)2 * 12 * nk);
+    float2* k7 = reinterpret_cast<float2*>(c.ktmp.get());
+    float2* k3 = k7 + 7 * nk;
+    fft_exec(c, c.r2c7, const_cast<float*>(src7), k7, s);
+    KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
+             6.28318530717958647692 / c.g.Ly};
+    darwin_et_kernel<<<blocks_for(nk), kThreads, 0, s>>>(k7, k3, kg, eps0, cl, n0);
+    AW_LAUNCH_CHECK();
+    fft_exec(c, c.c2r3, k3, out3, s);
 }
 
 // General form with separate output buffers.
@@ -150,7 +297,7 @@ void solve_es(aw_ctx& c, const float* rho, float* Ex, float* Ey, double eps0, hi
     float2* rk = reinterpret_cast<float2*>(c.ktmp.get());
     float2* exk = rk + nk;
     float2* eyk = exk + nk;
-    fft_exec(c, c.r2c, const_cast<float*>(rho), rk, s);
+    fft_exec(c, c.r2c, r2c_input(c, rho, s), rk, s);
     KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
              6.28318530717958647692 / c.g.Ly};
     es_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(rk, exk, eyk, kg, eps0);

@@ -176,6 +176,15 @@ class Simulation:
         L.call("aw_sim_sources", self._h, C.byref(s))
         return d2h(s.rho, self.grid.nx * self.grid.ny).reshape(self.grid.ny, self.grid.nx)
 
+    def sources_all(self) -> dict:
+        """Host copies of every allocated source grid (rho, J, dcu, amu)."""
+        self.synchronize()
+        s = L.AwSources()
+        L.call("aw_sim_sources", self._h, C.byref(s))
+        nc = self.grid.nx * self.grid.ny
+        return {k: d2h(getattr(s, k), nc).reshape(self.grid.ny, self.grid.nx)
+                for k, _ in L.AwSources._fields_ if getattr(s, k)}
+
     def history(self) -> list[dict]:
         cnt = C.c_int64()
         L.call("aw_sim_history", self._h, None, 0, C.byref(cnt))
@@ -196,3 +205,9 @@ class Simulation:
         ms, n, s = C.c_double(), C.c_int64(), C.c_int64()
         L.call("aw_sim_kernel_time", self._h, C.byref(ms), C.byref(n), C.byref(s))
         return ms.value, n.value, s.value
+
+    def exec_config(self) -> dict:
+        """Tile size and workgroups per tile in use (after autotuning)."""
+        tx, ty, b = C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("aw_sim_exec_config", self._h, C.byref(tx), C.byref(ty), C.byref(b))
+        return {"tile_x": tx.value, "tile_y": ty.value, "blocks_per_tile": b.value}

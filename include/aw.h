@@ -141,6 +141,44 @@ typedef struct aw_diag_sample {
 #define AW_MODEL_ES 0
 #define AW_MODEL_DARWIN 1
 
+/* ---- input deck (Deck / load_deck, include/pic/deck.hpp:31-274) ------------- */
+
+#define AW_DECK_MAX_SPECIES 16
+#define AW_DECK_MAX_PROBES 64
+
+/* Deck, member for member; std::string members as fixed NUL-terminated
+ * buffers (diag_enable: the module keys joined by single spaces), the
+ * species list and the probe list as counted arrays. */
+typedef struct aw_deck {
+    int32_t nx, ny;
+    double Lx, Ly;
+    aw_run_params rp;
+    int32_t nspecies;
+    aw_species species[AW_DECK_MAX_SPECIES];
+    int64_t dump_every;
+    char outdir[256];
+    int32_t darwin, yee;
+    double dx_wpe_c, c_direct, wce, theta_deg;
+    int32_t b0_direct, b0_xc_set;
+    double bnoise;
+    int32_t pump_enable, pump_M;
+    double pump_w0, pump_ex0, pump_ey0, pump_ez0, pump_amp, pump_trmp, pump_toff;
+    char diag_enable[256];
+    int32_t nprobes;
+    double probes[AW_DECK_MAX_PROBES];
+    double tsnap;
+    int32_t band_lo, band_hi, kt_stride, n_frames;
+    char prefix[128];
+} aw_deck;
+
+/* load_deck (include/pic/deck.hpp:99-274): parses the INI deck at path and
+ * applies the finalize step (c from dx_wpe_c, B0 from wce/theta_deg, pump
+ * amplitudes and k0, B0-profile centre).  Host only (no GPU work).  Errors
+ * carry the reference's messages ("deck: cannot open <path>", "deck: bad
+ * section at line N", ...).  The dipole-profile polynomial fit (Yee-branch
+ * input, bg::fit_dipole) is not computed: rp.b0_c keeps its defaults. */
+aw_status aw_deck_load(const char* path, aw_deck* out);
+
 /* ---- errors -------------------------------------------------------------- */
 
 /* Copies the calling thread's last error message (NUL-terminated). */
@@ -218,6 +256,20 @@ aw_status aw_load_legacy(aw_ctx* ctx, const aw_run_params* rp, aw_particles* p,
 aw_status aw_deposit_rho(aw_ctx* ctx, const aw_particles* p, double qm, int32_t policy,
                          float* rho, aw_stream s);
 
+/* Esirkepov charge-conserving current, the deposit half of k_push_esirkepov /
+ * k_push_esirkepov_tiled with esirkepov_scatter + GlobalJSink
+ * (include/pic/yee2d.hpp:124-132,310-389,401-469): for the moves
+ * (x0[i], y0[i]) -> (p->x[i], p->y[i]) of less than one cell (cell units; p's
+ * positions may be wrapped or unwrapped, the minimal image is used),
+ * jx += staggered Jx at (i+1/2, j), jy += Jy at (i, j+1/2), jz += nodal
+ * Jz = q w uz <S>_t/(dx dy), q = qm.  Accumulates (caller zeroes).  Charge
+ * conservation holds against aw_deposit_rho of the two positions.  policy =
+ * AW_DEPOSIT_* (BINNED bins by the old position).  Deterministic: int64
+ * fixed point, every policy returns the same bits. */
+aw_status aw_deposit_esirkepov(aw_ctx* ctx, const float* x0, const float* y0,
+                               const aw_particles* p, double qm, double dt, int32_t policy,
+                               float* jx, float* jy, float* jz, aw_stream s);
+
 /* ---- push ---------------------------------------------------------------------- */
 
 /* Pusher<Cfg>::boris (include/pic/pusher.hpp:112-129,265-275): CIC gather of
@@ -255,6 +307,16 @@ aw_status aw_fft_c2r(aw_ctx* ctx, void* in_k, float* out, aw_stream s);
 aw_status aw_solve_es(aw_ctx* ctx, const float* rho, float* Ex, float* Ey, double eps0,
                       aw_stream s);
 
+/* DarwinSpectralSolver::solve_el_b (include/pic/solver_darwin.hpp:188-216) +
+ * add_background_b0 (include/pic/pump.hpp:59-67): src4 = [rho|Jx|Jy|Jz]
+ * (4*nx*ny floats, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz], B = B_self + B0. */
+aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, double eps0,
+                               double c, const float B0[3], aw_stream s);
+/* DarwinSpectralSolver::solve_et (include/pic/solver_darwin.hpp:235-263):
+ * src7 = [dcux|dcuy|dcuz|amu0|amu1|amu2|amu3] -> out3 = [ETx|ETy|ETz]. */
+aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double eps0,
+                             double c, double n0, aw_stream s);
+
 /* ---- diagnostics ------------------------------------------------------------------- */
 
 /* Diagnostics::compute (include/pic/diagnostics.hpp:58-108) as an on-device
@@ -275,7 +337,7 @@ typedef struct aw_sim_config {
     int32_t has_b0;         /* SimConfig::has_b0 */
     int32_t deposit_policy; /* AW_DEPOSIT_* used for init deposits */
     int32_t field_model;    /* AW_MODEL_ES | AW_MODEL_DARWIN */
-    int32_t tile_x, tile_y; /* 0 -> 16 */
+    int32_t tile_x, tile_y; /* 0 -> 8 (ES at >= 64 particles per cell), else 16 */
     int32_t sort_every;     /* re-sort cadence in steps (0 -> adaptive only) */
     double stray_frac;      /* re-sort when strays exceed this fraction (0 -> 0.02) */
     int32_t n_modes;        /* x-displacement modes applied at load */
@@ -283,8 +345,10 @@ typedef struct aw_sim_config {
     int32_t legacy_load;    /* 1: Particles::initialize(rp) path */
     int32_t fixed_scale;    /* 0 -> chosen from the species densities */
     int32_t blocks_per_tile; /* workgroups sharing one tile (0 -> heuristic) */
-    int32_t autotune;       /* 1: time candidate tile / bpt configurations online
-                               during the first steps and keep the fastest */
+    int32_t autotune;       /* 1 (ES model): at aw_sim_init, time the fused
+                               push+deposit for tile 8/16/32 x 1..16 workgroups
+                               per tile and keep the fastest (cached per grid
+                               and particle count for the process) */
 } aw_sim_config;
 
 aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
@@ -315,6 +379,12 @@ aw_status aw_sim_request_sort(aw_sim* sim);
 aw_status aw_sim_set_profiling(aw_sim* sim, int32_t on);
 aw_status aw_sim_kernel_time(aw_sim* sim, double* total_ms, int64_t* launches,
                              int64_t* sorts);
+/* No reference counterpart (the reference fixes 16x16 tiles and picks blocks
+ * per tile from a cached occupancy, include/pic/depositor.hpp:344-364): the
+ * tile size and workgroups per tile in use, after the autotuner when
+ * cfg.autotune was set. */
+aw_status aw_sim_exec_config(aw_sim* sim, int32_t* tile_x, int32_t* tile_y,
+                             int32_t* blocks_per_tile);
 
 #ifdef __cplusplus
 }

@@ -448,6 +448,42 @@ void orc_push_em_f64(const orc_grid* g, double qm, double dt, double* x, double*
     }
 }
 
+/* em_push_tiled_kernel (include/pic/pusher.hpp:211-258) with boris_update_full
+ * (:103-109) as the FP32 sequence the HIP kernel executes (explicit fmaf);
+ * E, B are the gathered grids, move x += u dt/dx, periodic wrap. */
+void orc_push_em_f32(const orc_grid* g, double qm, double dt, float* x, float* y, float* ux,
+                     float* uy, float* uz, int64_t n, const float* Ex, const float* Ey,
+                     const float* Ez, const float* Bx, const float* By, const float* Bz) {
+    const float qmh = (float)(qm * 0.5 * dt);
+    const float cx = (float)(dt / g->dx), cy = (float)(dt / g->dy);
+    const float fnx = (float)g->nx, fny = (float)g->ny;
+    for (int64_t i = 0; i < n; ++i) {
+        const cic_f s = cic_float(g, x[i], y[i]);
+        const float ex = gather_f(Ex, &s), ey = gather_f(Ey, &s), ez = gather_f(Ez, &s);
+        const float bx = gather_f(Bx, &s), by = gather_f(By, &s), bz = gather_f(Bz, &s);
+        float u0 = ux[i], u1 = uy[i], u2 = uz[i];
+        u0 = fmaf(qmh, ex, u0);
+        u1 = fmaf(qmh, ey, u1);
+        u2 = fmaf(qmh, ez, u2);
+        const float tx = qmh * bx, ty = qmh * by, tz = qmh * bz;
+        const float t2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
+        const float sf = 2.0f / (1.0f + t2);
+        const float sx = sf * tx, sy = sf * ty, sz = sf * tz;
+        const float upx = fmaf(u1, tz, fmaf(-u2, ty, u0));
+        const float upy = fmaf(u2, tx, fmaf(-u0, tz, u1));
+        const float upz = fmaf(u0, ty, fmaf(-u1, tx, u2));
+        u0 = fmaf(upy, sz, fmaf(-upz, sy, u0));
+        u1 = fmaf(upz, sx, fmaf(-upx, sz, u1));
+        u2 = fmaf(upx, sy, fmaf(-upy, sx, u2));
+        u0 = fmaf(qmh, ex, u0);
+        u1 = fmaf(qmh, ey, u1);
+        u2 = fmaf(qmh, ez, u2);
+        ux[i] = u0; uy[i] = u1; uz[i] = u2;
+        x[i] = wrap_coord(fmaf(u0, cx, x[i]), fnx);
+        y[i] = wrap_coord(fmaf(u1, cy, y[i]), fny);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* FFT (double, mixed radix, unnormalised like cuFFT / rocFFT)               */
 /* ------------------------------------------------------------------------ */

@@ -84,6 +84,9 @@ void orc_push_es_f64(const orc_grid* g, const orc_push_params* pp, double* x, do
 void orc_half_back_f32(const orc_grid* g, const orc_push_params* pp, const float* x,
                        const float* y, float* ux, float* uy, float* uz, int64_t n,
                        const float* Ex, const float* Ey);
+void orc_push_em_f32(const orc_grid* g, double qm, double dt, float* x, float* y, float* ux,
+                     float* uy, float* uz, int64_t n, const float* Ex, const float* Ey,
+                     const float* Ez, const float* Bx, const float* By, const float* Bz);
 void orc_push_em_f64(const orc_grid* g, double qm, double dt, double* x, double* y,
                      double* ux, double* uy, double* uz, int64_t n,
                      const double* Ex, const double* Ey, const double* Ez,

@@ -157,6 +157,91 @@ def half_back_f32(g, pp, state, Ex, Ey):
     return s
 
 
+def push_em_f32(g, qm, dt, state, E3, B3):
+    s = {k: _f(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    f = [_f(a).ravel() for a in (*E3, *B3)]
+    lib().orc_push_em_f32(C.byref(g), C.c_double(qm), C.c_double(dt),
+                          *(_p(s[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                          C.c_int64(s["x"].size), *(_p(a) for a in f))
+    return s
+
+
+def push_em_f64(g, qm, dt, state, E3, B3, wrap=True):
+    s = {k: _d(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    f = [_d(a).ravel() for a in (*E3, *B3)]
+    lib().orc_push_em_f64(C.byref(g), C.c_double(qm), C.c_double(dt),
+                          *(_p(s[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                          C.c_int64(s["x"].size), *(_p(a) for a in f), C.c_int32(int(wrap)))
+    return s
+
+
+def deposit_rho_j_f64(g, state, qm, w=None, w_uniform=0.0):
+    x, y, ux, uy, uz = (_f(state[k]) for k in ("x", "y", "ux", "uy", "uz"))
+    w = _f(w) if w is not None else None
+    out = [np.zeros(g.nx * g.ny) for _ in range(4)]
+    lib().orc_deposit_rho_j_f64(C.byref(g), _p(x), _p(y), _p(ux), _p(uy), _p(uz), _p(w),
+                                C.c_double(w_uniform), C.c_int64(x.size), C.c_double(qm),
+                                *(_p(a) for a in out))
+    return [a.reshape(g.ny, g.nx) for a in out]
+
+
+def deposit_amu_f64(g, state, qm, w=None, w_uniform=0.0):
+    x, y, ux, uy, uz = (_f(state[k]) for k in ("x", "y", "ux", "uy", "uz"))
+    w = _f(w) if w is not None else None
+    out = [np.zeros(g.nx * g.ny) for _ in range(4)]
+    lib().orc_deposit_amu_f64(C.byref(g), _p(x), _p(y), _p(ux), _p(uy), _p(uz), _p(w),
+                              C.c_double(w_uniform), C.c_int64(x.size), C.c_double(qm),
+                              *(_p(a) for a in out))
+    return [a.reshape(g.ny, g.nx) for a in out]
+
+
+def deposit_dcu_f64(g, state, qm, E3, B3, w=None, w_uniform=0.0):
+    x, y, ux, uy, uz = (_f(state[k]) for k in ("x", "y", "ux", "uy", "uz"))
+    w = _f(w) if w is not None else None
+    f = [_f(a).ravel() for a in (*E3, *B3)]
+    out = [np.zeros(g.nx * g.ny) for _ in range(3)]
+    lib().orc_deposit_dcu_f64(C.byref(g), _p(x), _p(y), _p(ux), _p(uy), _p(uz), _p(w),
+                              C.c_double(w_uniform), C.c_int64(x.size), C.c_double(qm),
+                              *(_p(a) for a in f), *(_p(a) for a in out))
+    return [a.reshape(g.ny, g.nx) for a in out]
+
+
+def esirkepov_f64(g, x0, y0, x1, y1, uz, qm, dt, w=None, w_uniform=0.0):
+    """Charge-conserving current of moves (x0, y0) -> (x1, y1), |step| < 1 cell:
+    staggered Jx (i+1/2, j), Jy (i, j+1/2), nodal Jz, each [ny][nx]."""
+    x0, y0, x1, y1, uz = (_f(a) for a in (x0, y0, x1, y1, uz))
+    w = _f(w) if w is not None else None
+    out = [np.zeros(g.nx * g.ny) for _ in range(3)]
+    lib().orc_esirkepov_f64(C.byref(g), _p(x0), _p(y0), _p(x1), _p(y1), _p(uz), _p(w),
+                            C.c_double(w_uniform), C.c_int64(x0.size), C.c_double(qm),
+                            C.c_double(dt), *(_p(a) for a in out))
+    return [a.reshape(g.ny, g.nx) for a in out]
+
+
+def rho_nodes_f64(g, x, y, qm, w=None, w_uniform=0.0):
+    x, y = _f(x), _f(y)
+    w = _f(w) if w is not None else None
+    rho = np.zeros(g.nx * g.ny, np.float64)
+    lib().orc_rho_nodes_f64(C.byref(g), _p(x), _p(y), _p(w), C.c_double(w_uniform),
+                            C.c_int64(x.size), C.c_double(qm), _p(rho))
+    return rho.reshape(g.ny, g.nx)
+
+
+def solve_darwin_b_f64(g, jx, jy, jz, eps0=1.0, c=1.0):
+    out = [np.empty((g.ny, g.nx)) for _ in range(3)]
+    lib().orc_solve_darwin_b_f64(C.byref(g), C.c_double(eps0), C.c_double(c), _p(_d(jx)),
+                                 _p(_d(jy)), _p(_d(jz)), *(_p(a) for a in out))
+    return out
+
+
+def solve_darwin_et_f64(g, dcu3, amu4, eps0=1.0, c=1.0, n0=1.0):
+    out = [np.empty((g.ny, g.nx)) for _ in range(3)]
+    src = [_d(a) for a in (*dcu3, *amu4)]
+    lib().orc_solve_darwin_et_f64(C.byref(g), C.c_double(eps0), C.c_double(c), C.c_double(n0),
+                                  *(_p(a) for a in src), *(_p(a) for a in out))
+    return out
+
+
 # ---- spectral ----------------------------------------------------------------------
 
 def fft2_r2c(field):

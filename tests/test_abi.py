@@ -48,9 +48,10 @@ def test_struct_layouts_match_header_sizes():
     src = r"""
 #include <stdio.h>
 #include "aw.h"
-int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
  sizeof(aw_run_params), sizeof(aw_species), sizeof(aw_particles), sizeof(aw_fields),
- sizeof(aw_sources), sizeof(aw_diag_sample), sizeof(aw_sim_config));return 0;}
+ sizeof(aw_sources), sizeof(aw_diag_sample), sizeof(aw_sim_config), sizeof(aw_deck));
+ return 0;}
 """
     with tempfile.TemporaryDirectory() as d:
         c = Path(d) / "probe.c"
@@ -61,5 +62,5 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
                                                 text=True).stdout.split()]
     py = [ctypes.sizeof(t) for t in (_lib.AwGrid, _lib.AwRunParams, _lib.AwSpecies,
                                      _lib.AwParticles, _lib.AwFields, _lib.AwSources,
-                                     _lib.AwDiagSample, _lib.AwSimConfig)]
+                                     _lib.AwDiagSample, _lib.AwSimConfig, _lib.AwDeck)]
     assert py == sizes

@@ -1,0 +1,209 @@
+"""GPU tests of the spectral Darwin path: closed-form solves, one step against
+the FP64 oracle (sources, fields, particle state), the EM push bit for bit
+against the host FP32 restatement, the drifting-beam current, and the
+Weibel growth rate."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from arcwarden_amd import _lib as L
+from arcwarden_amd.sim import SimConfig, Simulation
+
+import physics
+from gpu_util import Ctx, dev, host, rel_l2, stream
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * math.pi
+
+
+def test_darwin_magnetostatic_closed_form(gpu):
+    """tests/test_darwin_magnetostatic.cu part 1: Jz = cos x -> By = sin x."""
+    nx, ny = 64, 32
+    g = L.grid(nx, ny, TWO_PI, TWO_PI)
+    x = np.arange(nx) * g.dx
+    src = np.zeros((4, ny, nx), np.float32)
+    src[3] = np.cos(x)[None, :]
+    with Ctx(g) as ctx:
+        out = torch.empty(5 * nx * ny, dtype=torch.float32, device="cuda")
+        b0 = (C.c_float * 3)(0, 0, 0)
+        L.call("aw_solve_darwin_el_b", ctx.h, C.c_void_p(dev(src).data_ptr()),
+               C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(1.0), b0, stream())
+        o = host(out).reshape(5, ny, nx)
+        assert np.max(np.abs(o[3] - np.sin(x)[None, :])) < 1e-4
+        assert np.max(np.abs(o[2])) < 1e-4 and np.max(np.abs(o[4])) < 1e-4
+        assert np.max(np.abs(o[:2])) < 1e-6          # rho = 0 -> E_L = 0
+        # background field enters as a uniform offset
+        b0 = (C.c_float * 3)(0.25, 0.0, -0.5)
+        L.call("aw_solve_darwin_el_b", ctx.h, C.c_void_p(dev(src).data_ptr()),
+               C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(1.0), b0, stream())
+        o = host(out).reshape(5, ny, nx)
+        assert np.max(np.abs(o[2] - 0.25)) < 1e-5
+        assert np.max(np.abs(o[4] + 0.5)) < 1e-5
+
+
+def test_darwin_et_closed_form(gpu):
+    nx = ny = 32
+    g = L.grid(nx, ny, TWO_PI, TWO_PI)
+    X, Y = np.meshgrid(np.arange(nx) * g.dx, np.arange(ny) * g.dy)
+    src = np.zeros((7, ny, nx), np.float32)
+    src[0] = np.cos(Y)
+    src[1] = np.cos(X)
+    c, n0 = 3.0, 1.0
+    with Ctx(g) as ctx:
+        out = torch.empty(3 * nx * ny, dtype=torch.float32, device="cuda")
+        L.call("aw_solve_darwin_et", ctx.h, C.c_void_p(dev(src).data_ptr()),
+               C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(c), C.c_double(n0),
+               stream())
+        o = host(out).reshape(3, ny, nx)
+        G = 1.0 / (c * c + n0)
+        assert np.max(np.abs(o[0] + G * np.cos(Y))) < 1e-5
+        assert np.max(np.abs(o[1] + G * np.cos(X))) < 1e-5
+
+
+def darwin_sim(nx=64, ny=32, ppc=32, pump=True, sort_every=0, stray_frac=1.0, dump_every=1,
+               c=5.0, B0=(0.4, 0.0, 0.23), seed=11, v0=0.3, uth=0.05):
+    g = L.grid(nx, ny, nx * 0.5, ny * 0.5)
+    rp = L.run_params(dt=0.05, qm=-1.0, eps0=1.0, c=c, n0=1.0, noisy_load=1, rng_seed=seed,
+                      B0=B0, dump_every=dump_every)
+    if pump:
+        rp.pump = 1
+        rp.pump_ex, rp.pump_ey, rp.pump_ez = 1.39e-3, 1.81e-3, -1.81e-3
+        rp.pump_k0 = TWO_PI * 3 / (nx * 0.5)
+        rp.pump_w0 = 0.15
+        rp.pump_trmp, rp.pump_toff = 2.0, 100.0
+    sp = [L.species("beam_p", 0.5, ppc // 2, (uth,) * 3, (0, 0, v0)),
+          L.species("beam_m", 0.5, ppc // 2, (uth,) * 3, (0, 0, -v0))]
+    cfg = SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN, sort_every=sort_every,
+                    stray_frac=stray_frac)
+    return Simulation(g, rp, sp, cfg), g, rp
+
+
+def _pump_grid(rp, g, t):
+    if not rp.pump or t >= rp.pump_toff:
+        return [np.zeros((g.ny, g.nx))] * 3
+    x = np.arange(g.nx) * g.dx
+    th = rp.pump_k0 * x - rp.pump_w0 * t
+    r = t / rp.pump_trmp if t < rp.pump_trmp else (
+        (rp.pump_toff - t) / rp.pump_trmp if t > rp.pump_toff - rp.pump_trmp else 1.0)
+    return [np.tile(v, (g.ny, 1)) for v in (rp.pump_ex * np.cos(th) * r,
+                                           -rp.pump_ey * np.sin(th) * r,
+                                           rp.pump_ez * np.cos(th) * r)]
+
+
+def test_darwin_one_step_matches_fp64_oracle(gpu, oracle):
+    sim, g, rp = darwin_sim()
+    sim.init()
+    s0 = sim.particles()
+    w = s0["w_uniform"]
+    n_step = 3
+    # advance a few steps so E_T, B and the pump are all non-trivial
+    for n in range(n_step):
+        sim.step(n)
+    s0 = sim.particles()
+    sim.step(n_step)
+    s1 = sim.particles()
+    f = sim.fields()
+    og = oracle.grid(g.nx, g.ny, g.dx, g.dy)
+    t = n_step * rp.dt
+    rho64, jx64, jy64, jz64 = oracle.deposit_rho_j_f64(og, s0, -1.0, w_uniform=w)
+    amu64 = oracle.deposit_amu_f64(og, s0, -1.0, w_uniform=w)
+    el64 = oracle.solve_es_f64(og, rho64)
+    b64 = oracle.solve_darwin_b_f64(og, jx64, jy64, jz64, c=rp.c)
+    b64 = [b + rp.B0[i] for i, b in enumerate(b64)]
+    p = _pump_grid(rp, g, t)
+    ed64 = (el64[0] + p[0], el64[1] + p[1], p[2])
+    dcu64 = oracle.deposit_dcu_f64(og, s0, -1.0, [a.astype(np.float32) for a in ed64],
+                                   [b.astype(np.float32) for b in b64], w_uniform=w)
+    et64 = oracle.solve_darwin_et_f64(og, dcu64, amu64, c=rp.c, n0=rp.n0)
+    e64 = (el64[0] + et64[0] + p[0], el64[1] + et64[1] + p[1], et64[2] + p[2])
+    # sources of step n_step
+    srcs = sim.sources_all()
+    assert rel_l2(srcs["rho"], rho64) < 1e-6
+    for name, ref in (("Jx", jx64), ("Jy", jy64), ("Jz", jz64)):
+        assert rel_l2(srcs[name], ref) < 1e-6, name
+    for q in range(4):
+        assert rel_l2(srcs[f"amu{q}"], amu64[q]) < 2e-6, q
+    for name, ref in zip(("dcux", "dcuy", "dcuz"), dcu64):
+        assert rel_l2(srcs[name], ref) < 1e-5, name
+    # fields
+    assert rel_l2(f["ELx"], el64[0]) < 1e-5 and rel_l2(f["ELy"], el64[1]) < 1e-5
+    for name, ref in zip(("Bx", "By", "Bz"), b64):
+        assert rel_l2(f[name], ref) < 1e-6, name
+    for name, ref in zip(("ETx", "ETy", "ETz"), et64):
+        assert rel_l2(f[name], ref) < 1e-4, name
+    for name, ref in zip(("Ex", "Ey", "Ez"), e64):
+        assert rel_l2(f[name], ref) < 1e-5, name
+    # particle state after the EM push with the FP64 fields
+    ref = oracle.push_em_f64(og, -1.0, rp.dt, s0, e64, b64)
+    for k in ("ux", "uy", "uz"):
+        assert rel_l2(s1[k], ref[k]) < 1e-6, k
+    for k, nn in (("x", g.nx), ("y", g.ny)):
+        d = (s1[k].astype(np.float64) - ref[k] + 0.5 * nn) % nn - 0.5 * nn
+        assert np.linalg.norm(d) / np.linalg.norm(ref[k]) < 1e-6
+    sim.close()
+
+
+def test_em_push_bit_exact_against_host_restatement(gpu, oracle):
+    sim, g, rp = darwin_sim()
+    sim.init()
+    sim.step(0)
+    s0 = sim.particles()
+    sim.step(1)
+    s1 = sim.particles()
+    f = sim.fields()
+    ref = oracle.push_em_f32(oracle.grid(g.nx, g.ny, g.dx, g.dy), -1.0, rp.dt, s0,
+                             (f["Ex"], f["Ey"], f["Ez"]), (f["Bx"], f["By"], f["Bz"]))
+    for k in ("x", "y", "ux", "uy", "uz"):
+        assert np.array_equal(s1[k], ref[k]), k
+    sim.close()
+
+
+def test_drifting_beam_current(gpu):
+    """tests/test_darwin_magnetostatic.cu part 2: mean Jx = q n0 vd = -0.3."""
+    g = L.grid(64, 32, TWO_PI, TWO_PI)
+    rp = L.run_params(dt=0.05, qm=-1.0, c=1.0, noisy_load=1, rng_seed=7, dump_every=1)
+    sp = [L.species("beam", 1.0, 64, (0.05, 0.05, 0.05), (0.3, 0, 0))]
+    sim = Simulation(g, rp, sp, SimConfig(field_model=L.AW_MODEL_DARWIN))
+    sim.init()
+    sim.step(0)
+    s = sim.sources_all()
+    assert s["Jx"].mean() == pytest.approx(-0.3, abs=0.02)
+    assert abs(s["Jy"].mean()) < 0.02 and abs(s["Jz"].mean()) < 0.02
+    assert s["rho"].mean() == pytest.approx(-1.0, rel=1e-5)
+    sim.close()
+
+
+def test_weibel_growth(gpu):
+    """tests/test_weibel.cu setup (64x64, L = 2 pi, v0 = 1 along +-z, c = 5, uth = .05,
+    dt = .1): magnetic energy grows exponentially; the fitted rate is compared
+    with the cold filamentation root gamma(k) = k v0/sqrt(1+k^2 c^2) of the
+    fastest resolved modes (tests/physics.py)."""
+    nx = ny = 64
+    g = L.grid(nx, ny, TWO_PI, TWO_PI)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, c=5.0, n0=1.0, ndc=2, noisy_load=1,
+                      rng_seed=20260628, dump_every=0)
+    sp = [L.species("beam_p", 0.5, 64, (0.05,) * 3, (0, 0, 1.0)),
+          L.species("beam_m", 0.5, 64, (0.05,) * 3, (0, 0, -1.0))]
+    sim = Simulation(g, rp, sp, SimConfig(field_model=L.AW_MODEL_DARWIN, sort_every=20))
+    sim.init()
+    ts, eb = [], []
+    for n in range(1000):
+        sim.step(n)
+        if n % 5 == 0:
+            f = sim.fields()
+            eb.append(0.5 * float(sum((f[k].astype(np.float64) ** 2).sum()
+                                      for k in ("Bx", "By", "Bz"))) * g.dx * g.dy)
+            ts.append(n * rp.dt)
+    eb = np.array(eb)
+    decades = (np.log(eb.max()) - np.log(eb.min())) / math.log(10)
+    gamma = physics.growth_rate(np.array(ts), eb)
+    print(f"Weibel: decades={decades:.1f} gamma={gamma:.4f} (v0/c = 0.2, "
+          f"k=4 root {physics.weibel_growth(4, 1.0, 5.0):.4f})", flush=True)
+    assert decades > 3.0
+    assert int(np.argmax(eb)) < len(eb) - 1          # saturates
+    assert 0.10 <= gamma <= 0.32                     # the reference's band
+    sim.close()

@@ -124,6 +124,43 @@ def test_resort_keeps_results_identical(gpu):
     b.close()
 
 
+def _state_multiset(sim):
+    s = sim.particles()
+    cols = [s[k] for k in ("x", "y", "ux", "uy", "uz")]
+    order = np.lexsort(cols[::-1])
+    return np.stack([c[order] for c in cols])
+
+
+@pytest.mark.parametrize("tile,bpt,autotune", [(16, 0, False), (32, 3, False), (0, 0, True)])
+def test_exec_config_changes_no_result(gpu, tile, bpt, autotune):
+    """Tile size, workgroups per tile and the autotuner only change the
+    physical particle order: rho bits, the particle multiset and the field
+    diagnostics are identical to the default configuration's."""
+    a = config1_sim(sort_every=5)
+    b = config1_sim(sort_every=5, tile_x=tile, tile_y=tile, blocks_per_tile=bpt,
+                    autotune=autotune)
+    a.init()
+    b.init()
+    ec = b.exec_config()
+    assert ec["tile_x"] in (8, 16, 32) and ec["tile_y"] == ec["tile_x"]
+    assert 1 <= ec["blocks_per_tile"] <= 64
+    if tile:
+        assert ec["tile_x"] == tile
+    if bpt:
+        assert ec["blocks_per_tile"] == bpt
+    assert a.exec_config()["tile_x"] == 8  # 64 particles per cell -> 8x8 default
+    for n in range(12):
+        a.step(n)
+        b.step(n)
+    assert np.array_equal(a.rho(), b.rho())
+    assert np.array_equal(_state_multiset(a), _state_multiset(b))
+    for x, y in zip(a.history(), b.history()):
+        assert x["ee"] == y["ee"] and x["charge"] == y["charge"]
+        assert x["ke"] == pytest.approx(y["ke"], rel=1e-12)
+    a.close()
+    b.close()
+
+
 def test_langmuir_frequency_within_1pct(gpu):
     """tests/test_langmuir.cu setup; gate tightened from 5% to 1%."""
     g = L.grid(64, 8, TWO_PI, TWO_PI)
