@@ -175,6 +175,16 @@ _SIGS = {
     "aw_sim_kernel_time": (C.c_int32, [P, PP(C.c_double), PP(C.c_int64), PP(C.c_int64)]),
     "aw_sim_exec_config": (C.c_int32, [P, PP(C.c_int32), PP(C.c_int32), PP(C.c_int32)]),
     "aw_deck_load": (C.c_int32, [C.c_char_p, PP(AwDeck)]),
+    "aw_device_alloc": (C.c_int32, [C.c_size_t, PP(P)]),
+    "aw_device_free": (C.c_int32, [P]),
+    "aw_copy": (C.c_int32, [P, P, C.c_size_t]),
+    "aw_memset_zero": (C.c_int32, [P, C.c_size_t, P]),
+    "aw_stream_synchronize": (C.c_int32, [P]),
+    "aw_form_e": (C.c_int32, [P, PP(AwFields), PP(AwRunParams), C.c_double, P]),
+    "aw_deposit_rho_j_amu": (C.c_int32, [P, PP(AwParticles), C.c_double, PP(AwSources), P]),
+    "aw_deposit_dcu": (C.c_int32, [P, PP(AwParticles), C.c_double, PP(AwFields),
+                                   PP(AwSources), P]),
+    "aw_push_boris_em": (C.c_int32, [P, PP(AwParticles), PP(AwFields), PP(AwRunParams), P]),
 }
 
 _LIB = None

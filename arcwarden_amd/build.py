@@ -80,7 +80,30 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB.relative_to(ROOT)} from {len(sources)} sources", flush=True)
+    for src, exe in PROGRAMS:
+        _link_program(src, exe)
+        if verbose:
+            print(f"built {exe.relative_to(ROOT)}", flush=True)
     return LIB
+
+
+# C++ programs on the facade (include/pic) linked against the library; the
+# run-path is relative to the executable so the tree can move.
+PROGRAMS = [
+    (PKG / "tools" / "run_deck.cpp", PKG / "tools" / "run_deck"),
+    (ROOT / "tests" / "cpp" / "facade_check.cpp", ROOT / "tests" / "cpp" / "facade_check"),
+]
+
+
+def _link_program(src: Path, exe: Path) -> None:
+    rel = os.path.relpath(PKG, exe.parent)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-Wall",
+           f"-I{ROOT / 'include'}", f"-I{PKG / 'include'}", str(src), "-o", str(exe),
+           f"-L{PKG}", "-larcwarden_hip", f"-Wl,-rpath,$ORIGIN/{rel}",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed for {src.name}:\n{res.stdout}\n{res.stderr}")
 
 
 if __name__ == "__main__":

@@ -161,6 +161,21 @@ inline BorisConsts boris_make(double qm, double dt_eff, const float B0[3], bool 
 // context
 // ---------------------------------------------------------------------------
 
+// Field-combination pointers: E = (E_L + E_T) + pump; ET[0] null means E_L + pump.
+struct FormE {
+    float* E[3];
+    const float* EL[2];
+    const float* ET[3];
+};
+
+// scratch of the standalone Darwin operators (aw_deposit_rho_j_amu, ...)
+struct EmScratch {
+    DevBuf<unsigned long long> acc;  // up to 8 fixed-point grids
+    DevBuf<unsigned int> bound;      // max|w|, max|u| (float bits)
+    DevBuf<float> cq, cdcu, fpart;   // moment / dcu coefficients, field maxima
+    DevBuf<double> inv, dv;          // inverse scales (11), dcu bound inputs (2)
+};
+
 struct SortScratch {
     DevBuf<uint32_t> keyA, keyB;
     DevBuf<int32_t> valA, valB;
@@ -191,6 +206,7 @@ struct aw_ctx {
     aw::DevBuf<unsigned long long> jacc;     // Esirkepov Jx|Jy|Jz fixed-point accumulators
     aw::DevBuf<unsigned int> jbound;         // Esirkepov scale bounds (max|w|, max|w uz|)
     aw::SortScratch sort;
+    aw::EmScratch em;
     ~aw_ctx();
 };
 

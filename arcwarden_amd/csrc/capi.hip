@@ -49,23 +49,31 @@ void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, do
                       const float B0[3], hipStream_t s);
 void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
                      double n0, hipStream_t s);
-void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, unsigned long long* acc8,
-                const float cq[3], int bpt, unsigned long long* strays, hipStream_t s);
-void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
-            const float* B3, unsigned long long* acc3, const float* c_dcu, int bpt,
+void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+                unsigned long long* acc8, const float* cq, int bpt, unsigned long long* strays,
+                hipStream_t s);
+void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+            const float* const* F, unsigned long long* acc3, const float* c_dcu, int bpt,
             unsigned long long* strays, hipStream_t s);
-int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
-            const float* B3, unsigned long long* acc8, const float cq[3], double qm, double dt,
-            int bpt, unsigned long long* strays, double* ke_part, hipStream_t s);
+int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+            const float* const* F, unsigned long long* acc8, const float* cq, double qm,
+            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s);
+void deposit_rho_j_amu(aw_ctx& c, const aw_particles& p, double qm, float* const out[8],
+                       hipStream_t s);
+void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const* F,
+                 float* const out[3], hipStream_t s);
+void push_boris_em(aw_ctx& c, const aw_particles& p, double qm, double dt, const float* const* F,
+                   hipStream_t s);
 void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,
                        hipStream_t s);
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
                    const double* inv, hipStream_t s);
-void form_e(float* E3, const float* EL2, const float* ET3, const aw_grid& g,
-            const aw_run_params& rp, double t, hipStream_t s);
-void dcu_scale(const float* E3, const float* B3, int64_t ncell, float* part, double coef_abs,
-               double coef_signed, double umax, float* c_dcu, double* inv_dcu, hipStream_t s);
+void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t,
+            hipStream_t s);
+void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_abs,
+               double coef_signed, double umax, const double* dev, float* c_dcu, double* inv_dcu,
+               hipStream_t s);
 
 namespace {
 
@@ -166,6 +174,7 @@ struct aw_sim {
     DevBuf<float> ed3;   // E_L + pump (dcu gather)
     DevBuf<double> inv;  // inverse scales: rho, J x3, amu x4, dcu x3
     DevBuf<float> cdcu, fmax_part;
+    DevBuf<float> cqd;  // rho, J, amu coefficients (device)
     float cq[3] = {0.f, 0.f, 0.f};
     double dcu_abs = 0.0, dcu_signed = 0.0, umax = 1.0;
     bool moments_ready = false;
@@ -174,6 +183,14 @@ struct aw_sim {
     float* Ey() const { return em ? e3.get() + ncell(*ctx) : E.get() + ncell(*ctx); }
     float* rho_ptr() const { return em ? src.get() : rho.get(); }
     float* B3() const { return elb.get() + 2 * ncell(*ctx); }
+    // E (3 grids) and B as the six component pointers of the particle kernels
+    void fields6(const float* E3, const float* F[6]) const {
+        const int64_t nc = ncell(*ctx);
+        for (int q = 0; q < 3; ++q) {
+            F[q] = E3 + q * nc;
+            F[3 + q] = B3() + q * nc;
+        }
+    }
 
     void setup_em() {
         em = true;
@@ -210,6 +227,8 @@ struct aw_sim {
                         std::ldexp(1.0, -Sj), std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa),
                         std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa), 1.0, 1.0, 1.0};
         AW_HIP(hipMemcpy(inv.get(), h, sizeof(h), hipMemcpyHostToDevice));
+        cqd.resize(3);
+        AW_HIP(hipMemcpy(cqd.get(), cq, sizeof(cq), hipMemcpyHostToDevice));
         dcu_abs = rp.qm * rp.qm * wmax / area;
         dcu_signed = rp.qm * rp.qm * mult / area;
         acc8.zero(stream);
@@ -218,7 +237,8 @@ struct aw_sim {
     }
 
     void init_em() {
-        em_moments(*ctx, particles(), tile_off.get(), acc8.get(), cq, bpt, nullptr, stream);
+        em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
+                   nullptr, stream);
         moments_ready = true;
     }
 
@@ -228,7 +248,8 @@ struct aw_sim {
         const int64_t nc = ncell(*ctx);
         const double t = (double)nstep * rp.dt;
         if (!moments_ready)
-            em_moments(*ctx, particles(), tile_off.get(), acc8.get(), cq, bpt, nullptr, s);
+            em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
+                       nullptr, s);
         moments_ready = false;
         // rho, J -> src4; amu -> src7[3..6]; fftin holds the copies rocFFT may clobber
         float* fin = fftin.get();
@@ -236,15 +257,23 @@ struct aw_sim {
         convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin + 7 * nc, nc, 4, inv.get() + 4,
                       s);
         solve_darwin_elb(*ctx, fin, elb.get(), rp.eps0, rp.c, rp.B0, s);
-        form_e(ed3.get(), elb.get(), nullptr, g, rp, t, s);
+        form_e(FormE{{ed3.get(), ed3.get() + nc, ed3.get() + 2 * nc},
+                     {elb.get(), elb.get() + nc},
+                     {nullptr, nullptr, nullptr}},
+               g, rp, t, s);
         if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) do_sort();
-        dcu_scale(ed3.get(), B3(), nc, fmax_part.get(), dcu_abs, dcu_signed, umax, cdcu.get(),
+        const float* Fd[6];
+        fields6(ed3.get(), Fd);
+        dcu_scale(Fd, nc, fmax_part.get(), dcu_abs, dcu_signed, umax, nullptr, cdcu.get(),
                   inv.get() + 8, s);
-        em_dcu(*ctx, particles(), tile_off.get(), ed3.get(), B3(), acc3.get(), cdcu.get(), bpt,
+        em_dcu(*ctx, particles(), tile_off.get(), nullptr, Fd, acc3.get(), cdcu.get(), bpt,
                strays.get(), s);
         convert_multi(acc3.get(), src.get() + 4 * nc, fin + 4 * nc, nc, 3, inv.get() + 8, s);
         solve_darwin_et(*ctx, fin + 4 * nc, et.get(), rp.eps0, rp.c, rp.n0, s);
-        form_e(e3.get(), elb.get(), et.get(), g, rp, t, s);
+        form_e(FormE{{e3.get(), e3.get() + nc, e3.get() + 2 * nc},
+                     {elb.get(), elb.get() + nc},
+                     {et.get(), et.get() + nc, et.get() + 2 * nc}},
+               g, rp, t, s);
         const bool dump = rp.dump_every <= 0 || (nstep % rp.dump_every) == 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
@@ -252,8 +281,10 @@ struct aw_sim {
             AW_HIP(hipEventCreate(&e1));
             AW_HIP(hipEventRecord(e0, s));
         }
-        const int nblk = em_push(*ctx, particles(), tile_off.get(), e3.get(), B3(), acc8.get(),
-                                 cq, rp.qm, rp.dt, bpt, strays.get(),
+        const float* Fp[6];
+        fields6(e3.get(), Fp);
+        const int nblk = em_push(*ctx, particles(), tile_off.get(), nullptr, Fp, acc8.get(),
+                                 cqd.get(), rp.qm, rp.dt, bpt, strays.get(),
                                  dump ? ke_part.get() : nullptr, s);
         if (prof) {
             AW_HIP(hipEventRecord(e1, s));
@@ -555,6 +586,40 @@ void aw_last_error(char* buf, size_t len) {
 
 const char* aw_version(void) { return "arcwarden-amd 0.1 gfx950"; }
 
+aw_status aw_device_alloc(size_t bytes, void** out) {
+    return guard("aw_device_alloc", [&] {
+        AW_REQUIRE(out, "out is NULL");
+        *out = nullptr;
+        if (bytes) AW_HIP(hipMalloc(out, bytes));
+    });
+}
+
+aw_status aw_device_free(void* p) {
+    return guard("aw_device_free", [&] {
+        if (p) AW_HIP(hipFree(p));
+    });
+}
+
+aw_status aw_copy(void* dst, const void* src, size_t bytes) {
+    return guard("aw_copy", [&] {
+        if (!bytes) return;
+        AW_REQUIRE(dst && src, "NULL pointer");
+        AW_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    });
+}
+
+aw_status aw_memset_zero(void* p, size_t bytes, aw_stream s) {
+    return guard("aw_memset_zero", [&] {
+        if (!bytes) return;
+        AW_REQUIRE(p, "NULL pointer");
+        AW_HIP(hipMemsetAsync(p, 0, bytes, S(s)));
+    });
+}
+
+aw_status aw_stream_synchronize(aw_stream s) {
+    return guard("aw_stream_synchronize", [&] { AW_HIP(hipStreamSynchronize(S(s))); });
+}
+
 void aw_run_params_defaults(aw_run_params* rp) {
     if (!rp) return;
     std::memset(rp, 0, sizeof(*rp));
@@ -797,6 +862,66 @@ aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double
         AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
         enable_em_fft(*ctx);
         solve_darwin_et(*ctx, src7, out3, eps0, c, n0, S(s));
+    });
+}
+
+aw_status aw_form_e(aw_ctx* ctx, aw_fields* f, const aw_run_params* rp, double t,
+                    aw_stream s) {
+    return guard("aw_form_e", [&] {
+        AW_REQUIRE(ctx && f && rp, "NULL argument");
+        AW_REQUIRE(f->Ex && f->Ey && f->Ez && f->ELx && f->ELy, "E / E_L grids must be non-NULL");
+        AW_REQUIRE(!f->ETx || (f->ETy && f->ETz), "E_T must be all NULL or all non-NULL");
+        form_e(FormE{{f->Ex, f->Ey, f->Ez}, {f->ELx, f->ELy}, {f->ETx, f->ETy, f->ETz}},
+               ctx->g, *rp, t, S(s));
+    });
+}
+
+namespace {
+void fields_in(const aw_fields* f, const float* F[6]) {
+    AW_REQUIRE(f && f->Ex && f->Ey && f->Ez && f->Bx && f->By && f->Bz,
+               "fields: Ex, Ey, Ez, Bx, By, Bz must be non-NULL");
+    F[0] = f->Ex;
+    F[1] = f->Ey;
+    F[2] = f->Ez;
+    F[3] = f->Bx;
+    F[4] = f->By;
+    F[5] = f->Bz;
+}
+}  // namespace
+
+aw_status aw_deposit_rho_j_amu(aw_ctx* ctx, const aw_particles* p, double qm, aw_sources* out,
+                               aw_stream s) {
+    return guard("aw_deposit_rho_j_amu", [&] {
+        AW_REQUIRE(ctx && out, "NULL argument");
+        check_particles(p);
+        float* const o[8] = {out->rho, out->Jx, out->Jy, out->Jz,
+                             out->amu0, out->amu1, out->amu2, out->amu3};
+        for (float* q : o) AW_REQUIRE(q, "sources: rho, J and amu grids must be non-NULL");
+        deposit_rho_j_amu(*ctx, *p, qm, o, S(s));
+    });
+}
+
+aw_status aw_deposit_dcu(aw_ctx* ctx, const aw_particles* p, double qm, const aw_fields* f,
+                         aw_sources* out, aw_stream s) {
+    return guard("aw_deposit_dcu", [&] {
+        AW_REQUIRE(ctx && out, "NULL argument");
+        check_particles(p);
+        const float* F[6];
+        fields_in(f, F);
+        float* const o[3] = {out->dcux, out->dcuy, out->dcuz};
+        for (float* q : o) AW_REQUIRE(q, "sources: dcu grids must be non-NULL");
+        deposit_dcu(*ctx, *p, qm, F, o, S(s));
+    });
+}
+
+aw_status aw_push_boris_em(aw_ctx* ctx, aw_particles* p, const aw_fields* f,
+                           const aw_run_params* rp, aw_stream s) {
+    return guard("aw_push_boris_em", [&] {
+        AW_REQUIRE(ctx && rp, "NULL argument");
+        check_particles(p);
+        const float* F[6];
+        fields_in(f, F);
+        push_boris_em(*ctx, *p, rp->qm, rp->dt, F, S(s));
     });
 }
 

@@ -35,22 +35,24 @@ struct EmArgs {
     float *x, *y, *ux, *uy, *uz;
     const float* w;
     const int32_t* off;
+    const int32_t* idx;          // bin permutation (unsorted particles) or nullptr
     int ntx, tx, ty, apron, W, H, bpt;
     int nx, ny, ncell;
     // fields (E = total, B), each [ny][nx]
     const float *Ex, *Ey, *Ez, *Bx, *By, *Bz;
     unsigned long long* acc;     // nq grids of ncell int64
-    float c_rho, c_j, c_amu;     // 2^S * qm/area (* w_uniform when uniform)
+    const float* cq;             // device: rho, J, amu coefficients 2^S * qm/area
+                                 // (* w_uniform when uniform)
     const float* c_dcu;          // device: 2^S_dcu * qm^2/area (* w_uniform)
     float qmh, cx, cy;
     unsigned long long* strays;
     double* ke_part;
 };
 
-// the 8 moment values of one particle (already scaled)
-__device__ __forceinline__ void moments8(const EmArgs& a, float mw, float ux, float uy, float uz,
+// the 8 moment values of one particle (already scaled); cq = a.cq
+__device__ __forceinline__ void moments8(const float cq[3], float mw, float ux, float uy, float uz,
                                          float v[8]) {
-    const float br = a.c_rho * mw, bj = a.c_j * mw, ba = a.c_amu * mw;
+    const float br = cq[0] * mw, bj = cq[1] * mw, ba = cq[2] * mw;
     v[0] = br;
     v[1] = bj * ux;
     v[2] = bj * uy;
@@ -175,10 +177,12 @@ __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
     for (int c = threadIdx.x; c < 8 * f.R; c += kThreads) smem[c] = 0ull;
     __syncthreads();
     unsigned int nstray = 0;
-    for (int k = f.r0 + (int)threadIdx.x; k < f.r1; k += kThreads) {
+    const float cq[3] = {a.cq[0], a.cq[1], a.cq[2]};
+    for (int r = f.r0 + (int)threadIdx.x; r < f.r1; r += kThreads) {
+        const int k = a.idx ? a.idx[r] : r;
         const float mw = PER_W ? a.w[k] : 1.0f;
         float v[8];
-        moments8(a, mw, a.ux[k], a.uy[k], a.uz[k], v);
+        moments8(cq, mw, a.ux[k], a.uy[k], a.uz[k], v);
         deposit_q<8>(smem, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
                      dev::cic_f(a.x[k], a.y[k]), v, nstray);
     }
@@ -199,7 +203,8 @@ __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
     __syncthreads();
     const float cd = *a.c_dcu;
     unsigned int nstray = 0;
-    for (int k = f.r0 + (int)threadIdx.x; k < f.r1; k += kThreads) {
+    for (int r = f.r0 + (int)threadIdx.x; r < f.r1; r += kThreads) {
+        const int k = a.idx ? a.idx[r] : r;
         const dev::CicF s = dev::cic_f(a.x[k], a.y[k]);
         float e[3], b[3];
         if (!gather6(a, f, fa, fb, s, e, b)) ++nstray;
@@ -217,19 +222,29 @@ __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
     if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
 }
 
-template <bool PER_W, bool KE>
+// DEP = false: push, move and wrap only (Pusher::boris_em + migrate); the
+// LDS moment tile is then not allocated.
+template <bool PER_W, bool KE, bool DEP>
 __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
     extern __shared__ unsigned long long smem[];
     const Frame f = frame(a);
+    constexpr int kNq = DEP ? 8 : 0;
     unsigned long long* s_acc = smem;
-    float4* fa = reinterpret_cast<float4*>(smem + 8 * f.R);
+    float4* fa = reinterpret_cast<float4*>(smem + kNq * f.R);
     float2* fb = reinterpret_cast<float2*>(fa + f.R);
-    for (int c = threadIdx.x; c < 8 * f.R; c += kThreads) s_acc[c] = 0ull;
+    for (int c = threadIdx.x; c < kNq * f.R; c += kThreads) s_acc[c] = 0ull;
     load_fields(a, f, fa, fb);
     __syncthreads();
     unsigned int nstray = 0;
     double ke = 0.0;
-    for (int k = f.r0 + (int)threadIdx.x; k < f.r1; k += kThreads) {
+    float cq[3] = {0.f, 0.f, 0.f};
+    if constexpr (DEP) {
+        cq[0] = a.cq[0];
+        cq[1] = a.cq[1];
+        cq[2] = a.cq[2];
+    }
+    for (int r = f.r0 + (int)threadIdx.x; r < f.r1; r += kThreads) {
+        const int k = a.idx ? a.idx[r] : r;
         const float x = a.x[k], y = a.y[k];
         float ux = a.ux[k], uy = a.uy[k], uz = a.uz[k];
         const float mw = PER_W ? a.w[k] : 1.0f;
@@ -247,10 +262,12 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
         if constexpr (KE)
             ke += (PER_W ? (double)mw : 1.0) *
                   ((double)ux * ux + (double)uy * uy + (double)uz * uz);
-        float v[8];
-        moments8(a, mw, ux, uy, uz, v);
-        deposit_q<8>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
-                     dev::cic_f(xn, yn), v, nstray);
+        if constexpr (DEP) {
+            float v[8];
+            moments8(cq, mw, ux, uy, uz, v);
+            deposit_q<8>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
+                         dev::cic_f(xn, yn), v, nstray);
+        }
     }
     __shared__ double s_ke[kThreads / 64];
     if constexpr (KE) {
@@ -258,7 +275,7 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
         if ((threadIdx.x & 63) == 0) s_ke[threadIdx.x >> 6] = ke;
     }
     __syncthreads();
-    flush_q<8>(s_acc, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
+    if constexpr (DEP) flush_q<8>(s_acc, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
     if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
     if constexpr (KE) {
         if (threadIdx.x == 0) {
@@ -300,16 +317,15 @@ struct PumpArgs {
 };
 
 // E = (E_L + s*E_T) + pump(t) (combine_etotal_kernel, solver_darwin.hpp:134-144;
-// add_pump_kernel, pump.hpp:34-47); ET may be null (s = 0).
-__global__ void form_e_kernel(float* __restrict__ E3, const float* __restrict__ EL2,
-                              const float* __restrict__ ET3, int nx, int ncell, PumpArgs p) {
+// add_pump_kernel, pump.hpp:34-47); ET[0] may be null (s = 0).
+__global__ void form_e_kernel(FormE f, int nx, int ncell, PumpArgs p) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    float ex = EL2[c], ey = EL2[ncell + c], ez = 0.0f;
-    if (ET3) {
-        ex = ex + 1.0f * ET3[c];
-        ey = ey + 1.0f * ET3[ncell + c];
-        ez = 1.0f * ET3[2 * ncell + c];
+    float ex = f.EL[0][c], ey = f.EL[1][c], ez = 0.0f;
+    if (f.ET[0]) {
+        ex = ex + 1.0f * f.ET[0][c];
+        ey = ey + 1.0f * f.ET[1][c];
+        ez = 1.0f * f.ET[2][c];
     }
     if (p.on && p.t < p.toff) {
         const double x = (c % nx) * p.dx;
@@ -320,21 +336,24 @@ __global__ void form_e_kernel(float* __restrict__ E3, const float* __restrict__ 
         ey += (float)(-p.ey * st * r);
         ez += (float)(p.ez * ct * r);
     }
-    E3[c] = ex;
-    E3[ncell + c] = ey;
-    E3[2 * ncell + c] = ez;
+    f.E[0][c] = ex;
+    f.E[1][c] = ey;
+    f.E[2][c] = ez;
 }
+
+struct Field6 {
+    const float* f[6];  // Ex, Ey, Ez, Bx, By, Bz
+};
 
 // block maxima of |E|, |B| components -> part[blockIdx]
 __global__ void __launch_bounds__(kThreads)
-field_max_kernel(const float* __restrict__ E3, const float* __restrict__ B3, int ncell,
-                 float* __restrict__ part) {
+field_max_kernel(Field6 F, int ncell, float* __restrict__ part) {
     __shared__ float sh[2][kThreads];
     float me = 0.f, mb = 0.f;
     for (int c = blockIdx.x * kThreads + threadIdx.x; c < ncell; c += gridDim.x * kThreads) {
         for (int q = 0; q < 3; ++q) {
-            me = fmaxf(me, fabsf(E3[(size_t)q * ncell + c]));
-            mb = fmaxf(mb, fabsf(B3[(size_t)q * ncell + c]));
+            me = fmaxf(me, fabsf(F.f[q][c]));
+            mb = fmaxf(mb, fabsf(F.f[3 + q][c]));
         }
     }
     sh[0][threadIdx.x] = me;
@@ -356,10 +375,15 @@ field_max_kernel(const float* __restrict__ E3, const float* __restrict__ B3, int
 // dcu scale: per-contribution bound = |qm^2 w/area| (max|E| + umax max|B|);
 // S = 48 - ceil(log2(bound)) maps the bound to 2^48.  Writes the float
 // coefficient used by dcu_kernel and the inverse scale used by conversion.
+// dev (optional, device): {coef_abs, umax} measured from the particles.
 __global__ void dcu_scale_kernel(const float* __restrict__ part, int nb, double coef_abs,
-                                 double coef_signed, double umax, float* __restrict__ c_dcu,
-                                 double* __restrict__ inv_dcu) {
+                                 double coef_signed, double umax, const double* __restrict__ dev,
+                                 float* __restrict__ c_dcu, double* __restrict__ inv_dcu) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (dev) {
+        coef_abs = dev[0];
+        umax = dev[1];
+    }
     float me = 0.f, mb = 0.f;
     for (int i = 0; i < nb; ++i) {
         me = fmaxf(me, part[2 * i]);
@@ -377,20 +401,79 @@ __global__ void dcu_scale_kernel(const float* __restrict__ part, int nb, double 
     inv_dcu[2] = inv;
 }
 
+__device__ __forceinline__ int scale_exp(double bound) {
+    if (!(bound > 1e-300)) bound = 1e-300;
+    const int S = 48 - (int)ceil(log2(bound));
+    return S > 120 ? 120 : (S < -120 ? -120 : S);
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// max |w| (per-particle weights) and max |u| component, as positive-float bit
+// patterns (unsigned order = value order)
+__global__ void __launch_bounds__(kThreads)
+moment_bound_kernel(const float* __restrict__ ux, const float* __restrict__ uy,
+                    const float* __restrict__ uz, const float* __restrict__ w, int n,
+                    unsigned int* bound) {
+    float mw = 0.f, mu = 0.f;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        mu = fmaxf(mu, fmaxf(fabsf(ux[i]), fmaxf(fabsf(uy[i]), fabsf(uz[i]))));
+        if (w) mw = fmaxf(mw, fabsf(w[i]));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mw = fmaxf(mw, __shfl_xor(mw, o, 64));
+        mu = fmaxf(mu, __shfl_xor(mu, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&bound[0], __float_as_uint(mw));
+        atomicMax(&bound[1], __float_as_uint(mu));
+    }
+}
+
+// Moment scales from the measured bounds (the same rule as the Simulation's
+// static scales): cq = {rho, J, amu} coefficients, inv[0..7] the inverse
+// scales of rho, Jx..Jz, amu0..3, dv = {dcu coef_abs, umax}.
+__global__ void moment_scale_kernel(const unsigned int* __restrict__ bound, int per_w,
+                                    double qm, double area, double w_uniform,
+                                    float* __restrict__ cq, double* __restrict__ inv,
+                                    double* __restrict__ dv) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double wmax = per_w ? (double)__uint_as_float(bound[0]) : fabs(w_uniform);
+    const double umax = (double)__uint_as_float(bound[1]);
+    const double cabs = fabs(qm) * wmax / area;
+    const int S[3] = {scale_exp(cabs), scale_exp(cabs * umax), scale_exp(cabs * umax * umax)};
+    const double mult = per_w ? 1.0 : w_uniform;
+    for (int q = 0; q < 3; ++q) cq[q] = (float)ldexp(qm * mult / area, S[q]);
+    inv[0] = ldexp(1.0, -S[0]);
+    for (int q = 1; q < 4; ++q) inv[q] = ldexp(1.0, -S[1]);
+    for (int q = 4; q < 8; ++q) inv[q] = ldexp(1.0, -S[2]);
+    dv[0] = qm * qm * wmax / area;
+    dv[1] = umax;
+}
+
+struct Out8 {
+    float* o[8];
+};
+
+// out_q += acc_q * inv[q] for q < nq (separate output grids)
+__global__ void convert_add_kernel(const unsigned long long* __restrict__ acc, Out8 out, int ncell,
+                                   int nq, const double* __restrict__ inv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    for (int q = 0; q < nq; ++q)
+        out.o[q][c] += (float)((double)(long long)acc[(size_t)q * ncell + c] * inv[q]);
+}
+
 }  // namespace
 
 // ---- host side ------------------------------------------------------------------------
 
 int tile_bpt(const aw_ctx& c, int64_t n);
 
-struct EmLaunch {
-    EmArgs a;
-    int nblk;
-    size_t lds_fields;
-};
-
-static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
-                      const float* B3, unsigned long long* acc, const float cq[3],
+static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+                      const float* const* F, unsigned long long* acc, const float* cq,
                       const float* c_dcu, double qm, double dt, int bpt,
                       unsigned long long* strays, double* ke_part) {
     EmArgs a{};
@@ -401,6 +484,7 @@ static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, cons
     a.uz = p.uz;
     a.w = p.w;
     a.off = off;
+    a.idx = idx;
     a.ntx = c.ntx;
     a.tx = c.tx;
     a.ty = c.ty;
@@ -411,20 +495,16 @@ static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, cons
     a.nx = c.g.nx;
     a.ny = c.g.ny;
     a.ncell = (int)ncell(c);
-    if (E3) {
-        a.Ex = E3;
-        a.Ey = E3 + a.ncell;
-        a.Ez = E3 + 2 * a.ncell;
-    }
-    if (B3) {
-        a.Bx = B3;
-        a.By = B3 + a.ncell;
-        a.Bz = B3 + 2 * a.ncell;
+    if (F) {
+        a.Ex = F[0];
+        a.Ey = F[1];
+        a.Ez = F[2];
+        a.Bx = F[3];
+        a.By = F[4];
+        a.Bz = F[5];
     }
     a.acc = acc;
-    a.c_rho = cq[0];
-    a.c_j = cq[1];
-    a.c_amu = cq[2];
+    a.cq = cq;
     a.c_dcu = c_dcu;
     a.qmh = (float)(qm * 0.5 * dt);
     a.cx = (float)(dt / c.g.dx);
@@ -434,10 +514,11 @@ static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, cons
     return a;
 }
 
-void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, unsigned long long* acc8,
-                const float cq[3], int bpt, unsigned long long* strays, hipStream_t s) {
+void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+                unsigned long long* acc8, const float* cq, int bpt, unsigned long long* strays,
+                hipStream_t s) {
     if (p.n == 0) return;
-    EmArgs a = em_args(c, p, off, nullptr, nullptr, acc8, cq, nullptr, 0.0, 0.0, bpt, strays,
+    EmArgs a = em_args(c, p, off, idx, nullptr, acc8, cq, nullptr, 0.0, 0.0, bpt, strays,
                        nullptr);
     const size_t sh = (size_t)a.W * a.H * 8 * 8;
     const int nblk = c.ntiles * bpt;
@@ -446,12 +527,11 @@ void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, unsigned l
     AW_LAUNCH_CHECK();
 }
 
-void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
-            const float* B3, unsigned long long* acc3, const float* c_dcu, int bpt,
+void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+            const float* const* F, unsigned long long* acc3, const float* c_dcu, int bpt,
             unsigned long long* strays, hipStream_t s) {
     if (p.n == 0) return;
-    const float zero[3] = {0.f, 0.f, 0.f};
-    EmArgs a = em_args(c, p, off, E3, B3, acc3, zero, c_dcu, 0.0, 0.0, bpt, strays, nullptr);
+    EmArgs a = em_args(c, p, off, idx, F, acc3, nullptr, c_dcu, 0.0, 0.0, bpt, strays, nullptr);
     const size_t sh = (size_t)a.W * a.H * (3 * 8 + 16 + 8);
     const int nblk = c.ntiles * bpt;
     if (p.w) dcu_kernel<true><<<nblk, kThreads, sh, s>>>(a);
@@ -459,19 +539,31 @@ void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E
     AW_LAUNCH_CHECK();
 }
 
-int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* E3,
-            const float* B3, unsigned long long* acc8, const float cq[3], double qm, double dt,
-            int bpt, unsigned long long* strays, double* ke_part, hipStream_t s) {
+// acc8 == nullptr: push, move and wrap only
+int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
+            const float* const* F, unsigned long long* acc8, const float* cq, double qm,
+            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s) {
     if (p.n == 0) return 0;
-    EmArgs a = em_args(c, p, off, E3, B3, acc8, cq, nullptr, qm, dt, bpt, strays, ke_part);
-    const size_t sh = (size_t)a.W * a.H * (8 * 8 + 16 + 8);
+    EmArgs a = em_args(c, p, off, idx, F, acc8, cq, nullptr, qm, dt, bpt, strays, ke_part);
+    const bool dep = acc8 != nullptr;
+    const size_t sh = (size_t)a.W * a.H * ((dep ? 8 * 8 : 0) + 16 + 8);
     const int nblk = c.ntiles * bpt;
     const bool pw = p.w != nullptr, ke = ke_part != nullptr;
-    if (pw && ke) em_push_kernel<true, true><<<nblk, kThreads, sh, s>>>(a);
-    else if (pw) em_push_kernel<true, false><<<nblk, kThreads, sh, s>>>(a);
-    else if (ke) em_push_kernel<false, true><<<nblk, kThreads, sh, s>>>(a);
-    else em_push_kernel<false, false><<<nblk, kThreads, sh, s>>>(a);
-    AW_LAUNCH_CHECK();
+    auto launch = [&](auto kernel) {
+        kernel<<<nblk, kThreads, sh, s>>>(a);
+        AW_LAUNCH_CHECK();
+    };
+    if (dep) {
+        if (pw && ke) launch(em_push_kernel<true, true, true>);
+        else if (pw) launch(em_push_kernel<true, false, true>);
+        else if (ke) launch(em_push_kernel<false, true, true>);
+        else launch(em_push_kernel<false, false, true>);
+    } else {
+        if (pw && ke) launch(em_push_kernel<true, true, false>);
+        else if (pw) launch(em_push_kernel<true, false, false>);
+        else if (ke) launch(em_push_kernel<false, true, false>);
+        else launch(em_push_kernel<false, false, false>);
+    }
     return nblk;
 }
 
@@ -482,8 +574,8 @@ void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t nce
     AW_LAUNCH_CHECK();
 }
 
-void form_e(float* E3, const float* EL2, const float* ET3, const aw_grid& g,
-            const aw_run_params& rp, double t, hipStream_t s) {
+void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t,
+            hipStream_t s) {
     PumpArgs p{};
     p.on = rp.pump ? 1 : 0;
     p.ex = rp.pump_ex;
@@ -496,17 +588,102 @@ void form_e(float* E3, const float* EL2, const float* ET3, const aw_grid& g,
     p.t = t;
     p.dx = g.dx;
     const int64_t nc = (int64_t)g.nx * g.ny;
-    form_e_kernel<<<blocks_for(nc), kThreads, 0, s>>>(E3, EL2, ET3, g.nx, (int)nc, p);
+    form_e_kernel<<<blocks_for(nc), kThreads, 0, s>>>(f, g.nx, (int)nc, p);
     AW_LAUNCH_CHECK();
 }
 
-void dcu_scale(const float* E3, const float* B3, int64_t ncell, float* part, double coef_abs,
-               double coef_signed, double umax, float* c_dcu, double* inv_dcu, hipStream_t s) {
+void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_abs,
+               double coef_signed, double umax, const double* dev, float* c_dcu, double* inv_dcu,
+               hipStream_t s) {
     const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (ncell + 4095) / 4096));
-    field_max_kernel<<<nb, kThreads, 0, s>>>(E3, B3, (int)ncell, part);
+    Field6 f6{{F[0], F[1], F[2], F[3], F[4], F[5]}};
+    field_max_kernel<<<nb, kThreads, 0, s>>>(f6, (int)ncell, part);
     AW_LAUNCH_CHECK();
-    dcu_scale_kernel<<<1, 64, 0, s>>>(part, nb, coef_abs, coef_signed, umax, c_dcu, inv_dcu);
+    dcu_scale_kernel<<<1, 64, 0, s>>>(part, nb, coef_abs, coef_signed, umax, dev, c_dcu, inv_dcu);
     AW_LAUNCH_CHECK();
+}
+
+// ---- standalone operators (any particle order) ------------------------------------------
+//
+// aw_deposit_rho_j_amu, aw_deposit_dcu and aw_push_boris_em take particles in
+// any order: they bin them by tile (stable counting sort into the context's
+// bin permutation) and read them through it; the fixed-point scales come from
+// the measured max |w| and max |u|.
+
+static void bin_particles(aw_ctx& c, const aw_particles& p, hipStream_t s) {
+    c.sort.keyA.reserve((size_t)p.n);
+    c.sort.bin_off.reserve((size_t)c.ntiles + 1);
+    c.sort.bin_perm.reserve((size_t)p.n);
+    compute_tile_keys(c, p, c.sort.keyA.get(), false, s);
+    stable_sort_keys(c, p.n, (uint32_t)c.ntiles, c.sort.bin_perm.get(), c.sort.bin_off.get(), s);
+}
+
+static void measure_scales(aw_ctx& c, const aw_particles& p, double qm, hipStream_t s) {
+    EmScratch& e = c.em;
+    e.bound.reserve(2);
+    e.cq.reserve(3);
+    e.inv.reserve(11);
+    e.dv.reserve(2);
+    zero_u32_kernel<<<1, 64, 0, s>>>(e.bound.get(), 2);
+    AW_LAUNCH_CHECK();
+    const int nb = (int)std::min<int64_t>(4ll * std::max(1, c.num_cu), (p.n + kThreads - 1) / kThreads);
+    moment_bound_kernel<<<std::max(1, nb), kThreads, 0, s>>>(p.ux, p.uy, p.uz, p.w, (int)p.n,
+                                                            e.bound.get());
+    AW_LAUNCH_CHECK();
+    moment_scale_kernel<<<1, 64, 0, s>>>(e.bound.get(), p.w ? 1 : 0, qm, c.g.dx * c.g.dy,
+                                         p.w_uniform, e.cq.get(), e.inv.get(), e.dv.get());
+    AW_LAUNCH_CHECK();
+}
+
+void deposit_rho_j_amu(aw_ctx& c, const aw_particles& p, double qm, float* const out[8],
+                       hipStream_t s) {
+    if (p.n == 0) return;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    const int64_t nc = ncell(c);
+    EmScratch& e = c.em;
+    e.acc.reserve((size_t)(8 * nc));
+    AW_HIP(hipMemsetAsync(e.acc.get(), 0, (size_t)(8 * nc) * sizeof(unsigned long long), s));
+    bin_particles(c, p, s);
+    measure_scales(c, p, qm, s);
+    em_moments(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), e.acc.get(), e.cq.get(),
+               tile_bpt(c, p.n), nullptr, s);
+    Out8 o{};
+    for (int q = 0; q < 8; ++q) o.o[q] = out[q];
+    convert_add_kernel<<<blocks_for(nc), kThreads, 0, s>>>(e.acc.get(), o, (int)nc, 8, e.inv.get());
+    AW_LAUNCH_CHECK();
+}
+
+void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const* F,
+                 float* const out[3], hipStream_t s) {
+    if (p.n == 0) return;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    const int64_t nc = ncell(c);
+    EmScratch& e = c.em;
+    e.acc.reserve((size_t)(8 * nc));
+    e.cdcu.reserve(1);
+    e.fpart.reserve(512);
+    AW_HIP(hipMemsetAsync(e.acc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
+    bin_particles(c, p, s);
+    measure_scales(c, p, qm, s);
+    const double mult = p.w ? 1.0 : p.w_uniform;
+    dcu_scale(F, nc, e.fpart.get(), 0.0, qm * qm * mult / (c.g.dx * c.g.dy), 0.0, e.dv.get(),
+              e.cdcu.get(), e.inv.get() + 8, s);
+    em_dcu(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), F, e.acc.get(), e.cdcu.get(),
+           tile_bpt(c, p.n), nullptr, s);
+    Out8 o{};
+    for (int q = 0; q < 3; ++q) o.o[q] = out[q];
+    convert_add_kernel<<<blocks_for(nc), kThreads, 0, s>>>(e.acc.get(), o, (int)nc, 3,
+                                                            e.inv.get() + 8);
+    AW_LAUNCH_CHECK();
+}
+
+void push_boris_em(aw_ctx& c, const aw_particles& p, double qm, double dt, const float* const* F,
+                   hipStream_t s) {
+    if (p.n == 0) return;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    bin_particles(c, p, s);
+    em_push(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), F, nullptr, nullptr, qm, dt,
+            tile_bpt(c, p.n), nullptr, nullptr, s);
 }
 
 }  // namespace aw

@@ -186,6 +186,15 @@ void aw_last_error(char* buf, size_t len);
 /* Library version string, e.g. "arcwarden-amd 0.1 gfx950". */
 const char* aw_version(void);
 
+/* Device memory for C / C++ callers without the HIP headers: DeviceArray's
+ * allocate / release / copies (include/pic/device_array.hpp:45-107).
+ * aw_copy is a blocking copy between any two host or device pointers. */
+aw_status aw_device_alloc(size_t bytes, void** out);
+aw_status aw_device_free(void* p);
+aw_status aw_copy(void* dst, const void* src, size_t bytes);
+aw_status aw_memset_zero(void* p, size_t bytes, aw_stream s);
+aw_status aw_stream_synchronize(aw_stream s);
+
 /* Default-constructed RunParams / Species: the member initialisers of
  * include/pic/config.hpp:127-313 and include/pic/species.hpp:20-45. */
 void aw_run_params_defaults(aw_run_params* rp);
@@ -316,6 +325,36 @@ aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, doub
  * src7 = [dcux|dcuy|dcuz|amu0|amu1|amu2|amu3] -> out3 = [ETx|ETy|ETz]. */
 aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double eps0,
                              double c, double n0, aw_stream s);
+
+/* form_el / form_etotal + add_pump_field (include/pic/solver_darwin.hpp:134-144,
+ * 220-231; include/pic/pump.hpp:27-79): f->Ex, Ey, Ez = E_L + E_T + pump(t)
+ * from f->ELx, ELy and f->ETx, ETy, ETz (f->ETx NULL: E_L + pump only, the
+ * field the dcu deposit gathers).  The pump is rp->pump_* (off when rp->pump
+ * is 0). */
+aw_status aw_form_e(aw_ctx* ctx, aw_fields* f, const aw_run_params* rp, double t,
+                    aw_stream s);
+
+/* ---- Darwin particle operators (any particle order) ----------------------------- */
+
+/* Depositor<Cfg>::charge_current + deposit_amu (include/pic/depositor.hpp:160-270,
+ * 369-422): with W the CIC weights and c_p = qm w_p/(dx dy),
+ *   rho += c_p W, J += c_p (ux, uy, uz) W,
+ *   amu0 += c_p (ux^2 - uy^2)/2 W, amu1 += c_p ux uy W, amu2 += c_p uz ux W,
+ *   amu3 += c_p uz uy W
+ * into out->rho, Jx, Jy, Jz, amu0..amu3 (all non-NULL).  Accumulates. */
+aw_status aw_deposit_rho_j_amu(aw_ctx* ctx, const aw_particles* p, double qm, aw_sources* out,
+                               aw_stream s);
+/* Depositor<Cfg>::deposit_dcu (include/pic/depositor.hpp:275-324,426-437):
+ * dcu += qm^2 w/(dx dy) (E + u x B) W with E = f->Ex, Ey, Ez (E_L + pump) and
+ * B = f->Bx, By, Bz gathered at the particle, into out->dcux, dcuy, dcuz.
+ * Accumulates. */
+aw_status aw_deposit_dcu(aw_ctx* ctx, const aw_particles* p, double qm, const aw_fields* f,
+                         aw_sources* out, aw_stream s);
+/* Pusher<Cfg>::boris_em + Particles::migrate (include/pic/pusher.hpp:211-258,
+ * 300-317; particles.hpp:293-315): E = f->Ex, Ey, Ez and B = f->Bx, By, Bz
+ * gathered at the particle, full Boris rotation, move, periodic wrap. */
+aw_status aw_push_boris_em(aw_ctx* ctx, aw_particles* p, const aw_fields* f,
+                           const aw_run_params* rp, aw_stream s);
 
 /* ---- diagnostics ------------------------------------------------------------------- */
 
