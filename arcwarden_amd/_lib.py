@@ -99,6 +99,18 @@ class AwSimConfig(C.Structure):
                 ("autotune", C.c_int32)]
 
 
+class AwRunDesc(C.Structure):
+    _fields_ = [("grid", AwGrid), ("rp", AwRunParams), ("species", C.c_void_p),
+                ("nspecies", C.c_int32), ("has_b0", C.c_int32), ("n_modes", C.c_int32),
+                ("mode_amp", C.c_double * 32)]
+
+
+class AwEnsembleStats(C.Structure):
+    _fields_ = [("nruns", C.c_int32), ("ngroups", C.c_int32), ("particles", C.c_int64),
+                ("cells", C.c_int64), ("workgroups", C.c_int32), ("graph", C.c_int32),
+                ("sorts", C.c_int64), ("graph_launches", C.c_int64)]
+
+
 AW_DECK_MAX_SPECIES = 16
 AW_DECK_MAX_PROBES = 64
 
@@ -180,6 +192,27 @@ _SIGS = {
     "aw_copy": (C.c_int32, [P, P, C.c_size_t]),
     "aw_memset_zero": (C.c_int32, [P, C.c_size_t, P]),
     "aw_stream_synchronize": (C.c_int32, [P]),
+    "aw_ensemble_create": (C.c_int32, [PP(AwRunDesc), C.c_int32, PP(AwSimConfig), PP(P)]),
+    "aw_ensemble_destroy": (C.c_int32, [P]),
+    "aw_ensemble_set_graph": (C.c_int32, [P, C.c_int32]),
+    "aw_ensemble_init": (C.c_int32, [P]),
+    "aw_ensemble_step": (C.c_int32, [P, C.c_int64]),
+    "aw_ensemble_run": (C.c_int32, [P, C.c_int64, C.c_int64]),
+    "aw_ensemble_synchronize": (C.c_int32, [P]),
+    "aw_ensemble_info": (C.c_int32, [P, PP(AwEnsembleStats)]),
+    "aw_ensemble_particles": (C.c_int32, [P, C.c_int32, PP(AwParticles)]),
+    "aw_ensemble_fields": (C.c_int32, [P, C.c_int32, PP(AwFields)]),
+    "aw_ensemble_rho": (C.c_int32, [P, C.c_int32, PP(P)]),
+    "aw_ensemble_history": (C.c_int32, [P, C.c_int32, PP(AwDiagSample), C.c_int64,
+                                        PP(C.c_int64)]),
+    "aw_adj_create": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(AwSpecies), C.c_int32,
+                                  C.c_int32, C.c_int32, C.c_int32, PP(P)]),
+    "aw_adj_destroy": (C.c_int32, [P]),
+    "aw_adj_forward": (C.c_int32, [P, PP(C.c_double), PP(C.c_double), C.c_int64,
+                                   PP(C.c_double)]),
+    "aw_adj_backward": (C.c_int32, [P, PP(C.c_double), PP(C.c_double)]),
+    "aw_adj_state_grad": (C.c_int32, [P, P, P, P, P]),
+    "aw_adj_initial_state": (C.c_int32, [P, PP(AwParticles)]),
     "aw_form_e": (C.c_int32, [P, PP(AwFields), PP(AwRunParams), C.c_double, P]),
     "aw_deposit_rho_j_amu": (C.c_int32, [P, PP(AwParticles), C.c_double, PP(AwSources), P]),
     "aw_deposit_dcu": (C.c_int32, [P, PP(AwParticles), C.c_double, PP(AwFields),

@@ -196,6 +196,8 @@ struct aw_ctx {
     rocfft_plan r2c = nullptr, c2r = nullptr, c2r2 = nullptr;
     rocfft_plan r2c3 = nullptr, c2r3 = nullptr;
     rocfft_plan r2c4 = nullptr, r2c7 = nullptr, c2r5 = nullptr;  // Darwin batches
+    rocfft_plan r2cB = nullptr, c2rB = nullptr;                  // ensemble batches
+    int fft_batch = 0;
     rocfft_execution_info info = nullptr;
     aw::DevBuf<unsigned char> fft_work;
     // workspace
@@ -234,11 +236,40 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign = false,
                       unsigned long long* counter = nullptr, float* out2 = nullptr);
+// acc == nullptr: the context's accumulator
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
-                       const int32_t* sorted_off, hipStream_t s);
+                       const int32_t* sorted_off, hipStream_t s,
+                       unsigned long long* acc = nullptr);
+int tile_bpt(const aw_ctx& c, int64_t n);
 
 // Spectral (spectral.hip)
 void solve_es_contig(aw_ctx& c, const float* rho, float* E2, double eps0, hipStream_t s);
+void enable_batch_fft(aw_ctx& c, int batch);
+void solve_es_batch(aw_ctx& c, int batch, float* rho_in, float* E2B, double eps0,
+                    hipStream_t s);
+
+// Particles, push (particles.hip, push.hip)
+void load_species(aw_ctx& c, const aw_species* sp, int nsp, const aw_run_params& rp,
+                  const double* mode_amp, int n_modes, const aw_particles& p,
+                  double* weights_out, hipStream_t s);
+void push_es(aw_ctx& c, const aw_particles& p, const float* Ex, const float* Ey,
+             const aw_run_params& rp, bool has_b0, bool half_back, hipStream_t s);
+bool uz_evolves(const aw_run_params& rp, bool has_b0);
+size_t run_slot_bytes();
+int fill_run_slot(void* slot, aw_ctx& c, const aw_particles& p, const int32_t* off,
+                  const float* Ex, const float* Ey, const aw_run_params& rp, bool has_b0,
+                  unsigned long long* acc, unsigned long long* strays, double* ke_part, int bpt,
+                  int wg_begin);
+size_t push_lds_bytes(const aw_ctx& c);
+void push_deposit_multi(const void* slots, int nruns, int total_wg, size_t lds, bool ke, bool uz,
+                        hipStream_t s);
+
+// Contexts (capi.hip)
+aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y);
+int fixed_scale_for(const std::vector<aw_species>& sp, const aw_run_params& rp, bool legacy);
+void set_ctx_tiles(aw_ctx& c, int tile_x, int tile_y);
+void sort_tiles(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* perm,
+                int32_t* off, hipStream_t s);
 
 }  // namespace aw
 

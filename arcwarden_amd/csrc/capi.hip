@@ -101,6 +101,8 @@ void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s,
                      assign ? c.rin.get() : nullptr);
 }
 
+}  // namespace
+
 void set_ctx_tiles(aw_ctx& c, int tile_x, int tile_y) {
     c.tx = std::min(tile_x > 0 ? tile_x : 16, c.g.nx);
     c.ty = std::min(tile_y > 0 ? tile_y : 16, c.g.ny);
@@ -117,8 +119,6 @@ void sort_tiles(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int
     stable_sort_keys(c, src.n, (uint32_t)c.ntiles, perm, off, s);
     permute_particles(src, dst, perm, s);
 }
-
-}  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 const std::string& last_error() { return g_last_error; }
@@ -543,7 +543,7 @@ struct aw_sim {
 // C ABI
 // ============================================================================
 
-namespace {
+namespace aw {
 
 aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y) {
     check_grid(g);
@@ -574,7 +574,7 @@ int fixed_scale_for(const std::vector<aw_species>& sp, const aw_run_params& rp, 
     return std::clamp(40 - e, 8, 60);
 }
 
-}  // namespace
+}  // namespace aw
 
 extern "C" {
 

@@ -163,8 +163,9 @@ int tile_bpt(const aw_ctx& c, int64_t n) {
 // particles are physically tile-sorted with those offsets (tiles policy,
 // direct reads).  policy BINNED without offsets bins first (stable).
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
-                       const int32_t* sorted_off, hipStream_t s) {
+                       const int32_t* sorted_off, hipStream_t s, unsigned long long* acc) {
     if (p.n == 0) return;
+    if (!acc) acc = c.acc.get();
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
     DepArgs a{};
     a.x = p.x;
@@ -185,7 +186,7 @@ void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
         else policy = avg_ppc >= 24.0 ? AW_DEPOSIT_BINNED : AW_DEPOSIT_GLOBAL;
     }
     if (policy == AW_DEPOSIT_GLOBAL) {
-        deposit_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, c.acc.get());
+        deposit_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, acc);
         AW_LAUNCH_CHECK();
     } else if (policy == AW_DEPOSIT_LDS_GRID) {
         const size_t sh = (size_t)nc * 8;
@@ -195,7 +196,7 @@ void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
             &per_cu, reinterpret_cast<const void*>(deposit_ldsgrid_kernel), kThreads, sh));
         int nb = std::max(1, per_cu) * std::max(1, c.num_cu);
         nb = std::min(nb, blocks_for(p.n));
-        deposit_ldsgrid_kernel<<<nb, kThreads, sh, s>>>(a, c.acc.get());
+        deposit_ldsgrid_kernel<<<nb, kThreads, sh, s>>>(a, acc);
         AW_LAUNCH_CHECK();
     } else if (policy == AW_DEPOSIT_BINNED) {
         TileGeom t{};
@@ -222,7 +223,7 @@ void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
         t.H = c.ty + 2 * t.apron + 1;
         t.bpt = tile_bpt(c, p.n);
         const size_t sh = (size_t)t.W * t.H * 8;
-        deposit_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, c.acc.get());
+        deposit_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, acc);
         AW_LAUNCH_CHECK();
     } else {
         fail(AW_ERR_INVALID, "unknown deposit policy " + std::to_string(policy), __FILE__,

@@ -13,6 +13,8 @@
 //                         (32 B when B0 is along z: uz is then invariant).
 #include "aw_internal.hpp"
 
+#include <cstring>
+
 namespace aw {
 
 namespace {
@@ -155,8 +157,10 @@ __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f,
     }
 }
 
+// One workgroup's share of the fused step: blk is the workgroup index within
+// the run (tile blk / bpt, part blk % bpt).
 template <bool PER_W, bool KE, bool UZ>
-__global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
+__device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) {
     constexpr bool kLoadUz = UZ || KE;
     extern __shared__ unsigned long long smem[];
     const int W = a.W, H = a.H, R = W * H;
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
     f.W = W;
     f.H = H;
 
-    const int tile = blockIdx.x / a.bpt, part = blockIdx.x % a.bpt;
+    const int tile = blk / a.bpt, part = blk % a.bpt;
     f.orgx = (tile % a.ntx) * a.tx - a.apron;
     f.orgy = (tile / a.ntx) * a.ty - a.apron;
     for (int c = threadIdx.x; c < R; c += kThreads) {
@@ -236,9 +240,35 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
         if constexpr (KE) {
             double t = 0.0;
             for (int w = 0; w < kThreads / 64; ++w) t += s_ke[w];
-            a.ke_part[blockIdx.x] = t;
+            a.ke_part[blk] = t;
         }
     }
+}
+
+template <bool PER_W, bool KE, bool UZ>
+__global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
+    push_deposit_block<PER_W, KE, UZ>(a, blockIdx.x);
+}
+
+// Ensembles: one launch over every run.  runs[r] holds run r's arguments
+// (pointers into its slices of the concatenated particle / grid arrays) and
+// wg_begin its first workgroup; a workgroup finds its run by binary search.
+struct RunSlot {
+    FusedArgs a;
+    int wg_begin;
+};
+
+template <bool KE, bool UZ>
+__global__ void __launch_bounds__(kThreads)
+push_deposit_multi_kernel(const RunSlot* __restrict__ runs, int nruns) {
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (runs[mid].wg_begin <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const FusedArgs a = runs[lo].a;
+    push_deposit_block<false, KE, UZ>(a, (int)blockIdx.x - runs[lo].wg_begin);
 }
 
 }  // namespace
@@ -270,12 +300,10 @@ void push_es(aw_ctx& c, const aw_particles& p, const float* Ex, const float* Ey,
     AW_LAUNCH_CHECK();
 }
 
-// Returns the number of workgroups launched (size of ke_part).
-int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
-                    const float* Ey, const aw_run_params& rp, bool has_b0,
-                    unsigned long long* strays, double* ke_part, int bpt, hipStream_t s) {
-    if (p.n == 0) return 0;
-    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+static FusedArgs make_fused(aw_ctx& c, const aw_particles& p, const int32_t* off,
+                            const float* Ex, const float* Ey, const aw_run_params& rp,
+                            bool has_b0, unsigned long long* acc, unsigned long long* strays,
+                            double* ke_part, int bpt) {
     FusedArgs a{};
     a.x = p.x;
     a.y = p.y;
@@ -295,7 +323,7 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
     a.ny = c.g.ny;
     a.Ex = Ex;
     a.Ey = Ey;
-    a.acc = c.acc.get();
+    a.acc = acc;
     a.bc = boris_make(rp.qm, rp.dt, rp.B0, has_b0);
     a.cx = (float)(rp.dt / c.g.dx);
     a.cy = (float)(rp.dt / c.g.dy);
@@ -306,6 +334,17 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
     a.scale = std::ldexp(1.0, c.fix_log2);
     a.strays = strays;
     a.ke_part = ke_part;
+    return a;
+}
+
+// Returns the number of workgroups launched (size of ke_part).
+int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
+                    const float* Ey, const aw_run_params& rp, bool has_b0,
+                    unsigned long long* strays, double* ke_part, int bpt, hipStream_t s) {
+    if (p.n == 0) return 0;
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    const FusedArgs a =
+        make_fused(c, p, off, Ex, Ey, rp, has_b0, c.acc.get(), strays, ke_part, bpt);
     const size_t sh = (size_t)a.W * a.H * (8 + 4 + 4);
     const int nblk = c.ntiles * a.bpt;
     const bool pw = p.w != nullptr, ke = ke_part != nullptr;
@@ -331,6 +370,42 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
 // uz changes in the ES push only through a B0 component in the plane
 bool uz_evolves(const aw_run_params& rp, bool has_b0) {
     return has_b0 && (rp.B0[0] != 0.0f || rp.B0[1] != 0.0f);
+}
+
+// ---- ensembles ----------------------------------------------------------------------
+
+size_t run_slot_bytes() { return sizeof(RunSlot); }
+
+// Writes run r's slot (host memory) for push_deposit_multi; returns its
+// workgroup count.  acc, strays and ke_part point at the run's slices.
+int fill_run_slot(void* slot, aw_ctx& c, const aw_particles& p, const int32_t* off,
+                  const float* Ex, const float* Ey, const aw_run_params& rp, bool has_b0,
+                  unsigned long long* acc, unsigned long long* strays, double* ke_part, int bpt,
+                  int wg_begin) {
+    AW_REQUIRE(p.w == nullptr, "ensemble runs carry one weight per run");
+    RunSlot r{};
+    r.a = make_fused(c, p, off, Ex, Ey, rp, has_b0, acc, strays, ke_part, bpt);
+    r.wg_begin = wg_begin;
+    std::memcpy(slot, &r, sizeof(r));
+    return c.ntiles * r.a.bpt;
+}
+
+size_t push_lds_bytes(const aw_ctx& c) {
+    return (size_t)(c.tx + 2 * c.apron + 1) * (c.ty + 2 * c.apron + 1) * (8 + 4 + 4);
+}
+
+void push_deposit_multi(const void* slots, int nruns, int total_wg, size_t lds, bool ke, bool uz,
+                        hipStream_t s) {
+    if (total_wg == 0) return;
+    const RunSlot* r = static_cast<const RunSlot*>(slots);
+    if (uz) {
+        if (ke) push_deposit_multi_kernel<true, true><<<total_wg, kThreads, lds, s>>>(r, nruns);
+        else push_deposit_multi_kernel<false, true><<<total_wg, kThreads, lds, s>>>(r, nruns);
+    } else {
+        if (ke) push_deposit_multi_kernel<true, false><<<total_wg, kThreads, lds, s>>>(r, nruns);
+        else push_deposit_multi_kernel<false, false><<<total_wg, kThreads, lds, s>>>(r, nruns);
+    }
+    AW_LAUNCH_CHECK();
 }
 
 }  // namespace aw

@@ -125,7 +125,8 @@ void enable_em_fft(aw_ctx& c) {
 }
 
 void destroy_fft(aw_ctx& c) noexcept {
-    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7, &c.c2r5})
+    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7, &c.c2r5,
+                           &c.r2cB, &c.c2rB})
         if (*p) {
             rocfft_plan_destroy(*p);
             *p = nullptr;
@@ -155,6 +156,72 @@ static float* r2c_input(aw_ctx& c, const float* rho, hipStream_t s) {
         AW_HIP(hipMemcpyAsync(c.rin.get(), rho, (size_t)ncell(c) * sizeof(float),
                               hipMemcpyDeviceToDevice, s));
     return c.rin.get();
+}
+
+// ---- ensembles: B same-size grids per transform ------------------------------------
+
+namespace {
+
+// es_kspace_kernel over B runs: rk [B][nk] -> ek [B][Ex_k | Ey_k]
+__global__ void es_kspace_batch_kernel(const float2* __restrict__ rk, float2* __restrict__ ek,
+                                       KGeom kg, double eps0, int batch) {
+    const int64_t nk = (int64_t)kg.nkx * kg.ny;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * batch) return;
+    const int r = (int)(t / nk);
+    const int m = (int)(t - (int64_t)r * nk);
+    const int i = m % kg.nkx, j = m / kg.nkx;
+    double kx, ky;
+    kvec(kg, i, j, kx, ky);
+    const double k2 = kx * kx + ky * ky;
+    const double g = k2 > 0.0 ? 1.0 / (eps0 * k2) : 0.0;
+    const float2 v = rk[t];
+    const double pr = (double)v.x * g, pi = (double)v.y * g;
+    float2 ex = make_float2(0.f, 0.f), ey = make_float2(0.f, 0.f);
+    if (!nyquist(kg, i, j)) {
+        ex = make_float2((float)(kx * pi), (float)(-kx * pr));
+        ey = make_float2((float)(ky * pi), (float)(-ky * pr));
+    }
+    ek[(int64_t)r * 2 * nk + m] = ex;
+    ek[(int64_t)r * 2 * nk + nk + m] = ey;
+}
+
+}  // namespace
+
+void enable_batch_fft(aw_ctx& c, int batch) {
+    if (c.fft_batch == batch) return;
+    for (rocfft_plan* p : {&c.r2cB, &c.c2rB})
+        if (*p) {
+            rocfft_plan_destroy(*p);
+            *p = nullptr;
+        }
+    const double inv = 1.0 / ((double)c.g.nx * (double)c.g.ny);
+    c.r2cB = make_plan(c, true, batch, 1.0);
+    c.c2rB = make_plan(c, false, 2 * batch, inv);
+    size_t work = c.fft_work.size();
+    for (rocfft_plan p : {c.r2cB, c.c2rB}) {
+        size_t w = 0;
+        AW_FFT(rocfft_plan_get_work_buffer_size(p, &w));
+        work = std::max(work, w);
+    }
+    if (work > c.fft_work.size()) c.fft_work.resize(work);
+    c.fft_batch = batch;
+}
+
+// rho_in: B grids [B][ny][nx] (rocFFT may overwrite them); E2B: [B][Ex|Ey].
+void solve_es_batch(aw_ctx& c, int batch, float* rho_in, float* E2B, double eps0,
+                    hipStream_t s) {
+    AW_REQUIRE(c.fft_batch == batch, "batched plans not prepared for this batch");
+    const int64_t nk = nkmodes(c);
+    c.ktmp.reserve((size_t)6 * nk * batch);
+    float2* rk = reinterpret_cast<float2*>(c.ktmp.get());
+    float2* ek = rk + nk * batch;
+    fft_exec(c, c.r2cB, rho_in, rk, s);
+    KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
+             6.28318530717958647692 / c.g.Ly};
+    es_kspace_batch_kernel<<<blocks_for(nk * batch), kThreads, 0, s>>>(rk, ek, kg, eps0, batch);
+    AW_LAUNCH_CHECK();
+    fft_exec(c, c.c2rB, ek, E2B, s);
 }
 
 void solve_es_contig(aw_ctx& c, const float* rho, float* E2, double eps0, hipStream_t s) {

@@ -74,6 +74,43 @@ def cpu_baseline(state, g_nx, g_ny, dx, w_uniform, rho, budget_s):
                       f"({n} particles) taken from the GPU state after init"}
 
 
+def ensemble_bench(steps: int, graph: bool = True):
+    """The 24-run batched ensemble (two-beam plasmas, seed 20260629: side in
+    {64..256}, ppc in {16, 64, 256}, vth log-uniform [0.005, 0.05], v0 in
+    [0.1, 0.3], dx 0.5, dt 0.1) stepped as one aw_ensemble versus the same
+    runs stepped one after another by aw_sim.  Loading is outside both timed
+    regions; each timed region ends with a device synchronisation."""
+    from arcwarden_amd.ensemble import Ensemble, random_ensemble
+    from arcwarden_amd.sim import SimConfig, Simulation
+    runs = random_ensemble(dump_every=100)
+    ntot = sum(r.grid.nx * r.grid.ny * sum(s.ppc for s in r.species) for r in runs)
+    seq = 0.0
+    for r in runs:
+        sim = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=20))
+        sim.init()
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        seq += time.perf_counter() - t0
+        sim.close()
+    print(f"# ensemble: sequential {seq:.3f}s for {steps} steps", file=sys.stderr, flush=True)
+    ens = Ensemble(runs, SimConfig(sort_every=20), graph=graph)
+    ens.init()
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run(0, steps)
+    bat = time.perf_counter() - t0
+    info = ens.info()
+    ens.close()
+    print(f"# ensemble: batched {bat:.3f}s (graph={graph})", file=sys.stderr, flush=True)
+    return {"runs": len(runs), "particles": ntot, "steps": steps,
+            "value": ntot * steps / bat, "unit": "particle-steps/s",
+            "sequential_value": ntot * steps / seq, "speedup_vs_sequential": seq / bat,
+            "graph": bool(graph), "size_groups": info["ngroups"],
+            "workload": "24 two-beam ES runs, seed 20260629, sides 64-256, ppc 16/64/256, "
+                        "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +121,8 @@ def main():
     ap.add_argument("--sort-every", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="steps of the batched-ensemble comparison (0: skip)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default tile / workgroups-per-tile heuristic")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -147,6 +186,7 @@ def main():
     print(f"# timed {args.steps} steps in {elapsed:.4f}s; push_deposit avg "
           f"{avg_kernel_s * 1e3:.4f} ms over {launches} launches; sorts={sorts}",
           file=sys.stderr, flush=True)
+    ens = ensemble_bench(args.ensemble_steps) if args.ensemble_steps > 0 else None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(state, nx, ny, dx, state["w_uniform"], rho0, args.cpu_seconds)
@@ -177,6 +217,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch,
                      "avg_kernel_ms": avg_kernel_s * 1e3},
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "checks": {"charge_drift": charge_drift, "sorts": sorts},
     }
     print(json.dumps(out), flush=True)

@@ -425,6 +425,88 @@ aw_status aw_sim_kernel_time(aw_sim* sim, double* total_ms, int64_t* launches,
 aw_status aw_sim_exec_config(aw_sim* sim, int32_t* tile_x, int32_t* tile_y,
                              int32_t* blocks_per_tile);
 
+/* ---- batched ensembles ------------------------------------------------------------ */
+
+/* No reference counterpart (the reference runs one Simulation per process):
+ * many independent electrostatic runs stepped together.  Runs are grouped
+ * by grid size; one step of the ensemble is one fixed-point conversion over
+ * every run, one batched R2C / k-space / C2R per grid size and ONE fused
+ * push+deposit launch over every run; optionally the step is captured once
+ * in a HIP graph and replayed.  Each run keeps its own grid, RunParams and
+ * species (one weight per run), with the same results as an aw_sim of the
+ * same run.  Requirements: every run shares rp.eps0 and rp.dump_every. */
+typedef struct aw_ensemble aw_ensemble;
+
+typedef struct aw_run_desc {
+    aw_grid grid;
+    aw_run_params rp;
+    const aw_species* species; /* read during aw_ensemble_create only */
+    int32_t nspecies;
+    int32_t has_b0;            /* SimConfig::has_b0 of this run */
+    int32_t n_modes;
+    double mode_amp[32];       /* x-displacement modes, as aw_sim_config */
+} aw_run_desc;
+
+typedef struct aw_ensemble_stats {
+    int32_t nruns, ngroups;
+    int64_t particles, cells;
+    int32_t workgroups, graph;
+    int64_t sorts, graph_launches;
+} aw_ensemble_stats;
+
+/* cfg: tile_x (0 -> 8 at >= 64 particles per cell, else 16), blocks_per_tile,
+ * sort_every (0 -> 20); the other members are ignored. */
+aw_status aw_ensemble_create(const aw_run_desc* runs, int32_t nruns, const aw_sim_config* cfg,
+                             aw_ensemble** out);
+aw_status aw_ensemble_destroy(aw_ensemble* e);
+/* 1: capture the step into HIP graphs at aw_ensemble_init and replay them. */
+aw_status aw_ensemble_set_graph(aw_ensemble* e, int32_t on);
+/* Loads every run, t=0 solve, leapfrog half step back. */
+aw_status aw_ensemble_init(aw_ensemble* e);
+/* Step n of every run (diagnostics when n % rp.dump_every == 0). */
+aw_status aw_ensemble_step(aw_ensemble* e, int64_t n);
+/* Steps first .. first+nsteps-1, then synchronises. */
+aw_status aw_ensemble_run(aw_ensemble* e, int64_t first, int64_t nsteps);
+aw_status aw_ensemble_synchronize(aw_ensemble* e);
+aw_status aw_ensemble_info(aw_ensemble* e, aw_ensemble_stats* out);
+/* Per run (index in the caller's order): its particles, Ex/Ey, rho^n and
+ * DiagSample history (blocking). */
+aw_status aw_ensemble_particles(aw_ensemble* e, int32_t run, aw_particles* out);
+aw_status aw_ensemble_fields(aw_ensemble* e, int32_t run, aw_fields* out);
+aw_status aw_ensemble_rho(aw_ensemble* e, int32_t run, float** out);
+aw_status aw_ensemble_history(aw_ensemble* e, int32_t run, aw_diag_sample* out, int64_t cap,
+                              int64_t* count);
+
+/* ---- adjoint (gradients of the field energy) -------------------------------------- */
+
+/* No reference counterpart (the reference computes no gradients): the
+ * discrete adjoint of the electrostatic run without B0.  The forward run is
+ * this library's ES step (fused push + fixed-point deposit, spectral Poisson
+ * solve) from a species load with x-displacement modes
+ * x += sum_m A_m sin(m 2 pi x / Lx) (physical units, m = 1..n_modes); the
+ * loss is EE(N) = eps0/2 sum |E^N|^2 dx dy.  The reverse sweep recomputes
+ * each segment of checkpoint_every steps from its checkpoint.  One particle
+ * weight for all species; uz does not enter the run. */
+typedef struct aw_adjoint aw_adjoint;
+
+/* tile: 0 -> 8 at >= 64 particles per cell, else 16. */
+aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
+                        int32_t nspecies, int32_t n_modes, int32_t checkpoint_every,
+                        int32_t tile, aw_adjoint** out);
+aw_status aw_adj_destroy(aw_adjoint* a);
+/* Loads with amplitudes mode_amp[n_modes] and drifts ufl[3*nspecies] (NULL:
+ * the species' own), runs nsteps steps keeping checkpoints, returns the loss. */
+aw_status aw_adj_forward(aw_adjoint* a, const double* mode_amp, const double* ufl,
+                         int64_t nsteps, double* loss);
+/* dL/dA_m (n_modes values) and dL/dufl (3 per species; the z entries are 0). */
+aw_status aw_adj_backward(aw_adjoint* a, double* grad_modes, double* grad_ufl);
+/* After aw_adj_backward: dL/d(x, y, ux, uy) of the initial state per particle
+ * (load order, x and y in cells; device arrays of n floats, NULL skips). */
+aw_status aw_adj_state_grad(aw_adjoint* a, float* gx, float* gy, float* gux, float* guy);
+/* After aw_adj_forward: the initial state (x, y, ux, uy before the leapfrog
+ * half step back, load order) into out (out->n = particle count). */
+aw_status aw_adj_initial_state(aw_adjoint* a, aw_particles* out);
+
 #ifdef __cplusplus
 }
 #endif

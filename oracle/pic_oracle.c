@@ -983,3 +983,202 @@ int32_t orc_es_step_f32(const orc_grid* g, const orc_push_params* pp, double eps
     free(acc);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* electrostatic forward in FP64 and its discrete adjoint (no B0)            */
+/* ------------------------------------------------------------------------ */
+
+/* CIC weights and their x / y derivatives (cell units) at (x, y). */
+typedef struct {
+    int32_t c[4];
+    double w[4], dwx[4], dwy[4];
+} cic_grad;
+
+static inline cic_grad cic_grad_d(const orc_grid* g, double x, double y) {
+    cic_grad s;
+    const int32_t i0 = (int32_t)floor(x), j0 = (int32_t)floor(y);
+    const double fx = x - i0, fy = y - j0, gx = 1.0 - fx, gy = 1.0 - fy;
+    const int32_t a = wrapi(i0, g->nx), b = wrapi(i0 + 1, g->nx);
+    const int32_t c = wrapi(j0, g->ny), d = wrapi(j0 + 1, g->ny);
+    s.c[0] = c * g->nx + a; s.w[0] = gx * gy; s.dwx[0] = -gy; s.dwy[0] = -gx;
+    s.c[1] = c * g->nx + b; s.w[1] = fx * gy; s.dwx[1] = gy;  s.dwy[1] = -fx;
+    s.c[2] = d * g->nx + a; s.w[2] = gx * fy; s.dwx[2] = -fy; s.dwy[2] = gx;
+    s.c[3] = d * g->nx + b; s.w[3] = fx * fy; s.dwx[3] = fy;  s.dwy[3] = fx;
+    return s;
+}
+
+/* grid[c] += sum_p v_p W_c(x_p) over particles, FP64, per-thread grids. */
+static void scatter_dd(const orc_grid* g, const double* x, const double* y, const double* v,
+                       double v_const, int64_t n, double* grid, int nthreads) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    double* priv = (double*)calloc((size_t)nc * (size_t)nthreads, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        double* my = priv + (size_t)omp_get_thread_num() * (size_t)nc;
+#else
+        double* my = priv;
+#endif
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; ++p) {
+            const cic_grad s = cic_grad_d(g, x[p], y[p]);
+            const double vp = v ? v[p] : v_const;
+            for (int k = 0; k < 4; ++k) my[s.c[k]] += vp * s.w[k];
+        }
+    }
+    for (int t = 0; t < nthreads; ++t)
+        for (int64_t c = 0; c < nc; ++c) grid[c] += priv[(size_t)t * nc + c];
+    free(priv);
+}
+
+/* Adjoint of the ES solve: lrho = S^T (lEx, lEy), S^T = IFFT(conj(H) FFT(.)),
+ * H = -i k /(eps0 k^2) with DC and Nyquist zero (the transpose of
+ * orc_solve_es_f64 under the real inner product). */
+void orc_solve_es_adj_f64(const orc_grid* g, double eps0, const double* lEx,
+                          const double* lEy, double* lrho) {
+    const int nx = g->nx, ny = g->ny, nkx = nx / 2 + 1;
+    const size_t nk = (size_t)nkx * ny;
+    double* ax = (double*)malloc(sizeof(double) * 2 * nk);
+    double* ay = (double*)malloc(sizeof(double) * 2 * nk);
+    double* r = (double*)malloc(sizeof(double) * 2 * nk);
+    orc_fft2_r2c(nx, ny, lEx, ax);
+    orc_fft2_r2c(nx, ny, lEy, ay);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nkx; ++i) {
+            const size_t t = (size_t)j * nkx + i;
+            double kx, ky;
+            kvec(g, i, j, &kx, &ky);
+            const double k2 = kx * kx + ky * ky;
+            const double gr = k2 > 0.0 ? 1.0 / (eps0 * k2) : 0.0;
+            if (is_nyquist(g, i, j)) {
+                r[2 * t] = r[2 * t + 1] = 0.0;
+            } else {  /* i G (kx ax + ky ay) */
+                r[2 * t] = -gr * (kx * ax[2 * t + 1] + ky * ay[2 * t + 1]);
+                r[2 * t + 1] = gr * (kx * ax[2 * t] + ky * ay[2 * t]);
+            }
+        }
+    orc_fft2_c2r(nx, ny, r, lrho);
+    const double s = 1.0 / ((double)nx * ny);
+    for (size_t c = 0; c < (size_t)nx * ny; ++c) lrho[c] *= s;
+    free(ax); free(ay); free(r);
+}
+
+/* The ES run of Simulation<Cfg> without B0, in FP64: rho^0 = D(x^0),
+ * E^0 = S rho^0, u^{-1/2} = u^0 - (qm dt/2) e(x^0); then per step
+ * u += qm dt e(x^n, E^n), x = wrap(x + u dt/dx), rho = D(x), E = S rho;
+ * loss = EE(N) = eps0/2 sum |E^N|^2 dx dy.  With gx != NULL also the
+ * discrete adjoint: gx, gy = dL/dx^0, dL/dy^0 (cell units), gux, guy =
+ * dL/du^0.  States x^n and fields E^n are kept for the reverse sweep. */
+double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
+                          double w_uniform, const double* x0, const double* y0,
+                          const double* ux0, const double* uy0, int64_t n, int32_t nsteps,
+                          double* gx, double* gy, double* gux, double* guy, int32_t nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const double area = g->dx * g->dy, coef = qm * w_uniform / area;
+    const double a = qm * dt, cx = dt / g->dx, cy = dt / g->dy;
+    const int adj = gx != NULL;
+    const int64_t keep = adj ? (int64_t)nsteps + 1 : 1;
+    double* X = (double*)malloc(sizeof(double) * (size_t)(n * keep));
+    double* Y = (double*)malloc(sizeof(double) * (size_t)(n * keep));
+    double* EX = (double*)malloc(sizeof(double) * (size_t)(nc * keep));
+    double* EY = (double*)malloc(sizeof(double) * (size_t)(nc * keep));
+    double* ux = (double*)malloc(sizeof(double) * (size_t)n);
+    double* uy = (double*)malloc(sizeof(double) * (size_t)n);
+    double* rho = (double*)malloc(sizeof(double) * (size_t)nc);
+    memcpy(X, x0, sizeof(double) * (size_t)n);
+    memcpy(Y, y0, sizeof(double) * (size_t)n);
+    memcpy(ux, ux0, sizeof(double) * (size_t)n);
+    memcpy(uy, uy0, sizeof(double) * (size_t)n);
+    const double fnx = (double)g->nx, fny = (double)g->ny;
+    for (int32_t s = 0; s <= nsteps; ++s) {
+        const int64_t ks = adj ? s : 0;
+        double* xs = X + n * ks;
+        double* ysv = Y + n * ks;
+        memset(rho, 0, sizeof(double) * (size_t)nc);
+        scatter_dd(g, xs, ysv, NULL, coef, n, rho, nthreads);
+        double* ex = EX + nc * ks;
+        double* ey = EY + nc * ks;
+        orc_solve_es_f64(g, eps0, rho, ex, ey);
+        if (s == nsteps) break;
+        const double kick = s == 0 ? 0.5 : 1.0;  /* half step back folded into step 0 */
+        double* xn = adj ? X + n * (ks + 1) : X;
+        double* yn = adj ? Y + n * (ks + 1) : Y;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t p = 0; p < n; ++p) {
+            const cic_grad c = cic_grad_d(g, xs[p], ysv[p]);
+            double e0 = 0.0, e1 = 0.0;
+            for (int k = 0; k < 4; ++k) { e0 += c.w[k] * ex[c.c[k]]; e1 += c.w[k] * ey[c.c[k]]; }
+            /* u^{-1/2} = u^0 - a/2 e, then u^{1/2} = u^{-1/2} + a e: net a/2 e at step 0 */
+            ux[p] += kick * a * e0;
+            uy[p] += kick * a * e1;
+            xn[p] = wrap_d(xs[p] + cx * ux[p], fnx);
+            yn[p] = wrap_d(ysv[p] + cy * uy[p], fny);
+        }
+    }
+    const double* exN = EX + nc * (adj ? nsteps : 0);
+    const double* eyN = EY + nc * (adj ? nsteps : 0);
+    double loss = 0.0;
+    for (int64_t c = 0; c < nc; ++c) loss += exN[c] * exN[c] + eyN[c] * eyN[c];
+    loss *= 0.5 * eps0 * area;
+    if (adj) {
+        double* lEx = (double*)malloc(sizeof(double) * (size_t)nc);
+        double* lEy = (double*)malloc(sizeof(double) * (size_t)nc);
+        double* lrho = (double*)malloc(sizeof(double) * (size_t)nc);
+        double* lex = (double*)malloc(sizeof(double) * (size_t)n);
+        double* ley = (double*)malloc(sizeof(double) * (size_t)n);
+        for (int64_t c = 0; c < nc; ++c) { lEx[c] = eps0 * area * exN[c]; lEy[c] = eps0 * area * eyN[c]; }
+        orc_solve_es_adj_f64(g, eps0, lEx, lEy, lrho);
+        const double* xN = X + n * nsteps;
+        const double* yN = Y + n * nsteps;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t p = 0; p < n; ++p) {
+            const cic_grad c = cic_grad_d(g, xN[p], yN[p]);
+            double sx = 0.0, sy = 0.0;
+            for (int k = 0; k < 4; ++k) { sx += c.dwx[k] * lrho[c.c[k]]; sy += c.dwy[k] * lrho[c.c[k]]; }
+            gx[p] = coef * sx;
+            gy[p] = coef * sy;
+            gux[p] = 0.0;
+            guy[p] = 0.0;
+        }
+        for (int32_t s = nsteps - 1; s >= 0; --s) {
+            const double* xs = X + n * s;
+            const double* ysv = Y + n * s;
+            const double* ex = EX + nc * s;
+            const double* ey = EY + nc * s;
+            const double kick = s == 0 ? 0.5 : 1.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t p = 0; p < n; ++p) {
+                gux[p] += cx * gx[p];
+                guy[p] += cy * gy[p];
+                const double le0 = kick * a * gux[p], le1 = kick * a * guy[p];
+                const cic_grad c = cic_grad_d(g, xs[p], ysv[p]);
+                double dxx = 0, dxy = 0, dyx = 0, dyy = 0;  /* d e_comp / d pos */
+                for (int k = 0; k < 4; ++k) {
+                    dxx += c.dwx[k] * ex[c.c[k]]; dxy += c.dwy[k] * ex[c.c[k]];
+                    dyx += c.dwx[k] * ey[c.c[k]]; dyy += c.dwy[k] * ey[c.c[k]];
+                }
+                gx[p] += le0 * dxx + le1 * dyx;
+                gy[p] += le0 * dxy + le1 * dyy;
+                lex[p] = le0;
+                ley[p] = le1;
+            }
+            memset(lEx, 0, sizeof(double) * (size_t)nc);
+            memset(lEy, 0, sizeof(double) * (size_t)nc);
+            scatter_dd(g, xs, ysv, lex, 0.0, n, lEx, nthreads);
+            scatter_dd(g, xs, ysv, ley, 0.0, n, lEy, nthreads);
+            orc_solve_es_adj_f64(g, eps0, lEx, lEy, lrho);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t p = 0; p < n; ++p) {
+                const cic_grad c = cic_grad_d(g, xs[p], ysv[p]);
+                double sx = 0.0, sy = 0.0;
+                for (int k = 0; k < 4; ++k) { sx += c.dwx[k] * lrho[c.c[k]]; sy += c.dwy[k] * lrho[c.c[k]]; }
+                gx[p] += coef * sx;
+                gy[p] += coef * sy;
+            }
+        }
+        free(lEx); free(lEy); free(lrho); free(lex); free(ley);
+    }
+    free(X); free(Y); free(EX); free(EY); free(ux); free(uy); free(rho);
+    return loss;
+}

@@ -48,9 +48,10 @@ def test_struct_layouts_match_header_sizes():
     src = r"""
 #include <stdio.h>
 #include "aw.h"
-int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
  sizeof(aw_run_params), sizeof(aw_species), sizeof(aw_particles), sizeof(aw_fields),
- sizeof(aw_sources), sizeof(aw_diag_sample), sizeof(aw_sim_config), sizeof(aw_deck));
+ sizeof(aw_sources), sizeof(aw_diag_sample), sizeof(aw_sim_config), sizeof(aw_deck),
+ sizeof(aw_run_desc), sizeof(aw_ensemble_stats));
  return 0;}
 """
     with tempfile.TemporaryDirectory() as d:
@@ -62,5 +63,6 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(aw_grid),
                                                 text=True).stdout.split()]
     py = [ctypes.sizeof(t) for t in (_lib.AwGrid, _lib.AwRunParams, _lib.AwSpecies,
                                      _lib.AwParticles, _lib.AwFields, _lib.AwSources,
-                                     _lib.AwDiagSample, _lib.AwSimConfig, _lib.AwDeck)]
+                                     _lib.AwDiagSample, _lib.AwSimConfig, _lib.AwDeck,
+                                     _lib.AwRunDesc, _lib.AwEnsembleStats)]
     assert py == sizes

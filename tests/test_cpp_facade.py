@@ -61,7 +61,7 @@ def test_run_deck(gpu, tmp_path, em):
     man = list(csv.DictReader(open(out / "manifest.csv")))
     assert [int(m["step"]) for m in man] == [0, 25, 50, 75, 100]
     frame = list(csv.DictReader(open(out / man[-1]["file"])))
-    assert len(frame) == 64 * 8 * 256
+    assert len(frame) == min(64 * 8 * 256, 100000)  # run_deck samples at most 100000
     assert all(0.0 <= float(f["x"]) < 10.2633 + 1e-4 for f in frame[:1000])
     energy = list(csv.DictReader(open(out / "energy.csv")))
     assert [int(e["step"]) for e in energy] == [0, 25, 50, 75]

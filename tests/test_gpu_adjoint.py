@@ -1,0 +1,107 @@
+"""GPU adjoint of the electrostatic run: the forward loss equals the engine's
+EE(N) of the same run; the gradients (per particle and per parameter) match
+the FP64 host adjoint (itself pinned to central differences) within 1e-4;
+the checkpoint interval changes nothing; torch.autograd carries them."""
+import numpy as np
+import pytest
+
+from arcwarden_amd import _lib as L
+from arcwarden_amd.adjoint import AdjointRun, field_energy
+from arcwarden_amd.sim import SimConfig, Simulation
+
+from gpu_util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NX, DX, PPC, V0, VTH = 32, 0.5, 16, 0.2, 0.02
+
+
+def _setup(n_modes=4, seed=11, nx=NX, ppc=PPC):
+    g = L.grid(nx, nx, nx * DX, nx * DX)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=seed)
+    sp = [L.species("beam_plus", 0.5, ppc // 2, (VTH,) * 3, (V0, 0, 0)),
+          L.species("beam_minus", 0.5, ppc // 2, (VTH,) * 3, (-V0, 0, 0))]
+    A = 3e-3 * np.random.default_rng(seed).uniform(-1, 1, n_modes)
+    return g, rp, sp, A
+
+
+def test_forward_loss_equals_engine_field_energy(gpu):
+    g, rp, sp, A = _setup()
+    nsteps = 25
+    run = AdjointRun(g, rp, sp, len(A), checkpoint_every=10)
+    loss = run.forward(A, None, nsteps)
+    run.close()
+    rp2 = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=11,
+                       dump_every=nsteps)
+    sim = Simulation(g, rp2, sp, SimConfig(has_b0=False, mode_amp=list(A)))
+    sim.init()
+    for n in range(nsteps + 1):
+        sim.step(n)
+    ee = [h["ee"] for h in sim.history() if h["step"] == nsteps][0]
+    sim.close()
+    assert loss == pytest.approx(ee, rel=1e-12)
+
+
+def test_adjoint_matches_fp64_host_adjoint(gpu, oracle):
+    g, rp, sp, A = _setup()
+    nsteps = 30
+    run = AdjointRun(g, rp, sp, len(A), checkpoint_every=10)
+    loss = run.forward(A, None, nsteps)
+    gA, gU = run.backward()
+    sg = run.state_grad()
+    st = run.initial_state()
+    base = AdjointRun(g, rp, sp, 0, checkpoint_every=10)
+    base.forward(np.zeros(0), None, 1)
+    xb = base.initial_state()["x"].astype(np.float64)
+    base.close()
+    run.close()
+    og = oracle.grid(NX, NX, DX, DX)
+    w = 0.5 * DX * DX / (PPC // 2)
+    loss64, gr = oracle.es_adjoint_f64(og, st, w, nsteps)
+    assert loss == pytest.approx(loss64, rel=1e-4)
+    for k in ("x", "y", "ux", "uy"):
+        assert rel_l2(sg[k], gr[k]) < 1e-3, k
+    k1 = 2 * np.pi / NX
+    gA64 = np.array([np.sum(gr["x"] * np.sin((m + 1) * k1 * xb) / DX) for m in range(len(A))])
+    n = xb.size
+    gu64 = [gr["ux"][: n // 2].sum(), gr["ux"][n // 2:].sum()]
+    print("gA", gA, gA64, "gU", gU[:, 0], gu64)
+    # the gate: dL/dA_m and dL/dv0 (beams at +v0 and -v0) within 1e-4
+    assert np.max(np.abs(gA - gA64)) <= 1e-4 * np.max(np.abs(gA64))
+    dv0, dv0_64 = gU[0, 0] - gU[1, 0], gu64[0] - gu64[1]
+    assert dv0 == pytest.approx(dv0_64, rel=1e-4)
+    # per-beam drift gradients: FP32 forward vs FP64, 5e-4
+    for s in range(2):
+        assert gU[s, 0] == pytest.approx(gu64[s], rel=5e-4)
+
+
+def test_checkpoint_interval_changes_nothing(gpu):
+    g, rp, sp, A = _setup(seed=5)
+    out = []
+    for K in (1, 7, 40):
+        run = AdjointRun(g, rp, sp, len(A), checkpoint_every=K)
+        loss = run.forward(A, None, 20)
+        gA, gU = run.backward()
+        run.close()
+        out.append((loss, gA, gU))
+    for loss, gA, gU in out[1:]:
+        assert loss == out[0][0]
+        assert np.allclose(gA, out[0][1], rtol=1e-5, atol=1e-9 * np.max(np.abs(out[0][1])))
+        assert np.allclose(gU, out[0][2], rtol=1e-5, atol=1e-9 * np.max(np.abs(out[0][2])))
+
+
+def test_torch_autograd_function(gpu):
+    import torch
+    g, rp, sp, A = _setup(seed=7)
+    run = AdjointRun(g, rp, sp, len(A), checkpoint_every=5)
+    At = torch.tensor(A, dtype=torch.float64, requires_grad=True)
+    U = torch.tensor([[V0, 0, 0], [-V0, 0, 0]], dtype=torch.float64, requires_grad=True)
+    loss = field_energy(At, U, run, 15)
+    (2.0 * loss).backward()
+    gA, gU = run.backward()
+    assert np.allclose(At.grad.numpy(), 2.0 * gA)
+    assert np.allclose(U.grad.numpy(), 2.0 * gU)
+    # dL/dv0 for a symmetric drift change (+dv0 on beam +, -dv0 on beam -)
+    dv0 = U.grad[0, 0] - U.grad[1, 0]
+    assert torch.isfinite(dv0)
+    run.close()

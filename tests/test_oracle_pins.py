@@ -480,3 +480,41 @@ def test_esirkepov_jz_is_time_averaged_shape(oracle):
     _, _, jz = oracle.esirkepov_f64(g, [x0], [y0], [x0], [y0], [uz], qm, dt, w=[w])
     rho = oracle.rho_nodes_f64(g, [x0], [y0], qm, w=[w])
     assert np.max(np.abs(jz - uz * rho)) < 1e-12
+
+
+# ---- discrete adjoint of the ES run ---------------------------------------------------
+
+def test_es_adjoint_matches_central_differences(oracle):
+    """The FP64 adjoint (dL/dA_m through x^0, dL/dv0 through u^0) against
+    central differences of the FP64 forward run, L = EE(30)."""
+    nx, dx, nsteps, M = 16, 0.5, 30, 4
+    g = oracle.grid(nx, nx, dx, dx)
+    v0, vth = 0.2, 0.02
+    st, (w, _) = oracle.load_species(g, [(0.5, 8, (vth,) * 3, (v0, 0, 0)),
+                                         (0.5, 8, (vth,) * 3, (-v0, 0, 0))], True, 3)
+    n = st["x"].size
+    sign = np.r_[np.ones(n // 2), -np.ones(n // 2)]
+    xb, yb = st["x"].astype(np.float64), st["y"].astype(np.float64)
+    th = st["ux"].astype(np.float64) - sign * v0
+    k1 = 2 * np.pi / nx
+    A = 1e-2 * np.random.default_rng(1).uniform(-1, 1, M)
+
+    def state(A, v0):
+        d = sum(A[m] * np.sin((m + 1) * k1 * xb) / dx for m in range(M))
+        return {"x": np.mod(xb + d, nx), "y": yb, "ux": th + sign * v0,
+                "uy": st["uy"].astype(np.float64)}
+
+    loss, gr = oracle.es_adjoint_f64(g, state(A, v0), w, nsteps)
+    assert loss > 0
+    gA = np.array([np.sum(gr["x"] * np.sin((m + 1) * k1 * xb) / dx) for m in range(M)])
+    gv = np.sum(gr["ux"] * sign)
+    h = 1e-7
+    f = lambda s: oracle.es_adjoint_f64(g, s, w, nsteps, grad=False)  # noqa: E731
+    for m in range(M):
+        Ap, Am = A.copy(), A.copy()
+        Ap[m] += h
+        Am[m] -= h
+        fd = (f(state(Ap, v0)) - f(state(Am, v0))) / (2 * h)
+        assert abs(fd - gA[m]) <= 1e-4 * abs(gA[m]) + 1e-12, m
+    fd = (f(state(A, v0 + h)) - f(state(A, v0 - h))) / (2 * h)
+    assert abs(fd - gv) <= 1e-4 * abs(gv)
