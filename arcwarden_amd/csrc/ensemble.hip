@@ -10,7 +10,8 @@
 //   3 launches per grid size: R2C over the size's runs, k-space, C2R (Ex|Ey),
 //   1 launch   fused gather + Boris + move + wrap + deposit over every run
 //              (each workgroup finds its run in a device-resident run table),
-// with the stable tile re-sort of every run on a fixed cadence.  The step can
+// with the stable tile re-sort of every run on a cadence, or when more than a
+// fraction of the particles strayed from their tile apron.  The step can
 // be captured once into a HIP graph and replayed (two graphs: with and
 // without the diagnostics reduction); the run table lives in device memory,
 // so the re-sort's buffer swap only rewrites the table, not the graphs.
@@ -126,12 +127,25 @@ struct aw_ensemble {
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // [dump]
     int64_t graph_launches = 0;
+    // stray feedback (read back one step late, without blocking)
+    unsigned long long* stray_host = nullptr;
+    hipEvent_t stray_ev = nullptr;
+    bool stray_pending = false, sort_requested = false;
 
     ~aw_ensemble() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& gx : graph)
             if (gx) (void)hipGraphExecDestroy(gx);
+        if (stray_ev) (void)hipEventDestroy(stray_ev);
+        if (stray_host) (void)hipHostFree(stray_host);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void poll_strays() {
+        if (!stray_pending || hipEventQuery(stray_ev) != hipSuccess) return;
+        stray_pending = false;
+        const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
+        if ((double)*stray_host > frac * (double)ntot) sort_requested = true;
     }
 
     aw_particles run_particles(const Run& r, int which) const {
@@ -251,8 +265,11 @@ struct aw_ensemble {
         if (!initialized) fail(AW_ERR_STATE, "aw_ensemble_step before aw_ensemble_init",
                                __FILE__, __LINE__);
         const hipStream_t s = stream;
-        const int every = cfg.sort_every > 0 ? cfg.sort_every : 20;
-        if (steps_since_sort >= every) sort_all();
+        poll_strays();
+        if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) {
+            sort_all();
+            sort_requested = false;
+        }
         const bool dump = cfg_dump(nstep);
         if (fields_ready) {
             // the init solve already holds E^0: push only (first step)
@@ -268,6 +285,12 @@ struct aw_ensemble {
             ++graph_launches;
         } else {
             enqueue_step(dump);
+        }
+        if (!stray_pending) {  // this step's stray count (reset by the next conversion)
+            AW_HIP(hipMemcpyAsync(stray_host, strays.get(), sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+            AW_HIP(hipEventRecord(stray_ev, s));
+            stray_pending = true;
         }
         if (dump) record_diag(nstep);
         ++steps_since_sort;
@@ -432,6 +455,9 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         AW_HIP(hipMemcpy(e->diag_runs.get(), dr.data(), R * sizeof(DiagRun),
                          hipMemcpyHostToDevice));
         e->build_slots();
+        AW_HIP(hipHostMalloc((void**)&e->stray_host, sizeof(unsigned long long),
+                             hipHostMallocDefault));
+        AW_HIP(hipEventCreateWithFlags(&e->stray_ev, hipEventDisableTiming));
         *out = e.release();
     });
 }

@@ -193,9 +193,20 @@ __global__ void key_offsets_kernel(const uint32_t* __restrict__ skeys, int n, ui
 // radix + permute path: one histogram pass over (x, y), one scan of the
 // chunk x tile count table, and one scatter that ranks each particle inside
 // its chunk with wavefront ballots and writes it straight to its final slot.
+// A chunk belongs to one single-wave workgroup: ranking needs no cross-wave
+// bookkeeping, and the LDS table is one 32-bit counter per tile, so many
+// workgroups stay resident per CU to hide the memory latency.
 
-constexpr int kTsRounds = 128;                   // 256 x 128 = 32768 particles / chunk
-constexpr int kTsChunk = kThreads * kTsRounds;
+constexpr int kTsWave = 64;
+
+// rounds of 64 particles per chunk: at least 16, more when the chunk x tile
+// count table would exceed 2^24 entries
+inline int ts_rounds(int64_t n, int ntiles) {
+    int64_t r = 16;
+    while (r < 4096 && ((n + kTsWave * r - 1) / (kTsWave * r)) * (int64_t)ntiles > (1ll << 24))
+        r *= 2;
+    return (int)r;
+}
 
 __device__ __forceinline__ int tile_id(float x, float y, int nx, int ny, int tx, int ty,
                                        int ntx) {
@@ -203,31 +214,32 @@ __device__ __forceinline__ int tile_id(float x, float y, int nx, int ny, int tx,
     return ((c / nx) / ty) * ntx + (c % nx) / tx;
 }
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kTsWave)
 ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, int nx, int ny,
-               int tx, int ty, int ntx, int ntiles, int nchunks, uint32_t* __restrict__ table) {
+               int tx, int ty, int ntx, int ntiles, int nchunks, int rounds,
+               uint32_t* __restrict__ table) {
     extern __shared__ uint32_t h[];
-    for (int t = threadIdx.x; t < ntiles; t += kThreads) h[t] = 0;
+    const int lane = threadIdx.x;
+    for (int t = lane; t < ntiles; t += kTsWave) h[t] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kTsChunk, end = min(n, base + kTsChunk);
-    const int lane = threadIdx.x & 63;
-    for (int i0 = base; i0 < end; i0 += kThreads) {
-        const int i = i0 + (int)threadIdx.x;
+    const int base = blockIdx.x * kTsWave * rounds;
+    const int end = min(n, base + kTsWave * rounds);
+    for (int i0 = base; i0 < end; i0 += kTsWave) {
+        const int i = i0 + lane;
         const bool valid = i < end;
         const int t = valid ? tile_id(x[i], y[i], nx, ny, tx, ty, ntx) : -1;
-        // aggregate equal tiles across the wave: one LDS atomic per distinct tile
+        // one LDS update per distinct tile of the wave (the wave owns h)
         unsigned long long todo = __ballot(valid);
         while (todo) {
             const int leader = __ffsll((long long)todo) - 1;
             const int lt = __shfl(t, leader, 64);
-            const unsigned long long m = __ballot(valid && t == lt) & todo;
-            if (lane == leader) atomicAdd(&h[lt], (uint32_t)__popcll(m));
-            todo &= ~m;
+            const unsigned long long mk = __ballot(valid && t == lt) & todo;
+            if (lane == leader) h[lt] += (uint32_t)__popcll(mk);
+            todo &= ~mk;
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kThreads)
-        table[(size_t)t * nchunks + blockIdx.x] = h[t];
+    for (int t = lane; t < ntiles; t += kTsWave) table[(size_t)t * nchunks + blockIdx.x] = h[t];
 }
 
 struct TsMove {
@@ -236,23 +248,18 @@ struct TsMove {
     int32_t* perm;  // optional: perm[pos] = source index
 };
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kTsWave)
 ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int ntiles,
-                  int nchunks, int tbits, const uint32_t* __restrict__ table) {
-    extern __shared__ uint32_t sh[];
-    uint32_t* base = sh;                         // ntiles running offsets
-    uint32_t* wst = sh + ntiles;                 // [4][ntiles] (round << 16 | count)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int t = tid; t < ntiles; t += kThreads) {
-        base[t] = table[(size_t)t * nchunks + blockIdx.x];
-        for (int w = 0; w < kThreads / 64; ++w) wst[w * ntiles + t] = 0xffffffffu;
-    }
+                  int nchunks, int rounds, int tbits, const uint32_t* __restrict__ table) {
+    extern __shared__ uint32_t base[];  // running output offset per tile
+    const int lane = threadIdx.x;
+    for (int t = lane; t < ntiles; t += kTsWave) base[t] = table[(size_t)t * nchunks + blockIdx.x];
     __syncthreads();
-    const int cbase = blockIdx.x * kTsChunk;
-    const int rounds = min(kTsRounds, (n - cbase + kThreads - 1) / kThreads);
+    const int cbase = blockIdx.x * kTsWave * rounds;
+    const int nr = min(rounds, (n - cbase + kTsWave - 1) / kTsWave);
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int r = 0; r < rounds; ++r) {
-        const int i = cbase + r * kThreads + tid;
+    for (int r = 0; r < nr; ++r) {
+        const int i = cbase + r * kTsWave + lane;
         const bool valid = i < n;
         float x = 0.f, y = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, w = 0.f;
         int t = 0;
@@ -265,6 +272,7 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
             if (m.w) w = m.w[i];
             t = tile_id(x, y, nx, ny, tx, ty, ntx);
         }
+        // lanes with the same tile, and this lane's rank among them
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < tbits; ++b) {
             const bool bit = (t >> b) & 1;
@@ -272,15 +280,8 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
             peers &= bit ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & below);
-        const uint32_t cnt = (uint32_t)__popcll(peers);
-        if (valid && rank == 0) wst[wave * ntiles + t] = ((uint32_t)r << 16) | cnt;
-        __syncthreads();
         if (valid) {
-            uint32_t pos = base[t] + rank;
-            for (int w2 = 0; w2 < wave; ++w2) {
-                const uint32_t s = wst[w2 * ntiles + t];
-                if ((s >> 16) == (uint32_t)r) pos += s & 0xffffu;
-            }
+            const uint32_t pos = base[t] + rank;
             m.xo[pos] = x;
             m.yo[pos] = y;
             m.uxo[pos] = ux;
@@ -289,8 +290,8 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
             if (m.wo) m.wo[pos] = w;
             if (m.perm) m.perm[pos] = i;
         }
-        __syncthreads();
-        if (valid && rank == 0) atomicAdd(&base[t], cnt);
+        __syncthreads();  // every lane has read base[] before the leaders advance it
+        if (valid && rank == 0) base[t] += (uint32_t)__popcll(peers);
         __syncthreads();
     }
 }
@@ -534,7 +535,7 @@ void stable_sort_keys(aw_ctx& c, int64_t n, uint32_t nkeys, int32_t* perm, int32
 // when the per-tile LDS tables do not fit (caller uses the radix path).
 bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* off,
                        int32_t* perm, hipStream_t s) {
-    const size_t lds = (size_t)c.ntiles * 4 * (1 + kThreads / 64);
+    const size_t lds = (size_t)c.ntiles * 4;
     if (lds > c.lds_per_block) return false;
     AW_REQUIRE(src.n < (int64_t)INT32_MAX, "particle count must fit int32");
     const int n = (int)src.n;
@@ -542,12 +543,14 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
         AW_HIP(hipMemsetAsync(off, 0, ((size_t)c.ntiles + 1) * sizeof(int32_t), s));
         return true;
     }
-    const int nchunks = (n + kTsChunk - 1) / kTsChunk;
+    const int rounds = ts_rounds(n, c.ntiles);
+    const int nchunks = (int)(((int64_t)n + kTsWave * rounds - 1) / (kTsWave * rounds));
     int tbits = 0;
     while ((1 << tbits) < c.ntiles) ++tbits;
     c.sort.table.reserve((size_t)c.ntiles * nchunks);
-    ts_hist_kernel<<<nchunks, kThreads, (size_t)c.ntiles * 4, s>>>(
-        src.x, src.y, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx, c.ntiles, nchunks, c.sort.table.get());
+    ts_hist_kernel<<<nchunks, kTsWave, lds, s>>>(src.x, src.y, n, c.g.nx, c.g.ny, c.tx, c.ty,
+                                                 c.ntx, c.ntiles, nchunks, rounds,
+                                                 c.sort.table.get());
     AW_LAUNCH_CHECK();
     exclusive_scan(c, c.sort.table.get(), (int64_t)c.ntiles * nchunks, s);
     ts_offsets_kernel<<<blocks_for((int64_t)c.ntiles + 1), kThreads, 0, s>>>(
@@ -555,8 +558,9 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
     AW_LAUNCH_CHECK();
     TsMove m{src.x, src.y, src.ux, src.uy, src.uz, src.w,
              dst.x, dst.y, dst.ux, dst.uy, dst.uz, dst.w, perm};
-    ts_scatter_kernel<<<nchunks, kThreads, lds, s>>>(m, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx,
-                                                     c.ntiles, nchunks, tbits, c.sort.table.get());
+    ts_scatter_kernel<<<nchunks, kTsWave, lds, s>>>(m, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx,
+                                                    c.ntiles, nchunks, rounds, tbits,
+                                                    c.sort.table.get());
     AW_LAUNCH_CHECK();
     return true;
 }

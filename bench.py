@@ -5,8 +5,10 @@ Workload (the measured configuration): ES thermal plasma, 512x512 periodic
 grid, dx = 0.5, CIC, 256 electrons per cell (67,108,864), Maxwellian
 v_th = 0.05 in 3 components, uniform B0 = 0.5 z (magnetised Boris), FP32 SoA
 particles, dt = 0.1.  A step is one Simulation::step: rho^n -> FFT Poisson ->
-fused gather+push+move+wrap+deposit, with the stable tile re-sort every 20
-steps and the scalar diagnostics every 10 steps, all inside the timed region.
+fused gather+push+move+wrap+deposit, the stable tile re-sort whenever more than
+2% of the particles left their tile apron (--sort-every N: every N steps), and
+the scalar diagnostics every 10 steps, all inside the timed region.  The same
+run with a re-sort every 20 steps is reported beside it.
 
 Prints ONE JSON line.  ``value`` is particle-steps/s over the whole timed
 region; ``roofline`` prices the dominant kernel (push_deposit) at its
@@ -14,7 +16,10 @@ algorithmic particle traffic against 8 TB/s: x, y, ux, uy read and written
 (32 B per particle; with B0 along z, uz is a constant of the motion), plus
 4 B to read uz on the diagnostics steps, which sum the kinetic energy;
 ``cpu_baseline`` times the oracle's FP32 same-algorithm ES step with OpenMP
-on this host's cores on the same particle state.
+on this host's cores on the same particle state.  ``ensemble`` compares a
+24-run batched ensemble with the same runs stepped one by one; ``adjoint``
+times the field-energy gradient of a 128^2 two-stream run against its plain
+forward run; ``darwin`` times the three-run Darwin whistler-pump scan.
 """
 from __future__ import annotations
 
@@ -86,7 +91,7 @@ def ensemble_bench(steps: int, graph: bool = True):
     ntot = sum(r.grid.nx * r.grid.ny * sum(s.ppc for s in r.species) for r in runs)
     seq = 0.0
     for r in runs:
-        sim = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=20))
+        sim = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=0))
         sim.init()
         sim.synchronize()
         t0 = time.perf_counter()
@@ -94,7 +99,7 @@ def ensemble_bench(steps: int, graph: bool = True):
         seq += time.perf_counter() - t0
         sim.close()
     print(f"# ensemble: sequential {seq:.3f}s for {steps} steps", file=sys.stderr, flush=True)
-    ens = Ensemble(runs, SimConfig(sort_every=20), graph=graph)
+    ens = Ensemble(runs, SimConfig(sort_every=0), graph=graph)
     ens.init()
     ens.synchronize()
     t0 = time.perf_counter()
@@ -111,6 +116,218 @@ def ensemble_bench(steps: int, graph: bool = True):
                         "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
 
 
+def adjoint_bench(nsteps: int = 100):
+    """Field-energy gradient of the differentiable two-stream run (128^2, dx
+    0.5, 64 ppc as beams +-0.2 with vth 0.02, 16 x-displacement modes
+    A_m ~ 1e-3 U[-1,1] seed 20260630, dt 0.1, 100 steps, checkpoints every 10):
+    forward with checkpoints, backward sweep, and the plain forward run of
+    the same state by aw_sim, each timed with a device synchronisation."""
+    from arcwarden_amd import _lib as L
+    from arcwarden_amd.adjoint import two_stream_adjoint
+    from arcwarden_amd.sim import SimConfig, Simulation
+    run, A, ufl = two_stream_adjoint()
+    run.forward(A, ufl, nsteps)  # warm-up (plans, allocations)
+    run.backward()
+    t0 = time.perf_counter()
+    loss = run.forward(A, ufl, nsteps)
+    t1 = time.perf_counter()
+    gA, gU = run.backward()
+    t2 = time.perf_counter()
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260630)
+    sim = Simulation(run.grid, rp, run.species, SimConfig(has_b0=False, mode_amp=list(A)))
+    sim.init()
+    sim.synchronize()
+    t3 = time.perf_counter()
+    sim.run(nsteps)
+    t4 = time.perf_counter()
+    sim.close()
+    run.close()
+    plain = t4 - t3
+    return {"workload": "two-stream 128^2, 64 ppc (1,048,576), 16 modes, 100 steps, "
+                        "checkpoint every 10, loss EE(100)",
+            "loss": loss, "forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
+            "plain_forward_ms": 1e3 * plain,
+            "gradient_cost_vs_plain": (t2 - t0) / plain,
+            "backward_vs_plain": (t2 - t1) / plain,
+            "dL_dv0": float(gU[0, 0] - gU[1, 0]), "dL_dA1": float(gA[0])}
+
+
+def whistler_run(vr_over_vth: float, nx: int = 512, ny: int = 128, ppc: int = 256,
+                 dump_every: int = 100):
+    """One run of the Darwin whistler-pump scan: 512x128, dx 0.5, c 10,
+    w_ce/w_pe 0.5 with B0 at 30 deg to x in the x-z plane, v_r = 0.04c = 0.4,
+    v_th = v_r / (v_r/v_th), pump w0 = 0.3 w_ce = 0.15 at mode M = 18
+    (k0 = 2 pi 18 / 256) with dB_pump = 1e-3 B0 converted to the pump's E
+    amplitude by Faraday (|E_perp| = (w0/k0) dB), circular (ey, ez = +-E),
+    dt 0.05, noisy Maxwellian load."""
+    import math
+    from arcwarden_amd import _lib as L
+    dx = 0.5
+    wce, th = 0.5, math.radians(30.0)
+    B0 = (wce * math.cos(th), 0.0, wce * math.sin(th))
+    vr = 0.4
+    vth = vr / vr_over_vth
+    k0 = 2 * math.pi * 18 / (nx * dx)
+    w0 = 0.3 * wce
+    e_perp = (w0 / k0) * 1e-3 * wce
+    g = L.grid(nx, ny, nx * dx, ny * dx)
+    rp = L.run_params(dt=0.05, qm=-1.0, eps0=1.0, c=10.0, n0=1.0, noisy_load=1,
+                      rng_seed=20260631, B0=B0, wce=wce, dump_every=dump_every)
+    rp.pump = 1
+    rp.pump_ex, rp.pump_ey, rp.pump_ez = 0.0, e_perp, -e_perp
+    rp.pump_k0, rp.pump_w0 = k0, w0
+    rp.pump_trmp, rp.pump_toff = 2 * math.pi / w0, 1e30
+    sp = [L.species("electrons", 1.0, ppc, (vth,) * 3, (0.0, 0.0, 0.0))]
+    return g, rp, sp
+
+
+def darwin_bench(steps: int):
+    """The Darwin whistler-pump scan (v_r/v_th in {3.2, 2.1, 1.0}, 512x128,
+    256 ppc = 16,777,216 electrons per run) stepped run after run by aw_sim
+    (spectral Darwin model, re-sort on 2% strays, diagnostics every 100
+    steps).  Loading is outside the timed region.  The EM push kernel is
+    timed with HIP events on the sim stream and priced at 40 B per particle
+    (x, y, ux, uy, uz read and written)."""
+    from arcwarden_amd import _lib as L
+    from arcwarden_amd.sim import SimConfig, Simulation
+    tot_ps, tot_s, kms, kl, n_last, sorts = 0.0, 0.0, 0.0, 0, 0, 0
+    ratios = (3.2, 2.1, 1.0)
+    ee = []
+    for r in ratios:
+        g, rp, sp = whistler_run(r)
+        sim = Simulation(g, rp, sp, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN,
+                                              sort_every=0))
+        sim.init()
+        n = sim.particle_views().n
+        sim.synchronize()
+        sim.set_profiling(True)
+        t0 = time.perf_counter()
+        sim.run(steps)
+        el = time.perf_counter() - t0
+        ms, launches, ns = sim.kernel_time()
+        kms += ms
+        kl += launches
+        sorts += ns
+        hist = sim.history()
+        ee.append(hist[-1]["ee"] if hist else None)
+        sim.close()
+        tot_ps += n * steps
+        tot_s += el
+        n_last = n
+        print(f"# darwin v_r/v_th={r}: {el / steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    avg_k = kms / 1e3 / max(kl, 1)
+    achieved = 40.0 * n_last / avg_k / 1e9
+    return {"workload": "Darwin whistler pump, 3 runs v_r/v_th 3.2/2.1/1.0, 512x128, dx 0.5, "
+                        "256 ppc (16,777,216 each), c 10, w_ce 0.5 at 30 deg, pump w0 0.15 "
+                        "M=18 dB=1e-3 B0, dt 0.05, run after run",
+            "steps": steps, "value": tot_ps / tot_s, "unit": "particle-steps/s",
+            "ms_per_step": tot_s / (len(ratios) * steps) * 1e3, "sorts": sorts,
+            "em_push": {"avg_kernel_ms": avg_k * 1e3, "bytes_per_particle": 40.0,
+                        "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS},
+            "final_ee": ee}
+
+
+def policy_bench(nx: int, ppc: int, reps: int = 5):
+    """Deposit / push microbench on the headline state (the reference's
+    deposit_bench): nx^2 grid, ppc electrons per cell loaded as in the main
+    run, tile-sorted (8x8 tiles).  Times, with HIP events on one stream, the
+    CIC charge deposit under each policy (global atomics, whole grid in LDS
+    when the int64 grid fits the 160 KiB of LDS, tile-binned LDS), the push
+    alone (Pusher::boris, no deposit), the fused push+deposit, and the
+    Esirkepov current deposit of one step's moves under each policy.  Bytes
+    are algorithmic: deposit 8 B/particle (x, y), push 40 B (x, y, ux, uy, uz
+    read and written), fused 32 B, Esirkepov 20 B (x0, y0, x1, y1, uz read;
+    w uniform)."""
+    import ctypes as C
+    import torch
+    from arcwarden_amd import _lib as L
+    from arcwarden_amd.sim import particles_struct
+    dx = 0.5
+    g = L.grid(nx, nx, nx * dx, nx * dx)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260628,
+                      B0=(0.0, 0.0, 0.5))
+    sp = L.species("electrons", 1.0, ppc, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0))
+    n = nx * nx * ppc
+    ncell = nx * nx
+    s = torch.cuda.current_stream()
+    st = C.c_void_p(s.cuda_stream)
+    h = C.c_void_p()
+    L.call("aw_ctx_create", C.byref(g), 8, 8, C.byref(h))
+    out = {}
+
+    def soa():
+        return [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(5)]
+
+    def timed(fn, nbytes):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        best = float("inf")
+        for _ in range(reps):
+            e0.record(s)
+            fn()
+            e1.record(s)
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return {"ms": best, "particles_per_s": n / (best * 1e-3),
+                "GBs": nbytes * n / (best * 1e-3) / 1e9,
+                "frac": nbytes * n / (best * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+    try:
+        nt, ntx, a, b = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("aw_ctx_tiles", h, C.byref(nt), C.byref(ntx), C.byref(a), C.byref(b))
+        off = torch.empty(nt.value + 1, dtype=torch.int32, device="cuda")
+        A, B = soa(), soa()
+        wout = (C.c_double * 1)()
+        L.call("aw_load_species", h, C.byref(sp), 1, C.byref(rp), None, 0,
+               C.byref(particles_struct(*A)), wout, st)
+        pa = particles_struct(*A, w_uniform=wout[0])
+        pb = particles_struct(*B, w_uniform=wout[0])
+        L.call("aw_sort_by_tile", h, C.byref(pa), C.byref(pb), C.c_void_p(off.data_ptr()), st)
+        rho = torch.zeros(ncell, dtype=torch.float32, device="cuda")
+        E = torch.randn(2 * ncell, device="cuda") * 1e-3
+        Ex, Ey = C.c_void_p(E.data_ptr()), C.c_void_p(E.data_ptr() + 4 * ncell)
+        J = [torch.zeros(ncell, dtype=torch.float32, device="cuda") for _ in range(3)]
+        pols = {"global": L.AW_DEPOSIT_GLOBAL, "binned": L.AW_DEPOSIT_BINNED}
+        if ncell * 8 <= 160 * 1024:
+            pols["lds_grid"] = L.AW_DEPOSIT_LDS_GRID
+        else:
+            out["lds_grid"] = f"not applicable: the {ncell * 8 >> 10} KiB int64 grid exceeds LDS"
+        # sorted state in B
+        for name, pol in pols.items():
+            def dep(pol=pol):
+                rho.zero_()
+                L.call("aw_deposit_rho", h, C.byref(pb), C.c_double(-1.0), pol,
+                       C.c_void_p(rho.data_ptr()), st)
+            out[f"deposit_{name}"] = timed(dep, 8.0)
+
+        def fused():
+            rho.zero_()
+            L.call("aw_push_deposit_es", h, C.byref(pb), C.c_void_p(off.data_ptr()), Ex, Ey,
+                   C.byref(rp), 1, C.c_void_p(rho.data_ptr()), None, st)
+        out["push_deposit_fused"] = timed(fused, 32.0)
+        out["push"] = timed(lambda: L.call("aw_push_es", h, C.byref(pb), Ex, Ey, C.byref(rp),
+                                           1, st), 40.0)
+        L.call("aw_migrate", h, C.byref(pb), None, st)
+        # one more fused step from saved positions gives the Esirkepov moves
+        x0, y0 = B[0].clone(), B[1].clone()
+        L.call("aw_push_deposit_es", h, C.byref(pb), C.c_void_p(off.data_ptr()), Ex, Ey,
+               C.byref(rp), 1, C.c_void_p(rho.data_ptr()), None, st)
+        for name, pol in pols.items():
+            def esk(pol=pol):
+                for j in J:
+                    j.zero_()
+                L.call("aw_deposit_esirkepov", h, C.c_void_p(x0.data_ptr()),
+                       C.c_void_p(y0.data_ptr()), C.byref(pb), C.c_double(-1.0),
+                       C.c_double(0.1), pol, *(C.c_void_p(j.data_ptr()) for j in J), st)
+            out[f"esirkepov_{name}"] = timed(esk, 20.0)
+        torch.cuda.synchronize()
+    finally:
+        L.lib().aw_ctx_destroy(h)
+    out["workload"] = (f"{nx}x{nx}, {ppc} ppc ({n} particles), tile-sorted 8x8 tiles, "
+                       f"best of {reps} after one warm-up")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,18 +335,34 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--ppc", type=int, default=256)
     ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--sort-every", type=int, default=20)
+    ap.add_argument("--sort-every", type=int, default=0,
+                    help="re-sort cadence in steps; 0: re-sort only when more than 2%% of "
+                         "the particles left their tile apron")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="steps of the batched-ensemble comparison (0: skip)")
+    ap.add_argument("--no-fixed-cadence", action="store_true",
+                    help="skip the comparison run with a re-sort every 20 steps")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="1: time the 128^2 two-stream adjoint (0: skip)")
+    ap.add_argument("--darwin-steps", type=int, default=2000,
+                    help="steps per run of the Darwin whistler-pump scan (0: skip)")
+    ap.add_argument("--no-policies", action="store_true",
+                    help="skip the deposit/push policy microbench")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default tile / workgroups-per-tile heuristic")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="fixed square tile size (implies --no-autotune; 0: default)")
+    ap.add_argument("--bpt", type=int, default=0,
+                    help="fixed workgroups per tile (implies --no-autotune; 0: default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device copy (torch clone) so that a "
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a "
                          "kernel of known traffic")
     args = ap.parse_args()
+    if args.tile or args.bpt:
+        args.no_autotune = True
     if args.calibrate_copy:
         import torch
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
@@ -139,7 +372,8 @@ def main():
         del src, dst
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the ensemble and particle-decomposition sharding is described in DESIGN.md)",
+              "(ensembles would shard one run per GPU; a single run by particles, with an "
+              "exact all-reduce of the int64 charge grid each step)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -153,31 +387,40 @@ def main():
     rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260628,
                       B0=(0.0, 0.0, 0.5), dump_every=10)
     sp = [L.species("electrons", 1.0, args.ppc, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0))]
-    sim = Simulation(g, rp, sp, SimConfig(has_b0=True, sort_every=args.sort_every,
-                                          autotune=not args.no_autotune))
-    sim.init()
-    n = sim.particle_views().n
-    exec_cfg = sim.exec_config()
-    print(f"# init done: {n} particles on {nx}x{ny}; exec {exec_cfg}", file=sys.stderr,
-          flush=True)
+    def timed_run(sort_every, keep_state):
+        sim = Simulation(g, rp, sp, SimConfig(has_b0=True, sort_every=sort_every,
+                                              tile_x=args.tile, tile_y=args.tile,
+                                              blocks_per_tile=args.bpt,
+                                              autotune=not args.no_autotune))
+        sim.init()
+        n = sim.particle_views().n
+        exec_cfg = sim.exec_config()
+        print(f"# init done: {n} particles on {nx}x{ny}; exec {exec_cfg}", file=sys.stderr,
+              flush=True)
+        state = rho0 = None
+        if keep_state:
+            state = sim.particles()
+            rho0 = sim.rho()
+        for k in range(args.warmup):
+            sim.step(k)
+        sim.synchronize()
+        sim.set_profiling(True)
+        t0 = time.perf_counter()
+        for k in range(args.warmup, args.warmup + args.steps):
+            sim.step(k)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        kms, launches, sorts = sim.kernel_time()
+        hist = sim.history()
+        sim.close()
+        return dict(n=n, exec_cfg=exec_cfg, elapsed=t1 - t0, kms=kms, launches=launches,
+                    sorts=sorts, hist=hist, state=state, rho0=rho0)
 
+    main_run = timed_run(args.sort_every, not args.no_cpu)
+    n, exec_cfg, elapsed = main_run["n"], main_run["exec_cfg"], main_run["elapsed"]
+    kms, launches, sorts, hist = (main_run[k] for k in ("kms", "launches", "sorts", "hist"))
+    state, rho0 = main_run["state"], main_run["rho0"]
     cpu = None
-    if not args.no_cpu:
-        state = sim.particles()
-        rho0 = sim.rho()
-    for k in range(args.warmup):
-        sim.step(k)
-    sim.synchronize()
-    sim.set_profiling(True)
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        sim.step(k)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    kms, launches, sorts = sim.kernel_time()
-    hist = sim.history()
-    sim.close()
-    elapsed = t1 - t0
     value = n * args.steps / elapsed
     avg_kernel_s = kms / max(1, launches) / 1e3
     dumps = sum(1 for k in range(args.warmup, args.warmup + args.steps) if k % 10 == 0)
@@ -186,7 +429,17 @@ def main():
     print(f"# timed {args.steps} steps in {elapsed:.4f}s; push_deposit avg "
           f"{avg_kernel_s * 1e3:.4f} ms over {launches} launches; sorts={sorts}",
           file=sys.stderr, flush=True)
+    fixed = None
+    if args.sort_every != 20 and not args.no_fixed_cadence:
+        r20 = timed_run(20, False)
+        fixed = {"sort_every": 20, "value": n * args.steps / r20["elapsed"],
+                 "ms_per_step": r20["elapsed"] / args.steps * 1e3, "sorts": r20["sorts"]}
+        print(f"# sort every 20: {fixed['ms_per_step']:.4f} ms/step", file=sys.stderr,
+              flush=True)
     ens = ensemble_bench(args.ensemble_steps) if args.ensemble_steps > 0 else None
+    adj = adjoint_bench() if args.adjoint else None
+    dar = darwin_bench(args.darwin_steps) if args.darwin_steps > 0 else None
+    pol = None if args.no_policies else policy_bench(nx, args.ppc)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(state, nx, ny, dx, state["w_uniform"], rho0, args.cpu_seconds)
@@ -205,8 +458,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Maxwellian load, seed 20260628)",
         "config": {"workload": f"ES thermal plasma {nx}x{ny}, dx=0.5, CIC, {args.ppc} ppc "
-                               f"({n} electrons), vth=0.05, B0=0.5z, dt=0.1; sort every "
-                               f"{args.sort_every} steps, diagnostics every 10",
+                               f"({n} electrons), vth=0.05, B0=0.5z, dt=0.1; "
+                               + (f"re-sort every {args.sort_every} steps"
+                                  if args.sort_every > 0 else
+                                  "re-sort when >2% of particles left their tile apron")
+                               + ", diagnostics every 10",
                    "particles": n, "grid": [nx, ny], "ppc": args.ppc,
                    "tile": [exec_cfg["tile_x"], exec_cfg["tile_y"]],
                    "blocks_per_tile": exec_cfg["blocks_per_tile"],
@@ -218,7 +474,11 @@ def main():
                      "avg_kernel_ms": avg_kernel_s * 1e3},
         "cpu_baseline": cpu,
         "ensemble": ens,
-        "checks": {"charge_drift": charge_drift, "sorts": sorts},
+        "adjoint": adj,
+        "darwin": dar,
+        "policies": pol,
+        "checks": {"charge_drift": charge_drift, "sorts": sorts,
+                   "fixed_sort_cadence": fixed},
     }
     print(json.dumps(out), flush=True)
 

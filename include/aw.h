@@ -455,7 +455,8 @@ typedef struct aw_ensemble_stats {
 } aw_ensemble_stats;
 
 /* cfg: tile_x (0 -> 8 at >= 64 particles per cell, else 16), blocks_per_tile,
- * sort_every (0 -> 20); the other members are ignored. */
+ * sort_every (0: re-sort only when strays exceed stray_frac), stray_frac
+ * (0 -> 0.02); the other members are ignored. */
 aw_status aw_ensemble_create(const aw_run_desc* runs, int32_t nruns, const aw_sim_config* cfg,
                              aw_ensemble** out);
 aw_status aw_ensemble_destroy(aw_ensemble* e);
