@@ -17,6 +17,8 @@ LIB_PATH = PKG / "libarcwarden_hip.so"
 
 AW_OK = 0
 AW_DEPOSIT_GLOBAL, AW_DEPOSIT_LDS_GRID, AW_DEPOSIT_BINNED, AW_DEPOSIT_AUTO = 0, 1, 2, 3
+AW_CURRENT_CIC, AW_CURRENT_ESIRKEPOV = 0, 1
+AW_J_STAGGERED = 1
 AW_MODEL_ES, AW_MODEL_DARWIN = 0, 1
 
 
@@ -96,7 +98,7 @@ class AwSimConfig(C.Structure):
                 ("sort_every", C.c_int32), ("stray_frac", C.c_double), ("n_modes", C.c_int32),
                 ("mode_amp", C.c_double * 32), ("legacy_load", C.c_int32),
                 ("fixed_scale", C.c_int32), ("blocks_per_tile", C.c_int32),
-                ("autotune", C.c_int32)]
+                ("autotune", C.c_int32), ("current_deposit", C.c_int32)]
 
 
 class AwRunDesc(C.Structure):
@@ -165,6 +167,8 @@ _SIGS = {
     "aw_solve_es": (C.c_int32, [P, P, P, P, C.c_double, P]),
     "aw_solve_darwin_el_b": (C.c_int32, [P, P, P, C.c_double, C.c_double,
                                          PP(C.c_float), P]),
+    "aw_solve_darwin_el_b_ex": (C.c_int32, [P, P, P, C.c_double, C.c_double,
+                                            PP(C.c_float), C.c_int32, P]),
     "aw_solve_darwin_et": (C.c_int32, [P, P, P, C.c_double, C.c_double, C.c_double, P]),
     "aw_reduce_energies": (C.c_int32, [P, PP(AwParticles), P, P, P, C.c_double,
                                        PP(AwDiagSample), P]),

@@ -405,15 +405,20 @@ __device__ __forceinline__ void boris_full(float qmh, float ex, float ey, float 
     uz = fmaf(qmh, ez, uz);
 }
 
-// Exact rint(x) as int64 for |x| < 2^62 with FP32/int32 operations only:
-// x = hi*2^31 + rem with hi = trunc(x/2^31); rem = x - hi*2^31 is an integer
-// below 2^31 in magnitude and representable in FP32, so the fma is exact.
-__device__ __forceinline__ long long f2ll_rn(float x) {
-    const float xr = rintf(x);
-    const int hi = (int)truncf(xr * 4.656612873077393e-10f);
-    const float rem = fmaf(-(float)hi, 2147483648.0f, xr);
-    return ((long long)hi << 31) + (long long)(int)rem;
+// rint(v) (ties to even) as int64.  For |v| < 2^51 one FP64 add does it:
+// v + 1.5*2^52 lies in [2^52, 2^53), where the spacing is 1, so the sum is
+// rounded to the nearest integer and its bit pattern exceeds that of 1.5*2^52
+// by exactly rint(v).  Larger magnitudes (never produced by the scales chosen
+// here) take the general conversion.
+__device__ __forceinline__ long long fix64(double v) {
+    constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+    if (fabs(v) < 2251799813685248.0)              // 2^51
+        return __double_as_longlong(v + kMagic) - __double_as_longlong(kMagic);
+    return (long long)rint(v);
 }
+
+// Exact rint(x) of an FP32 value as int64 (FP32 -> FP64 is exact).
+__device__ __forceinline__ long long f2ll_rn(float x) { return fix64((double)x); }
 
 // FP32 fixed-point contribution: rint(v * w) as a 64-bit two's-complement
 // integer, v carrying the quantity's 2^S scale.  Every product of at least
@@ -443,7 +448,94 @@ __device__ __forceinline__ CicD cic_d(float x, float y) {
 }
 
 __device__ __forceinline__ unsigned long long fixq(double coef, double w, double scale) {
-    return (unsigned long long)(long long)rint(coef * w * scale);
+    return (unsigned long long)fix64(coef * w * scale);
+}
+
+// ---- Esirkepov move geometry (include/pic/yee2d.hpp:124-132,310-389) --------
+// For a move (X0, Y0) -> (X1, Y1) of less than one cell (cell units; X1, Y1
+// taken as the minimal image of the stored, possibly wrapped, new position)
+// the 4x4 union stencil {ib-1..ib+2} x {jb-1..jb+2}, ib = floor(min(X0, X1)),
+// carries the CIC node shapes S0 = S(X0) and the differences DS = S(X1) - S(X0).
+struct EskShape {
+    int ib, jb;
+    double Sx0[4], Sy0[4], DSx[4], DSy[4];
+};
+
+__device__ __forceinline__ void esk_shape4(double x, int base, double S[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S[k] = fmax(0.0, 1.0 - fabs(x - (double)(base - 1 + k)));
+}
+
+__device__ __forceinline__ void esk_shape(double X0, double Y0, double X1, double Y1, int nx,
+                                          int ny, EskShape& m) {
+    if (X1 - X0 > 0.5 * nx) X1 -= nx;
+    else if (X0 - X1 > 0.5 * nx) X1 += nx;
+    if (Y1 - Y0 > 0.5 * ny) Y1 -= ny;
+    else if (Y0 - Y1 > 0.5 * ny) Y1 += ny;
+    m.ib = (int)floor(fmin(X0, X1));
+    m.jb = (int)floor(fmin(Y0, Y1));
+    double Sx1[4], Sy1[4];
+    esk_shape4(X0, m.ib, m.Sx0);
+    esk_shape4(X1, m.ib, Sx1);
+    esk_shape4(Y0, m.jb, m.Sy0);
+    esk_shape4(Y1, m.jb, Sy1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        m.DSx[k] = Sx1[k] - m.Sx0[k];
+        m.DSy[k] = Sy1[k] - m.Sy0[k];
+    }
+}
+
+__device__ __forceinline__ unsigned long long esk_fix(double v) {
+    return (unsigned long long)fix64(v);
+}
+
+// Calls add(q, k, b, v) for every non-zero fixed-point contribution v of
+// component q (0 Jx, 1 Jy, 2 Jz) at stencil node (ib-1+k, jb-1+b):
+//   Jx = cJx * sum_{a<=k} DSx_a (Sy0_b + DSy_b/2)
+//   Jy = cJy * sum_{c<=b} DSy_c (Sx0_k + DSx_k/2)
+//   Jz = cJz * (Sx0 Sy0 + (DSx Sy0 + Sx0 DSy)/2 + DSx DSy/3)
+// (cJx = -q w/(dt dy), cJy = -q w/(dt dx), cJz = q w uz/(dx dy), each times
+// the fixed-point scale).  Both end points lie at or beyond node ib (jb), so
+// every shape and difference at k = 0 (b = 0) is exactly zero and those
+// terms are skipped; the full prefix sum at k = 3 (b = 3) is the particle's
+// net charge change, zero, and is skipped too.  What remains is the 3x3
+// block of nodes ib..ib+2 x jb..jb+2 with 6 + 6 + 9 contributions.
+template <class Add>
+__device__ __forceinline__ void esk_emit(const EskShape& m, double cJx, double cJy, double cJz,
+                                         Add&& add) {
+#pragma unroll
+    for (int b = 1; b < 4; ++b) {
+        const double sy = m.Sy0[b] + 0.5 * m.DSy[b];
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 1; k < 3; ++k) {
+            acc += m.DSx[k] * sy;
+            const unsigned long long v = esk_fix(cJx * acc);
+            if (v) add(0, k, b, v);
+        }
+    }
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        const double sx = m.Sx0[k] + 0.5 * m.DSx[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 1; b < 3; ++b) {
+            acc += m.DSy[b] * sx;
+            const unsigned long long v = esk_fix(cJy * acc);
+            if (v) add(1, k, b, v);
+        }
+    }
+#pragma unroll
+    for (int b = 1; b < 4; ++b)
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            const double wz = m.Sx0[k] * m.Sy0[b] +
+                              0.5 * (m.DSx[k] * m.Sy0[b] + m.Sx0[k] * m.DSy[b]) +
+                              m.DSx[k] * m.DSy[b] * (1.0 / 3.0);
+            const unsigned long long v = esk_fix(cJz * wz);
+            if (v) add(2, k, b, v);
+        }
 }
 
 }  // namespace aw::dev

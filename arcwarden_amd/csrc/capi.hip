@@ -46,7 +46,7 @@ int tile_bpt(const aw_ctx& c, int64_t n);
 bool uz_evolves(const aw_run_params& rp, bool has_b0);
 void enable_em_fft(aw_ctx& c);
 void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
-                      const float B0[3], hipStream_t s);
+                      const float B0[3], hipStream_t s, bool stag);
 void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
                      double n0, hipStream_t s);
 void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
@@ -57,7 +57,8 @@ void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
             unsigned long long* strays, hipStream_t s);
 int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc8, const float* cq, double qm,
-            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s);
+            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s,
+            bool esk);
 void deposit_rho_j_amu(aw_ctx& c, const aw_particles& p, double qm, float* const out[8],
                        hipStream_t s);
 void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const* F,
@@ -157,6 +158,8 @@ struct aw_sim {
     unsigned long long* stray_host = nullptr;
     hipEvent_t stray_ev = nullptr;
     bool stray_pending = false;
+    int stray_age = 0;          // steps since the pending stray read-back was issued
+    bool j_staggered = false;   // acc8's current is the Esirkepov (staggered) one
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
     double prof_ms = 0.0;
@@ -240,6 +243,7 @@ struct aw_sim {
         em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
                    nullptr, stream);
         moments_ready = true;
+        j_staggered = false;
     }
 
     // darwin_fields (include/pic/simulation.hpp:103-119) + boris_em + migrate
@@ -247,16 +251,18 @@ struct aw_sim {
         const hipStream_t s = stream;
         const int64_t nc = ncell(*ctx);
         const double t = (double)nstep * rp.dt;
-        if (!moments_ready)
+        if (!moments_ready) {
             em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
                        nullptr, s);
+            j_staggered = false;
+        }
         moments_ready = false;
         // rho, J -> src4; amu -> src7[3..6]; fftin holds the copies rocFFT may clobber
         float* fin = fftin.get();
         convert_multi(acc8.get(), src.get(), fin, nc, 4, inv.get(), s);
         convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin + 7 * nc, nc, 4, inv.get() + 4,
                       s);
-        solve_darwin_elb(*ctx, fin, elb.get(), rp.eps0, rp.c, rp.B0, s);
+        solve_darwin_elb(*ctx, fin, elb.get(), rp.eps0, rp.c, rp.B0, s, j_staggered);
         form_e(FormE{{ed3.get(), ed3.get() + nc, ed3.get() + 2 * nc},
                      {elb.get(), elb.get() + nc},
                      {nullptr, nullptr, nullptr}},
@@ -283,9 +289,11 @@ struct aw_sim {
         }
         const float* Fp[6];
         fields6(e3.get(), Fp);
+        const bool esk = cfg.current_deposit == AW_CURRENT_ESIRKEPOV;
         const int nblk = em_push(*ctx, particles(), tile_off.get(), nullptr, Fp, acc8.get(),
                                  cqd.get(), rp.qm, rp.dt, bpt, strays.get(),
-                                 dump ? ke_part.get() : nullptr, s);
+                                 dump ? ke_part.get() : nullptr, s, esk);
+        j_staggered = esk;
         if (prof) {
             AW_HIP(hipEventRecord(e1, s));
             prof_ev.emplace_back(e0, e1);
@@ -296,6 +304,7 @@ struct aw_sim {
                                   hipMemcpyDeviceToHost, s));
             AW_HIP(hipEventRecord(stray_ev, s));
             stray_pending = true;
+            stray_age = 0;
         }
         AW_HIP(hipMemsetAsync(strays.get(), 0, sizeof(unsigned long long), s));
         if (dump) record_diag(nstep, nblk);
@@ -365,9 +374,17 @@ struct aw_sim {
         diag_pending.clear();
     }
 
+    // The stray count of a step is read back asynchronously.  Once the read-back
+    // is kStrayLag steps old the host waits for it, so the re-sort decision
+    // trails the device by at most that many steps however far ahead the host
+    // enqueues (the queue then stays about that deep, enough to keep the device
+    // busy).
+    static constexpr int kStrayLag = 2;
+
     void poll_strays() {
         if (!stray_pending) return;
-        if (hipEventQuery(stray_ev) != hipSuccess) return;
+        if (++stray_age >= kStrayLag) AW_HIP(hipEventSynchronize(stray_ev));
+        else if (hipEventQuery(stray_ev) != hipSuccess) return;
         stray_pending = false;
         const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
         if ((double)*stray_host > frac * (double)n) sort_requested = true;
@@ -520,6 +537,7 @@ struct aw_sim {
                                   hipMemcpyDeviceToHost, s));
             AW_HIP(hipEventRecord(stray_ev, s));
             stray_pending = true;
+            stray_age = 0;
         }
         if (dump) record_diag(nstep, nblk);
         ++steps_since_sort;
@@ -851,7 +869,20 @@ aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, doub
         AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
         const float zero[3] = {0.f, 0.f, 0.f};
         enable_em_fft(*ctx);
-        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s));
+        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s), false);
+    });
+}
+
+aw_status aw_solve_darwin_el_b_ex(aw_ctx* ctx, const float* src4, float* out5, double eps0,
+                                  double c, const float B0[3], int32_t flags, aw_stream s) {
+    return guard("aw_solve_darwin_el_b_ex", [&] {
+        AW_REQUIRE(ctx && src4 && out5, "NULL argument");
+        AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
+        AW_REQUIRE((flags & ~AW_J_STAGGERED) == 0, "unknown flags");
+        const float zero[3] = {0.f, 0.f, 0.f};
+        enable_em_fft(*ctx);
+        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s),
+                         (flags & AW_J_STAGGERED) != 0);
     });
 }
 
@@ -964,6 +995,11 @@ aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
             AW_REQUIRE(rp->c > 0 && rp->n0 >= 0, "Darwin needs rp.c > 0 and rp.n0 >= 0");
         AW_REQUIRE(rp->dt > 0, "rp.dt must be positive");
         AW_REQUIRE(rp->eps0 > 0, "rp.eps0 must be positive");
+        AW_REQUIRE(cfg->current_deposit == AW_CURRENT_CIC ||
+                       (cfg->current_deposit == AW_CURRENT_ESIRKEPOV &&
+                        cfg->field_model == AW_MODEL_DARWIN),
+                   "current_deposit must be AW_CURRENT_CIC, or AW_CURRENT_ESIRKEPOV with "
+                   "the Darwin model");
         auto sim = std::make_unique<aw_sim>();
         sim->g = *g;
         if (sim->g.Lx <= 0) sim->g.Lx = g->nx * g->dx;

@@ -45,6 +45,7 @@ struct EmArgs {
                                  // (* w_uniform when uniform)
     const float* c_dcu;          // device: 2^S_dcu * qm^2/area (* w_uniform)
     float qmh, cx, cy;
+    double dxdt, dydt;           // dx/dt, dy/dt (Esirkepov current coefficients)
     unsigned long long* strays;
     double* ke_part;
 };
@@ -63,7 +64,9 @@ __device__ __forceinline__ void moments8(const float cq[3], float mw, float ux, 
     v[7] = ba * (uz * uy);
 }
 
-template <int NQ>
+// Deposits v[q] for the q < NQ selected by MASK (grid q of the tile / global
+// accumulators).
+template <int NQ, unsigned MASK = (1u << NQ) - 1u>
 __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int W, int H,
                                           unsigned long long* g_acc, int ncell, int nx, int ny,
                                           int orgx, int orgy, const dev::CicF& d,
@@ -73,6 +76,7 @@ __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int 
         const int b = mj * W + mi;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+            if (!((MASK >> q) & 1u)) continue;
             unsigned long long* s = s_acc + q * R;
             atomicAdd(&s[b], dev::fixq32(v[q], d.w0));
             atomicAdd(&s[b + 1], dev::fixq32(v[q], d.w1));
@@ -85,6 +89,7 @@ __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int 
         dev::wrap_pair(d.j0, ny, j0, j1);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
+            if (!((MASK >> q) & 1u)) continue;
             unsigned long long* g = g_acc + (size_t)q * ncell;
             atomicAdd(&g[j0 * nx + i0], dev::fixq32(v[q], d.w0));
             atomicAdd(&g[j0 * nx + i1], dev::fixq32(v[q], d.w1));
@@ -170,6 +175,34 @@ __device__ __forceinline__ bool gather6(const EmArgs& a, const Frame& f, const f
     return false;
 }
 
+// Esirkepov current of the move (x0, y0) -> (x1, y1) into the J grids
+// (q = 1..3) of the moment tile, or into the global accumulators when the 4x4
+// union stencil leaves the tile frame.  cj = the J coefficient (fixed-point
+// scaled, times the particle weight): Jx = -cj dx/dt sum DSx(..),
+// Jy = -cj dy/dt sum DSy(..), Jz = cj uz W_z.  Every contribution is bounded
+// by cj |u|, the bound of the CIC current, so the J scale carries over.
+__device__ __forceinline__ void esk_deposit(const EmArgs& a, const Frame& f,
+                                            unsigned long long* s_acc, float x0, float y0,
+                                            float x1, float y1, float uz, double cj) {
+    dev::EskShape m;
+    dev::esk_shape(x0, y0, x1, y1, a.nx, a.ny, m);
+    const double cJx = -cj * a.dxdt, cJy = -cj * a.dydt, cJz = cj * (double)uz;
+    const int li = local_index(m.ib - 1, f.orgx, a.nx), lj = local_index(m.jb - 1, f.orgy, a.ny);
+    if (dev::in_tile(li, f.W - 4) && dev::in_tile(lj, f.H - 4)) {
+        unsigned long long* s = s_acc + f.R + lj * f.W + li;
+        dev::esk_emit(m, cJx, cJy, cJz, [&](int q, int k, int b, unsigned long long v) {
+            atomicAdd(&s[q * f.R + b * f.W + k], v);
+        });
+    } else {
+        unsigned long long* g = a.acc + (size_t)a.ncell;
+        dev::esk_emit(m, cJx, cJy, cJz, [&](int q, int k, int b, unsigned long long v) {
+            atomicAdd(&g[(size_t)q * a.ncell + (size_t)dev::wrapfar(m.jb - 1 + b, a.ny) * a.nx +
+                         dev::wrapfar(m.ib - 1 + k, a.nx)],
+                      v);
+        });
+    }
+}
+
 template <bool PER_W>
 __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
     extern __shared__ unsigned long long smem[];
@@ -203,28 +236,78 @@ __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
     __syncthreads();
     const float cd = *a.c_dcu;
     unsigned int nstray = 0;
-    for (int r = f.r0 + (int)threadIdx.x; r < f.r1; r += kThreads) {
-        const int k = a.idx ? a.idx[r] : r;
-        const dev::CicF s = dev::cic_f(a.x[k], a.y[k]);
+    // two particles per thread per iteration, both loaded first
+    auto one = [&](float x, float y, float vx, float vy, float vz, float mw) {
+        const dev::CicF s = dev::cic_f(x, y);
         float e[3], b[3];
         if (!gather6(a, f, fa, fb, s, e, b)) ++nstray;
-        const float vx = a.ux[k], vy = a.uy[k], vz = a.uz[k];
-        const float base = PER_W ? cd * a.w[k] : cd;
+        const float base = PER_W ? cd * mw : cd;
         float v[3];
         v[0] = base * (e[0] + (vy * b[2] - vz * b[1]));
         v[1] = base * (e[1] + (vz * b[0] - vx * b[2]));
         v[2] = base * (e[2] + (vx * b[1] - vy * b[0]));
         deposit_q<3>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy, s, v,
                      nstray);
+    };
+    int r = f.r0 + (int)threadIdx.x;
+    for (; r + kThreads < f.r1; r += 2 * kThreads) {
+        const int k0 = a.idx ? a.idx[r] : r;
+        const int k1 = a.idx ? a.idx[r + kThreads] : r + kThreads;
+        const float x0 = a.x[k0], y0 = a.y[k0], ux0 = a.ux[k0], uy0 = a.uy[k0], uz0 = a.uz[k0];
+        const float x1 = a.x[k1], y1 = a.y[k1], ux1 = a.ux[k1], uy1 = a.uy[k1], uz1 = a.uz[k1];
+        const float w0 = PER_W ? a.w[k0] : 1.0f, w1 = PER_W ? a.w[k1] : 1.0f;
+        one(x0, y0, ux0, uy0, uz0, w0);
+        one(x1, y1, ux1, uy1, uz1, w1);
+    }
+    if (r < f.r1) {
+        const int k = a.idx ? a.idx[r] : r;
+        one(a.x[k], a.y[k], a.ux[k], a.uy[k], a.uz[k], PER_W ? a.w[k] : 1.0f);
     }
     __syncthreads();
     flush_q<3>(s_acc, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
     if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
 }
 
+// One particle of em_push_kernel, its SoA values already loaded.
+template <bool PER_W, bool KE, bool DEP, bool ESK>
+__device__ __forceinline__ void em_one(const EmArgs& a, const Frame& f, const float4* fa,
+                                       const float2* fb, unsigned long long* s_acc,
+                                       const float cq[3], int k, float x, float y, float ux,
+                                       float uy, float uz, float mw, unsigned int& nstray,
+                                       double& ke) {
+    const dev::CicF s = dev::cic_f(x, y);
+    float e[3], b[3];
+    if (!gather6(a, f, fa, fb, s, e, b)) ++nstray;
+    dev::boris_full(a.qmh, e[0], e[1], e[2], b[0], b[1], b[2], ux, uy, uz);
+    const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), (float)a.nx);
+    const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), (float)a.ny);
+    a.x[k] = xn;
+    a.y[k] = yn;
+    a.ux[k] = ux;
+    a.uy[k] = uy;
+    a.uz[k] = uz;
+    if constexpr (KE)
+        ke += (PER_W ? (double)mw : 1.0) * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
+    if constexpr (DEP) {
+        float v[8];
+        moments8(cq, mw, ux, uy, uz, v);
+        if constexpr (ESK) {
+            deposit_q<8, 0xF1u>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx,
+                                f.orgy, dev::cic_f(xn, yn), v, nstray);
+            esk_deposit(a, f, s_acc, x, y, xn, yn, uz, (double)cq[1] * (double)mw);
+        } else {
+            deposit_q<8>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
+                         dev::cic_f(xn, yn), v, nstray);
+        }
+    }
+}
+
 // DEP = false: push, move and wrap only (Pusher::boris_em + migrate); the
-// LDS moment tile is then not allocated.
-template <bool PER_W, bool KE, bool DEP>
+// LDS moment tile is then not allocated.  ESK: the current of the next state
+// is the Esirkepov current of this step's move (staggered Jx, Jy), not the
+// CIC current at x^{n+1}; rho and amu stay CIC at x^{n+1}.  Two particles
+// per thread per iteration, both loaded before either is pushed.
+template <bool PER_W, bool KE, bool DEP, bool ESK = false>
 __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
     extern __shared__ unsigned long long smem[];
     const Frame f = frame(a);
@@ -243,31 +326,22 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
         cq[1] = a.cq[1];
         cq[2] = a.cq[2];
     }
-    for (int r = f.r0 + (int)threadIdx.x; r < f.r1; r += kThreads) {
+    int r = f.r0 + (int)threadIdx.x;
+    for (; r + kThreads < f.r1; r += 2 * kThreads) {
+        const int k0 = a.idx ? a.idx[r] : r;
+        const int k1 = a.idx ? a.idx[r + kThreads] : r + kThreads;
+        const float x0 = a.x[k0], y0 = a.y[k0], ux0 = a.ux[k0], uy0 = a.uy[k0], uz0 = a.uz[k0];
+        const float x1 = a.x[k1], y1 = a.y[k1], ux1 = a.ux[k1], uy1 = a.uy[k1], uz1 = a.uz[k1];
+        const float w0 = PER_W ? a.w[k0] : 1.0f, w1 = PER_W ? a.w[k1] : 1.0f;
+        em_one<PER_W, KE, DEP, ESK>(a, f, fa, fb, s_acc, cq, k0, x0, y0, ux0, uy0, uz0, w0,
+                                    nstray, ke);
+        em_one<PER_W, KE, DEP, ESK>(a, f, fa, fb, s_acc, cq, k1, x1, y1, ux1, uy1, uz1, w1,
+                                    nstray, ke);
+    }
+    if (r < f.r1) {
         const int k = a.idx ? a.idx[r] : r;
-        const float x = a.x[k], y = a.y[k];
-        float ux = a.ux[k], uy = a.uy[k], uz = a.uz[k];
-        const float mw = PER_W ? a.w[k] : 1.0f;
-        const dev::CicF s = dev::cic_f(x, y);
-        float e[3], b[3];
-        if (!gather6(a, f, fa, fb, s, e, b)) ++nstray;
-        dev::boris_full(a.qmh, e[0], e[1], e[2], b[0], b[1], b[2], ux, uy, uz);
-        const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), (float)a.nx);
-        const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), (float)a.ny);
-        a.x[k] = xn;
-        a.y[k] = yn;
-        a.ux[k] = ux;
-        a.uy[k] = uy;
-        a.uz[k] = uz;
-        if constexpr (KE)
-            ke += (PER_W ? (double)mw : 1.0) *
-                  ((double)ux * ux + (double)uy * uy + (double)uz * uz);
-        if constexpr (DEP) {
-            float v[8];
-            moments8(cq, mw, ux, uy, uz, v);
-            deposit_q<8>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
-                         dev::cic_f(xn, yn), v, nstray);
-        }
+        em_one<PER_W, KE, DEP, ESK>(a, f, fa, fb, s_acc, cq, k, a.x[k], a.y[k], a.ux[k],
+                                    a.uy[k], a.uz[k], PER_W ? a.w[k] : 1.0f, nstray, ke);
     }
     __shared__ double s_ke[kThreads / 64];
     if constexpr (KE) {
@@ -509,6 +583,8 @@ static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, cons
     a.qmh = (float)(qm * 0.5 * dt);
     a.cx = (float)(dt / c.g.dx);
     a.cy = (float)(dt / c.g.dy);
+    a.dxdt = dt != 0.0 ? c.g.dx / dt : 0.0;
+    a.dydt = dt != 0.0 ? c.g.dy / dt : 0.0;
     a.strays = strays;
     a.ke_part = ke_part;
     return a;
@@ -539,10 +615,12 @@ void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
     AW_LAUNCH_CHECK();
 }
 
-// acc8 == nullptr: push, move and wrap only
+// acc8 == nullptr: push, move and wrap only.  esk: the J moments of acc8 are
+// the Esirkepov current of the move (staggered Jx, Jy).
 int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc8, const float* cq, double qm,
-            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s) {
+            double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s,
+            bool esk) {
     if (p.n == 0) return 0;
     EmArgs a = em_args(c, p, off, idx, F, acc8, cq, nullptr, qm, dt, bpt, strays, ke_part);
     const bool dep = acc8 != nullptr;
@@ -553,7 +631,12 @@ int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
         kernel<<<nblk, kThreads, sh, s>>>(a);
         AW_LAUNCH_CHECK();
     };
-    if (dep) {
+    if (dep && esk) {
+        if (pw && ke) launch(em_push_kernel<true, true, true, true>);
+        else if (pw) launch(em_push_kernel<true, false, true, true>);
+        else if (ke) launch(em_push_kernel<false, true, true, true>);
+        else launch(em_push_kernel<false, false, true, true>);
+    } else if (dep) {
         if (pw && ke) launch(em_push_kernel<true, true, true>);
         else if (pw) launch(em_push_kernel<true, false, true>);
         else if (ke) launch(em_push_kernel<false, true, true>);
@@ -683,7 +766,7 @@ void push_boris_em(aw_ctx& c, const aw_particles& p, double qm, double dt, const
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
     bin_particles(c, p, s);
     em_push(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), F, nullptr, nullptr, qm, dt,
-            tile_bpt(c, p.n), nullptr, nullptr, s);
+            tile_bpt(c, p.n), nullptr, nullptr, s, false);
 }
 
 }  // namespace aw

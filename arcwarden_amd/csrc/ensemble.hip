@@ -127,10 +127,11 @@ struct aw_ensemble {
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // [dump]
     int64_t graph_launches = 0;
-    // stray feedback (read back one step late, without blocking)
+    // stray feedback (read back asynchronously, at most kStrayLag steps late)
     unsigned long long* stray_host = nullptr;
     hipEvent_t stray_ev = nullptr;
     bool stray_pending = false, sort_requested = false;
+    int stray_age = 0;  // steps since the pending stray read-back was issued
 
     ~aw_ensemble() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -141,8 +142,14 @@ struct aw_ensemble {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    // Asynchronous stray read-back, waited for once it is kStrayLag steps old so
+    // that the re-sort decision never trails the device by more than that.
+    static constexpr int kStrayLag = 2;
+
     void poll_strays() {
-        if (!stray_pending || hipEventQuery(stray_ev) != hipSuccess) return;
+        if (!stray_pending) return;
+        if (++stray_age >= kStrayLag) AW_HIP(hipEventSynchronize(stray_ev));
+        else if (hipEventQuery(stray_ev) != hipSuccess) return;
         stray_pending = false;
         const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
         if ((double)*stray_host > frac * (double)ntot) sort_requested = true;
@@ -291,6 +298,7 @@ struct aw_ensemble {
                                   hipMemcpyDeviceToHost, s));
             AW_HIP(hipEventRecord(stray_ev, s));
             stray_pending = true;
+            stray_age = 0;
         }
         if (dump) record_diag(nstep);
         ++steps_since_sort;

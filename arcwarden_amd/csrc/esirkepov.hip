@@ -50,85 +50,25 @@ __device__ __forceinline__ void esk_exponents(const EskArgs& a, int S[3]) {
     }
 }
 
-__device__ __forceinline__ unsigned long long fixd(double v) {
-    return (unsigned long long)__double2ll_rn(v);
-}
-
-__device__ __forceinline__ void shape4(double x, int base, double S[4]) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) S[k] = fmax(0.0, 1.0 - fabs(x - (double)(base - 1 + k)));
-}
-
-// One particle's move: stencil origin (ib, jb) and the FP64 shape factors.
+// One particle's move: the shared stencil geometry plus its fixed-point
+// scaled coefficients.
 struct EskMove {
-    int ib, jb;
-    double Sx0[4], Sy0[4], DSx[4], DSy[4];
-    double cJx, cJy, cJz;  // fixed-point scaled coefficients
+    dev::EskShape sh;
+    double cJx, cJy, cJz;
 };
 
 __device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double scale[3],
                                             EskMove& m) {
-    const double X0 = a.x0[i], Y0 = a.y0[i];
-    double X1 = a.x1[i], Y1 = a.y1[i];
-    // minimal image: p's positions may already be wrapped
-    if (X1 - X0 > 0.5 * a.nx) X1 -= a.nx;
-    else if (X0 - X1 > 0.5 * a.nx) X1 += a.nx;
-    if (Y1 - Y0 > 0.5 * a.ny) Y1 -= a.ny;
-    else if (Y0 - Y1 > 0.5 * a.ny) Y1 += a.ny;
-    m.ib = (int)floor(fmin(X0, X1));
-    m.jb = (int)floor(fmin(Y0, Y1));
-    double Sx1[4], Sy1[4];
-    shape4(X0, m.ib, m.Sx0);
-    shape4(X1, m.ib, Sx1);
-    shape4(Y0, m.jb, m.Sy0);
-    shape4(Y1, m.jb, Sy1);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        m.DSx[k] = Sx1[k] - m.Sx0[k];
-        m.DSy[k] = Sy1[k] - m.Sy0[k];
-    }
+    dev::esk_shape(a.x0[i], a.y0[i], a.x1[i], a.y1[i], a.nx, a.ny, m.sh);
     const double qw = a.qm * (a.w ? (double)a.w[i] : a.w_uniform);
     m.cJx = -qw / (a.dt * a.dy) * scale[0];
     m.cJy = -qw / (a.dt * a.dx) * scale[1];
     m.cJz = qw * (double)a.uz[i] / (a.dx * a.dy) * scale[2];
 }
 
-// Calls add(q, k, b, v) for every non-zero fixed-point contribution v of
-// component q (0 Jx, 1 Jy, 2 Jz) at stencil node (ib-1+k, jb-1+b).
 template <class Add>
 __device__ __forceinline__ void esk_emit(const EskMove& m, Add&& add) {
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const double sy = m.Sy0[b] + 0.5 * m.DSy[b];
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            acc += m.DSx[k] * sy;
-            const unsigned long long v = fixd(m.cJx * acc);
-            if (v) add(0, k, b, v);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double sx = m.Sx0[k] + 0.5 * m.DSx[k];
-        double acc = 0.0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            acc += m.DSy[b] * sx;
-            const unsigned long long v = fixd(m.cJy * acc);
-            if (v) add(1, k, b, v);
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double wz = m.Sx0[k] * m.Sy0[b] +
-                              0.5 * (m.DSx[k] * m.Sy0[b] + m.Sx0[k] * m.DSy[b]) +
-                              m.DSx[k] * m.DSy[b] * (1.0 / 3.0);
-            const unsigned long long v = fixd(m.cJz * wz);
-            if (v) add(2, k, b, v);
-        }
+    dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz, add);
 }
 
 __device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
@@ -180,7 +120,7 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
     EskMove m;
     esk_prepare(a, i, scale, m);
     esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
-        global_add(acc, a.nx, a.ny, nc, q, m.ib - 1 + k, m.jb - 1 + b, v);
+        global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
     });
 }
 
@@ -197,8 +137,8 @@ __global__ void __launch_bounds__(kThreads) esk_ldsgrid_kernel(EskArgs a,
         EskMove m;
         esk_prepare(a, i, scale, m);
         esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
-            const int gi = dev::wrapfar(m.ib - 1 + k, a.nx);
-            const int gj = dev::wrapfar(m.jb - 1 + b, a.ny);
+            const int gi = dev::wrapfar(m.sh.ib - 1 + k, a.nx);
+            const int gj = dev::wrapfar(m.sh.jb - 1 + b, a.ny);
             atomicAdd(&s_j[q * nc + gj * a.nx + gi], v);
         });
     }
@@ -240,7 +180,7 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
         const int i = t.idx[r];
         EskMove m;
         esk_prepare(a, i, scale, m);
-        const int li = wrap_local(m.ib - 1 - orgx, a.nx), lj = wrap_local(m.jb - 1 - orgy, a.ny);
+        const int li = wrap_local(m.sh.ib - 1 - orgx, a.nx), lj = wrap_local(m.sh.jb - 1 - orgy, a.ny);
         if (dev::in_tile(li, t.W - 4) && dev::in_tile(lj, t.H - 4)) {
             const int base = lj * t.W + li;
             esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
@@ -248,7 +188,7 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
             });
         } else {
             esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
-                global_add(acc, a.nx, a.ny, nc, q, m.ib - 1 + k, m.jb - 1 + b, v);
+                global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
             });
         }
     }

@@ -245,9 +245,11 @@ namespace {
 // (darwin_b_field_kernel, solver_darwin.hpp:38-75) from [rho|Jx|Jy|Jz]_k in
 // one pass.  The uniform background B0 (add_b0_kernel, pump.hpp:50-54) enters
 // as the DC mode of B: the C2R scale 1/(nx*ny) turns B0*nx*ny back into B0.
+// stag: Jx is sampled at (i+1/2, j) and Jy at (i, j+1/2) (Esirkepov current);
+// the half-cell shift moves them onto the nodes: J_k *= exp(-i k dx/2).
 __global__ void darwin_elb_kernel(const float2* __restrict__ src, float2* __restrict__ out,
                                   KGeom kg, double eps0, double c, float b0x, float b0y,
-                                  float b0z) {
+                                  float b0z, int stag) {
     const int nk = kg.nkx * kg.ny;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nk) return;
@@ -256,7 +258,21 @@ __global__ void darwin_elb_kernel(const float2* __restrict__ src, float2* __rest
     kvec(kg, i, j, kx, ky);
     const double k2 = kx * kx + ky * ky;
     const bool nyq = nyquist(kg, i, j);
-    const float2 r = src[t], jx = src[nk + t], jy = src[2 * nk + t], jz = src[3 * nk + t];
+    const float2 r = src[t], jz = src[3 * nk + t];
+    double jxr = src[nk + t].x, jxi = src[nk + t].y, jyr = src[2 * nk + t].x,
+           jyi = src[2 * nk + t].y;
+    if (stag) {
+        const double kPi = 3.14159265358979323846;
+        const double phx = kPi * (double)i / kg.nx;
+        const double phy = kPi * (double)(j <= kg.ny / 2 ? j : j - kg.ny) / kg.ny;
+        const double cx = cos(phx), sx = sin(phx), cy = cos(phy), sy = sin(phy);
+        const double xr = jxr * cx + jxi * sx, xi = jxi * cx - jxr * sx;
+        const double yr = jyr * cy + jyi * sy, yi = jyi * cy - jyr * sy;
+        jxr = xr;
+        jxi = xi;
+        jyr = yr;
+        jyi = yi;
+    }
     // longitudinal E
     const double g = k2 > 0.0 ? 1.0 / (eps0 * k2) : 0.0;
     const double pr = (double)r.x * g, pi = (double)r.y * g;
@@ -271,7 +287,7 @@ __global__ void darwin_elb_kernel(const float2* __restrict__ src, float2* __rest
         const double G = 1.0 / (eps0 * c * c * k2);
         bx = make_float2((float)(-G * ky * jz.y), (float)(G * ky * jz.x));
         by = make_float2((float)(G * kx * jz.y), (float)(-G * kx * jz.x));
-        const double cr = kx * jy.x - ky * jx.x, ci = kx * jy.y - ky * jx.y;
+        const double cr = kx * jyr - ky * jxr, ci = kx * jyi - ky * jxi;
         bz = make_float2((float)(-G * ci), (float)(G * cr));
     } else if (t == 0) {
         const float nn = (float)kg.nx * (float)kg.ny;
@@ -326,9 +342,10 @@ __global__ void darwin_et_kernel(const float2* __restrict__ src, float2* __restr
 
 }  // namespace
 
-// src4 = [rho|Jx|Jy|Jz] (float, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz].
+// src4 = [rho|Jx|Jy|Jz] (float, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz];
+// stag: Jx, Jy staggered half a cell (Esirkepov current).
 void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
-                      const float B0[3], hipStream_t s) {
+                      const float B0[3], hipStream_t s, bool stag) {
     const int64_t nk = nkmodes(c);
     c.ktmp.reserve((size_t)2 * 12 * nk);
     float2* k4 = reinterpret_cast<float2*>(c.ktmp.get());
@@ -337,7 +354,7 @@ void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, do
     KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
              6.28318530717958647692 / c.g.Ly};
     darwin_elb_kernel<<<blocks_for(nk), kThreads, 0, s>>>(k4, k5, kg, eps0, cl, B0[0], B0[1],
-                                                         B0[2]);
+                                                         B0[2], stag ? 1 : 0);
     AW_LAUNCH_CHECK();
     fft_exec(c, c.c2r5, k5, out5, s);
 }

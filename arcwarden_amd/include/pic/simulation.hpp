@@ -35,6 +35,8 @@ struct ExecConfig {
     int sort_every = 20;         // stable tile re-sort cadence (0: adaptive only)
     double stray_frac = 0.0;     // adaptive re-sort threshold (0: 2%)
     bool autotune = false;       // time tile / workgroup candidates at init
+    int current_deposit = AW_CURRENT_CIC;  // Darwin: AW_CURRENT_ESIRKEPOV for the
+                                           // charge-conserving current
 };
 
 template <class Cfg = arc::Cfg>
@@ -74,6 +76,7 @@ public:
         c.tile_y = exec_.tile_y;
         c.blocks_per_tile = exec_.blocks_per_tile;
         c.sort_every = exec_.sort_every;
+        c.current_deposit = exec_.current_deposit;
         c.stray_frac = exec_.stray_frac;
         c.autotune = exec_.autotune ? 1 : 0;
         c.legacy_load = species_.empty() ? 1 : 0;

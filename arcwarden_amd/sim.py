@@ -72,6 +72,7 @@ class SimConfig:
     fixed_scale: int = 0
     blocks_per_tile: int = 0
     autotune: bool = False
+    current_deposit: int = L.AW_CURRENT_CIC
 
     def to_c(self) -> L.AwSimConfig:
         c = L.AwSimConfig()
@@ -88,6 +89,7 @@ class SimConfig:
         c.fixed_scale = self.fixed_scale
         c.blocks_per_tile = self.blocks_per_tile
         c.autotune = int(self.autotune)
+        c.current_deposit = int(self.current_deposit)
         return c
 
 

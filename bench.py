@@ -19,7 +19,9 @@ algorithmic particle traffic against 8 TB/s: x, y, ux, uy read and written
 on this host's cores on the same particle state.  ``ensemble`` compares a
 24-run batched ensemble with the same runs stepped one by one; ``adjoint``
 times the field-energy gradient of a 128^2 two-stream run against its plain
-forward run; ``darwin`` times the three-run Darwin whistler-pump scan.
+forward run; ``darwin`` times the three-run Darwin whistler-pump scan with
+the Esirkepov current deposit (``darwin_cic_current``: with the reference's
+CIC current).
 """
 from __future__ import annotations
 
@@ -181,13 +183,13 @@ def whistler_run(vr_over_vth: float, nx: int = 512, ny: int = 128, ppc: int = 25
     return g, rp, sp
 
 
-def darwin_bench(steps: int):
+def darwin_bench(steps: int, current: int):
     """The Darwin whistler-pump scan (v_r/v_th in {3.2, 2.1, 1.0}, 512x128,
     256 ppc = 16,777,216 electrons per run) stepped run after run by aw_sim
-    (spectral Darwin model, re-sort on 2% strays, diagnostics every 100
-    steps).  Loading is outside the timed region.  The EM push kernel is
-    timed with HIP events on the sim stream and priced at 40 B per particle
-    (x, y, ux, uy, uz read and written)."""
+    (spectral Darwin model with the given current deposit, re-sort on 2%
+    strays, diagnostics every 100 steps).  Loading is outside the timed
+    region.  The EM push kernel is timed with HIP events on the sim stream
+    and priced at 40 B per particle (x, y, ux, uy, uz read and written)."""
     from arcwarden_amd import _lib as L
     from arcwarden_amd.sim import SimConfig, Simulation
     tot_ps, tot_s, kms, kl, n_last, sorts = 0.0, 0.0, 0.0, 0, 0, 0
@@ -196,7 +198,7 @@ def darwin_bench(steps: int):
     for r in ratios:
         g, rp, sp = whistler_run(r)
         sim = Simulation(g, rp, sp, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN,
-                                              sort_every=0))
+                                              sort_every=0, current_deposit=current))
         sim.init()
         n = sim.particle_views().n
         sim.synchronize()
@@ -214,12 +216,14 @@ def darwin_bench(steps: int):
         tot_ps += n * steps
         tot_s += el
         n_last = n
-        print(f"# darwin v_r/v_th={r}: {el / steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+        print(f"# darwin current={current} v_r/v_th={r}: {el / steps * 1e3:.3f} ms/step",
+              file=sys.stderr, flush=True)
     avg_k = kms / 1e3 / max(kl, 1)
     achieved = 40.0 * n_last / avg_k / 1e9
     return {"workload": "Darwin whistler pump, 3 runs v_r/v_th 3.2/2.1/1.0, 512x128, dx 0.5, "
                         "256 ppc (16,777,216 each), c 10, w_ce 0.5 at 30 deg, pump w0 0.15 "
                         "M=18 dB=1e-3 B0, dt 0.05, run after run",
+            "current_deposit": "esirkepov" if current == L.AW_CURRENT_ESIRKEPOV else "cic",
             "steps": steps, "value": tot_ps / tot_s, "unit": "particle-steps/s",
             "ms_per_step": tot_s / (len(ratios) * steps) * 1e3, "sorts": sorts,
             "em_push": {"avg_kernel_ms": avg_k * 1e3, "bytes_per_particle": 40.0,
@@ -438,7 +442,11 @@ def main():
               flush=True)
     ens = ensemble_bench(args.ensemble_steps) if args.ensemble_steps > 0 else None
     adj = adjoint_bench() if args.adjoint else None
-    dar = darwin_bench(args.darwin_steps) if args.darwin_steps > 0 else None
+    dar = dar_cic = None
+    if args.darwin_steps > 0:
+        from arcwarden_amd import _lib as L
+        dar = darwin_bench(args.darwin_steps, L.AW_CURRENT_ESIRKEPOV)
+        dar_cic = darwin_bench(args.darwin_steps, L.AW_CURRENT_CIC)
     pol = None if args.no_policies else policy_bench(nx, args.ppc)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -476,6 +484,7 @@ def main():
         "ensemble": ens,
         "adjoint": adj,
         "darwin": dar,
+        "darwin_cic_current": dar_cic,
         "policies": pol,
         "checks": {"charge_drift": charge_drift, "sorts": sorts,
                    "fixed_sort_cadence": fixed},

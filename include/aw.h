@@ -141,6 +141,13 @@ typedef struct aw_diag_sample {
 #define AW_MODEL_ES 0
 #define AW_MODEL_DARWIN 1
 
+/* Current deposit of the Darwin model (aw_sim_config.current_deposit). */
+#define AW_CURRENT_CIC 0        /* charge_current: CIC J at x^{n+1} (the reference) */
+#define AW_CURRENT_ESIRKEPOV 1  /* Esirkepov J of the move x^n -> x^{n+1} (staggered) */
+
+/* aw_solve_darwin_el_b_ex flags */
+#define AW_J_STAGGERED 1        /* Jx at (i+1/2, j), Jy at (i, j+1/2) */
+
 /* ---- input deck (Deck / load_deck, include/pic/deck.hpp:31-274) ------------- */
 
 #define AW_DECK_MAX_SPECIES 16
@@ -321,6 +328,13 @@ aw_status aw_solve_es(aw_ctx* ctx, const float* rho, float* Ex, float* Ey, doubl
  * (4*nx*ny floats, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz], B = B_self + B0. */
 aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, double eps0,
                                double c, const float B0[3], aw_stream s);
+/* aw_solve_darwin_el_b for a staggered current (flags = AW_J_STAGGERED): Jx
+ * sampled at (i+1/2, j) and Jy at (i, j+1/2), as the Esirkepov deposit writes
+ * them, are moved onto the nodes in k-space (J_k *= exp(-i k dx/2)) before B
+ * is formed.  No reference counterpart: the reference's Darwin solver takes
+ * the CIC (nodal) current only.  flags = 0 is aw_solve_darwin_el_b. */
+aw_status aw_solve_darwin_el_b_ex(aw_ctx* ctx, const float* src4, float* out5, double eps0,
+                                  double c, const float B0[3], int32_t flags, aw_stream s);
 /* DarwinSpectralSolver::solve_et (include/pic/solver_darwin.hpp:235-263):
  * src7 = [dcux|dcuy|dcuz|amu0|amu1|amu2|amu3] -> out3 = [ETx|ETy|ETz]. */
 aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double eps0,
@@ -388,6 +402,11 @@ typedef struct aw_sim_config {
                                push+deposit for tile 8/16/32 x 1..16 workgroups
                                per tile and keep the fastest (cached per grid
                                and particle count for the process) */
+    int32_t current_deposit; /* Darwin model: AW_CURRENT_CIC (the reference) or
+                               AW_CURRENT_ESIRKEPOV -- the next state's J is the
+                               charge-conserving current of the push's move
+                               (staggered, shifted onto the nodes in k-space);
+                               rho and amu stay CIC.  The t = 0 moments are CIC. */
 } aw_sim_config;
 
 aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,

@@ -623,10 +623,13 @@ void orc_solve_es_f64(const orc_grid* g, double eps0, const double* rho, double*
 }
 
 /* darwin_b_field_kernel (include/pic/solver_darwin.hpp:38-75):
- * B_k = i mu0 (k x J)/k^2, mu0 = 1/(eps0 c^2); DC and Nyquist zeroed. */
+ * B_k = i mu0 (k x J)/k^2, mu0 = 1/(eps0 c^2); DC and Nyquist zeroed.
+ * staggered: jx holds samples at (i+1/2, j) and jy at (i, j+1/2) (Esirkepov
+ * current); a sample offset by half a cell carries the factor exp(i k d/2)
+ * in its DFT, so J_k *= exp(-i k d/2) recovers the nodal spectrum. */
 void orc_solve_darwin_b_f64(const orc_grid* g, double eps0, double c, const double* jx,
                             const double* jy, const double* jz, double* bx, double* by,
-                            double* bz) {
+                            double* bz, int32_t staggered) {
     const int nx = g->nx, ny = g->ny, nkx = nx / 2 + 1;
     const size_t nk = (size_t)nkx * ny;
     double* J[3];
@@ -649,8 +652,16 @@ void orc_solve_darwin_b_f64(const orc_grid* g, double eps0, double c, const doub
                 continue;
             }
             const double G = 1.0 / (eps0 * c * c * k2);
-            const double jxr = J[0][2 * t], jxi = J[0][2 * t + 1];
-            const double jyr = J[1][2 * t], jyi = J[1][2 * t + 1];
+            double jxr = J[0][2 * t], jxi = J[0][2 * t + 1];
+            double jyr = J[1][2 * t], jyi = J[1][2 * t + 1];
+            if (staggered) {
+                const double ax = -0.5 * kx * g->dx, ay = -0.5 * ky * g->dy;
+                const double xr = jxr * cos(ax) - jxi * sin(ax);
+                const double xi = jxr * sin(ax) + jxi * cos(ax);
+                const double yr = jyr * cos(ay) - jyi * sin(ay);
+                const double yi = jyr * sin(ay) + jyi * cos(ay);
+                jxr = xr; jxi = xi; jyr = yr; jyi = yi;
+            }
             const double jzr = J[2][2 * t], jzi = J[2][2 * t + 1];
             B[0][2 * t] = -G * ky * jzi;  B[0][2 * t + 1] = G * ky * jzr;
             B[1][2 * t] = G * kx * jzi;   B[1][2 * t + 1] = -G * kx * jzr;
@@ -746,7 +757,15 @@ void orc_esirkepov_f64(const orc_grid* g, const float* x0, const float* y0,
                        double* jx, double* jy, double* jz) {
     for (int64_t p = 0; p < n; ++p) {
         const double qw = qm * (w ? (double)w[p] : w_uniform);
-        const double X0 = x0[p], Y0 = y0[p], X1 = x1[p], Y1 = y1[p];
+        const double X0 = x0[p], Y0 = y0[p];
+        double X1 = x1[p], Y1 = y1[p];
+        /* x1 wrapped by the periodic boundary: take its minimal image (the
+         * reference deposits before migrate, so its x1 is never wrapped and
+         * this is the identity there) */
+        if (X1 - X0 > 0.5 * g->nx) X1 -= g->nx;
+        else if (X0 - X1 > 0.5 * g->nx) X1 += g->nx;
+        if (Y1 - Y0 > 0.5 * g->ny) Y1 -= g->ny;
+        else if (Y0 - Y1 > 0.5 * g->ny) Y1 += g->ny;
         const int ib = (int)floor(fmin(X0, X1)), jb = (int)floor(fmin(Y0, Y1));
         double Sx0[4], Sx1[4], Sy0[4], Sy1[4], DSx[4], DSy[4];
         shape4_d(X0, ib, Sx0); shape4_d(X1, ib, Sx1);
@@ -942,6 +961,10 @@ int32_t orc_es_step_f32(const orc_grid* g, const orc_push_params* pp, double eps
     double* rd = (double*)malloc(sizeof(double) * (size_t)nc);
     double* exd = (double*)malloc(sizeof(double) * (size_t)nc);
     double* eyd = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (nc <= 0 || !rd || !exd || !eyd) {
+        free(rd); free(exd); free(eyd);
+        return -1;
+    }
     for (int64_t c = 0; c < nc; ++c) rd[c] = rho[c];
     orc_solve_es_f64(g, eps0, rd, exd, eyd);
     for (int64_t c = 0; c < nc; ++c) { Ex[c] = (float)exd[c]; Ey[c] = (float)eyd[c]; }

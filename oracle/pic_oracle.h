@@ -100,7 +100,7 @@ void orc_solve_es_f64(const orc_grid* g, double eps0, const double* rho, double*
                       double* Ey);
 void orc_solve_darwin_b_f64(const orc_grid* g, double eps0, double c, const double* jx,
                             const double* jy, const double* jz, double* bx, double* by,
-                            double* bz);
+                            double* bz, int32_t staggered);
 void orc_solve_darwin_et_f64(const orc_grid* g, double eps0, double c, double n0,
                              const double* dcx, const double* dcy, const double* dcz,
                              const double* a0, const double* a1, const double* a2,

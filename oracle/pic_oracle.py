@@ -227,10 +227,13 @@ def rho_nodes_f64(g, x, y, qm, w=None, w_uniform=0.0):
     return rho.reshape(g.ny, g.nx)
 
 
-def solve_darwin_b_f64(g, jx, jy, jz, eps0=1.0, c=1.0):
+def solve_darwin_b_f64(g, jx, jy, jz, eps0=1.0, c=1.0, staggered=False):
+    """B = curl^-1 of mu0 J (no B0).  staggered: jx at (i+1/2, j), jy at
+    (i, j+1/2), as the Esirkepov deposit writes them."""
     out = [np.empty((g.ny, g.nx)) for _ in range(3)]
     lib().orc_solve_darwin_b_f64(C.byref(g), C.c_double(eps0), C.c_double(c), _p(_d(jx)),
-                                 _p(_d(jy)), _p(_d(jz)), *(_p(a) for a in out))
+                                 _p(_d(jy)), _p(_d(jz)), *(_p(a) for a in out),
+                                 C.c_int32(1 if staggered else 0))
     return out
 
 

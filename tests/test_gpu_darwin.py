@@ -65,7 +65,8 @@ def test_darwin_et_closed_form(gpu):
 
 
 def darwin_sim(nx=64, ny=32, ppc=32, pump=True, sort_every=0, stray_frac=1.0, dump_every=1,
-               c=5.0, B0=(0.4, 0.0, 0.23), seed=11, v0=0.3, uth=0.05):
+               c=5.0, B0=(0.4, 0.0, 0.23), seed=11, v0=0.3, uth=0.05,
+               current=L.AW_CURRENT_CIC):
     g = L.grid(nx, ny, nx * 0.5, ny * 0.5)
     rp = L.run_params(dt=0.05, qm=-1.0, eps0=1.0, c=c, n0=1.0, noisy_load=1, rng_seed=seed,
                       B0=B0, dump_every=dump_every)
@@ -78,7 +79,7 @@ def darwin_sim(nx=64, ny=32, ppc=32, pump=True, sort_every=0, stray_frac=1.0, du
     sp = [L.species("beam_p", 0.5, ppc // 2, (uth,) * 3, (0, 0, v0)),
           L.species("beam_m", 0.5, ppc // 2, (uth,) * 3, (0, 0, -v0))]
     cfg = SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN, sort_every=sort_every,
-                    stray_frac=stray_frac)
+                    stray_frac=stray_frac, current_deposit=current)
     return Simulation(g, rp, sp, cfg), g, rp
 
 
@@ -207,3 +208,116 @@ def test_weibel_growth(gpu):
     assert int(np.argmax(eb)) < len(eb) - 1          # saturates
     assert 0.10 <= gamma <= 0.32                     # the reference's band
     sim.close()
+
+
+# ---- Esirkepov current in the Darwin model ---------------------------------------------
+
+def test_staggered_current_solve_closed_form(gpu):
+    """aw_solve_darwin_el_b_ex(AW_J_STAGGERED): a plane-wave current sampled
+    the way the Esirkepov deposit stores it gives the nodal B of the continuous
+    current (closed form); flags = 0 is aw_solve_darwin_el_b bit for bit."""
+    nx, ny, dx, dy = 32, 16, 0.5, 0.75
+    g = L.grid(nx, ny, nx * dx, ny * dy)
+    kx, ky = TWO_PI * 3 / (nx * dx), TWO_PI * 2 / (ny * dy)
+    X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dy)
+    ax, ay = 0.7, -0.4
+    src = np.zeros((4, ny, nx), np.float32)
+    src[1] = ax * np.cos(kx * (X + 0.5 * dx) + ky * Y)
+    src[2] = ay * np.cos(kx * X + ky * (Y + 0.5 * dy))
+    bz = -(kx * ay - ky * ax) / (kx * kx + ky * ky) * np.sin(kx * X + ky * Y)
+    b0 = (C.c_float * 3)(0, 0, 0)
+    with Ctx(g) as ctx:
+        outs = []
+        for flags in (L.AW_J_STAGGERED, 0):
+            out = torch.empty(5 * nx * ny, dtype=torch.float32, device="cuda")
+            L.call("aw_solve_darwin_el_b_ex", ctx.h, C.c_void_p(dev(src).data_ptr()),
+                   C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(1.0), b0, flags,
+                   stream())
+            outs.append(host(out).reshape(5, ny, nx))
+        out = torch.empty(5 * nx * ny, dtype=torch.float32, device="cuda")
+        L.call("aw_solve_darwin_el_b", ctx.h, C.c_void_p(dev(src).data_ptr()),
+               C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(1.0), b0, stream())
+        plain = host(out).reshape(5, ny, nx)
+        with pytest.raises(L.AwError):
+            L.call("aw_solve_darwin_el_b_ex", ctx.h, C.c_void_p(dev(src).data_ptr()),
+                   C.c_void_p(out.data_ptr()), C.c_double(1.0), C.c_double(1.0), b0, 2,
+                   stream())
+    stag = outs[0]
+    assert np.max(np.abs(stag[4] - bz)) < 1e-5 * np.max(np.abs(bz))
+    assert np.max(np.abs(stag[2])) < 1e-6 and np.max(np.abs(stag[3])) < 1e-6
+    assert np.array_equal(outs[1], plain)
+    assert np.max(np.abs(plain[4] - bz)) > 1e-2 * np.max(np.abs(bz))
+
+
+def test_esirkepov_current_step_matches_fp64_oracle(gpu, oracle):
+    """current_deposit = AW_CURRENT_ESIRKEPOV: the J the push deposits for the
+    next state is the charge-conserving current of the move (FP64 oracle,
+    1e-6), it satisfies continuity against the CIC charge of the two
+    positions (1e-6), rho and amu stay CIC at the new position, and B is
+    solved from the staggered current (FP64 oracle solve, 1e-5; no B0, so the
+    comparison sees the self field alone)."""
+    sim, g, rp = darwin_sim(current=L.AW_CURRENT_ESIRKEPOV, B0=(0.0, 0.0, 0.0))
+    sim.init()
+    s0 = sim.particles()
+    w = s0["w_uniform"]
+    sim.step(0)
+    s1 = sim.particles()
+    sim.step(1)
+    srcs = sim.sources_all()
+    f = sim.fields()
+    og = oracle.grid(g.nx, g.ny, g.dx, g.dy)
+    jx64, jy64, jz64 = oracle.esirkepov_f64(og, s0["x"], s0["y"], s1["x"], s1["y"], s1["uz"],
+                                            -1.0, rp.dt, w_uniform=w)
+    for name, ref in (("Jx", jx64), ("Jy", jy64), ("Jz", jz64)):
+        assert rel_l2(srcs[name], ref) < 1e-6, name
+    rho0 = oracle.deposit_rho_f64(og, s0["x"], s0["y"], -1.0, w_uniform=w)
+    rho1 = oracle.deposit_rho_f64(og, s1["x"], s1["y"], -1.0, w_uniform=w)
+    jx, jy = srcs["Jx"].astype(np.float64), srcs["Jy"].astype(np.float64)
+    div = (jx - np.roll(jx, 1, axis=1)) / g.dx + (jy - np.roll(jy, 1, axis=0)) / g.dy
+    resid = rho1 - rho0 + rp.dt * div
+    assert np.linalg.norm(resid) < 1e-6 * np.linalg.norm(rho1 - rho0)
+    assert rel_l2(srcs["rho"], rho1) < 1e-6
+    amu64 = oracle.deposit_amu_f64(og, s1, -1.0, w_uniform=w)
+    for q in range(4):
+        assert rel_l2(srcs[f"amu{q}"], amu64[q]) < 2e-6, q
+    b64 = oracle.solve_darwin_b_f64(og, jx64, jy64, jz64, c=rp.c, staggered=True)
+    for name, ref in zip(("Bx", "By", "Bz"), b64):
+        assert rel_l2(f[name], ref) < 1e-5, name
+    sim.close()
+
+
+def test_esirkepov_current_weibel_growth(gpu):
+    """The Weibel setup of test_weibel_growth with the Esirkepov current: the
+    magnetic energy grows at a rate in the reference's band."""
+    nx = ny = 64
+    g = L.grid(nx, ny, TWO_PI, TWO_PI)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, c=5.0, n0=1.0, ndc=2, noisy_load=1,
+                      rng_seed=20260628, dump_every=0)
+    sp = [L.species("beam_p", 0.5, 64, (0.05,) * 3, (0, 0, 1.0)),
+          L.species("beam_m", 0.5, 64, (0.05,) * 3, (0, 0, -1.0))]
+    sim = Simulation(g, rp, sp, SimConfig(field_model=L.AW_MODEL_DARWIN, sort_every=20,
+                                          current_deposit=L.AW_CURRENT_ESIRKEPOV))
+    sim.init()
+    ts, eb = [], []
+    for n in range(1000):
+        sim.step(n)
+        if n % 5 == 0:
+            f = sim.fields()
+            eb.append(0.5 * float(sum((f[k].astype(np.float64) ** 2).sum()
+                                      for k in ("Bx", "By", "Bz"))) * g.dx * g.dy)
+            ts.append(n * rp.dt)
+    eb = np.array(eb)
+    decades = (np.log(eb.max()) - np.log(eb.min())) / math.log(10)
+    gamma = physics.growth_rate(np.array(ts), eb)
+    print(f"Weibel (Esirkepov J): decades={decades:.1f} gamma={gamma:.4f}", flush=True)
+    assert decades > 3.0
+    assert 0.10 <= gamma <= 0.32
+    sim.close()
+
+
+def test_esirkepov_current_needs_darwin(gpu):
+    g = L.grid(16, 16, 8.0, 8.0)
+    rp = L.run_params(dt=0.1, qm=-1.0)
+    sp = [L.species("e", 1.0, 4, (0.05,) * 3, (0, 0, 0))]
+    with pytest.raises(L.AwError):
+        Simulation(g, rp, sp, SimConfig(current_deposit=L.AW_CURRENT_ESIRKEPOV))

@@ -332,6 +332,34 @@ def test_darwin_magnetostatic_closed_form(oracle):
     assert np.max(np.abs(by5 - np.sin(x)[None, :] / 25.0)) < 1e-12
 
 
+def staggered_plane_wave(nx, ny, dx, dy, mx, my, ax, ay):
+    """J = (ax, ay, 0) cos(k.r) with k = 2 pi (mx/Lx, my/Ly), sampled the way the
+    Esirkepov deposit stores it (Jx at (i+1/2, j), Jy at (i, j+1/2)), and the
+    nodal closed form B_z = -mu0 (kx ay - ky ax)/k^2 sin(k.r) (mu0 = 1)."""
+    kx, ky = TWO_PI * mx / (nx * dx), TWO_PI * my / (ny * dy)
+    X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dy)
+    jx = ax * np.cos(kx * (X + 0.5 * dx) + ky * Y)
+    jy = ay * np.cos(kx * X + ky * (Y + 0.5 * dy))
+    bz = -(kx * ay - ky * ax) / (kx * kx + ky * ky) * np.sin(kx * X + ky * Y)
+    return jx, jy, bz
+
+
+def test_darwin_b_staggered_current_closed_form(oracle):
+    """A staggered plane-wave current gives the nodal B of the continuous
+    current exactly once it is shifted by half a cell in k-space; read as a
+    nodal current it does not (the shift is an O(k dx) effect)."""
+    nx, ny, dx, dy = 32, 16, 0.5, 0.75
+    for mx, my, ax, ay in ((3, 2, 0.7, -0.4), (5, -3, -0.2, 1.1), (1, 4, 1.0, 0.3)):
+        jx, jy, bz = staggered_plane_wave(nx, ny, dx, dy, mx, my, ax, ay)
+        g = oracle.grid(nx, ny, dx, dy)
+        z = np.zeros_like(jx)
+        bx_s, by_s, bz_s = oracle.solve_darwin_b_f64(g, jx, jy, z, staggered=True)
+        assert np.max(np.abs(bz_s - bz)) < 1e-12 * max(1.0, np.max(np.abs(bz)))
+        assert np.max(np.abs(bx_s)) < 1e-12 and np.max(np.abs(by_s)) < 1e-12
+        _, _, bz_n = oracle.solve_darwin_b_f64(g, jx, jy, z, staggered=False)
+        assert np.max(np.abs(bz_n - bz)) > 1e-2 * np.max(np.abs(bz))
+
+
 def test_darwin_et_transverse_closed_form(oracle):
     """dcu = (cos y, cos x, 0), amu = 0: both already transverse, so
     E_T = -dcu/(eps0 c^2 k^2 + n0) mode by mode; a longitudinal dcu is removed."""
@@ -480,6 +508,22 @@ def test_esirkepov_jz_is_time_averaged_shape(oracle):
     _, _, jz = oracle.esirkepov_f64(g, [x0], [y0], [x0], [y0], [uz], qm, dt, w=[w])
     rho = oracle.rho_nodes_f64(g, [x0], [y0], qm, w=[w])
     assert np.max(np.abs(jz - uz * rho)) < 1e-12
+
+
+def test_esirkepov_wrapped_end_point_is_its_minimal_image(oracle):
+    """A move across the periodic boundary given with the wrapped end point
+    deposits exactly what the unwrapped move does (positions chosen so both
+    forms are exact in FP32), in x, in y and across the corner."""
+    nx, ny, dx, dy, dt, qm = 8, 6, 0.5, 0.75, 0.2, -1.0
+    g = oracle.grid(nx, ny, dx, dy)
+    for (x0, y0, x1, y1) in ((7.75, 2.5, 8.25, 2.75), (0.25, 3.0, -0.5, 2.5),
+                             (3.5, 5.75, 3.25, 6.5), (7.875, 0.125, 8.5, -0.25)):
+        ref = oracle.esirkepov_f64(g, [x0], [y0], [x1], [y1], [0.3], qm, dt, w_uniform=0.7)
+        got = oracle.esirkepov_f64(g, [x0], [y0], [x1 % nx], [y1 % ny], [0.3], qm, dt,
+                                   w_uniform=0.7)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+        assert np.max(np.abs(ref[0])) + np.max(np.abs(ref[1])) > 0.0
 
 
 # ---- discrete adjoint of the ES run ---------------------------------------------------
