@@ -166,6 +166,20 @@ struct FormE {
     float* E[3];
     const float* EL[2];
     const float* ET[3];
+    long long sE = 0, sL = 0, sT = 0;  // per-run strides of a batch (E, E_L, E_T)
+};
+
+// A uniform batch of Darwin runs (same grid, particle count and weights)
+// stored run after run: the particle kernels take blockIdx.y as the run and
+// offset each per-run pointer by run * stride.  runs = 1 is a single run.
+struct EmBatch {
+    int runs = 1;
+    long long pstride = 0;  // particles
+    long long estride = 0;  // E block (3 grids)
+    long long bstride = 0;  // B block
+    long long astride = 0;  // moment accumulators
+    long long kstride = 0;  // KE partials
+    int ostride = 0;        // tile offsets
 };
 
 // scratch of the standalone Darwin operators (aw_deposit_rho_j_amu, ...)
@@ -198,6 +212,8 @@ struct aw_ctx {
     rocfft_plan r2c4 = nullptr, r2c7 = nullptr, c2r5 = nullptr;  // Darwin batches
     rocfft_plan r2cB = nullptr, c2rB = nullptr;                  // ensemble batches
     int fft_batch = 0;
+    rocfft_plan r2c4B = nullptr, c2r5B = nullptr, r2c7B = nullptr, c2r3B = nullptr;
+    int em_batch = 0;                                            // Darwin batch size
     rocfft_execution_info info = nullptr;
     aw::DevBuf<unsigned char> fft_work;
     // workspace
@@ -405,26 +421,39 @@ __device__ __forceinline__ void boris_full(float qmh, float ex, float ey, float 
     uz = fmaf(qmh, ez, uz);
 }
 
-// rint(v) (ties to even) as int64.  For |v| < 2^51 one FP64 add does it:
-// v + 1.5*2^52 lies in [2^52, 2^53), where the spacing is 1, so the sum is
-// rounded to the nearest integer and its bit pattern exceeds that of 1.5*2^52
-// by exactly rint(v).  Larger magnitudes (never produced by the scales chosen
-// here) take the general conversion.
-__device__ __forceinline__ long long fix64(double v) {
+// rint(v) (ties to even) as int64 for |v| < 2^51, with one FP64 add: v +
+// 1.5*2^52 lies in [2^52, 2^53), where the spacing is 1, so the sum is rounded
+// to the nearest integer and its bit pattern exceeds that of 1.5*2^52 by
+// exactly rint(v).  Callers keep |v| < 2^51 by clamping a factor once (the
+// clamp_* helpers) rather than each contribution.
+__device__ __forceinline__ long long fix64_raw(double v) {
     constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
-    if (fabs(v) < 2251799813685248.0)              // 2^51
-        return __double_as_longlong(v + kMagic) - __double_as_longlong(kMagic);
-    return (long long)rint(v);
+    return __double_as_longlong(v + kMagic) - __double_as_longlong(kMagic);
 }
 
-// Exact rint(x) of an FP32 value as int64 (FP32 -> FP64 is exact).
-__device__ __forceinline__ long long f2ll_rn(float x) { return fix64((double)x); }
+// The fixed-point scales bound every contribution by 2^48 (J and amu assume
+// |u| <= 2c, so a particle would need |u| > 16c to reach 2^51).  Anything
+// larger, or NaN, saturates instead of wrapping.
+__device__ __forceinline__ double clamp_fix_f64(double v) {
+    constexpr double kLim = 2251799813685247.0;  // 2^51 - 1
+    return fmin(fmax(v, -kLim), kLim);
+}
+__device__ __forceinline__ float clamp_fix_f32(float v) {
+    constexpr float kLim = 2251799679467520.0f;  // 2^51 - 2^27, the largest float below 2^51
+    return fminf(fmaxf(v, -kLim), kLim);
+}
 
-// FP32 fixed-point contribution: rint(v * w) as a 64-bit two's-complement
-// integer, v carrying the quantity's 2^S scale.  Every product of at least
-// 2^23 is already an integer, so the sum keeps full FP32 product precision.
-__device__ __forceinline__ unsigned long long fixq32(float v, float w) {
-    return (unsigned long long)f2ll_rn(v * w);
+__device__ __forceinline__ long long fix64(double v) { return fix64_raw(clamp_fix_f64(v)); }
+
+
+// Fixed-point contribution rint(v * w) with one FP64 fma: the product of the
+// moment value v (carrying the quantity's 2^S scale) and the CIC weight w is
+// formed exactly and rounded once, as the magic-number add of fix64_raw.  v
+// must come through clamp_fix_f64 and |w| <= 1, so |v w| < 2^51.
+__device__ __forceinline__ unsigned long long fix_mul(double v, double w) {
+    constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+    return (unsigned long long)(__double_as_longlong(fma(v, w, kMagic)) -
+                                __double_as_longlong(kMagic));
 }
 
 // CIC weights in double (exact products of 24-bit factors) and the
@@ -487,7 +516,7 @@ __device__ __forceinline__ void esk_shape(double X0, double Y0, double X1, doubl
 }
 
 __device__ __forceinline__ unsigned long long esk_fix(double v) {
-    return (unsigned long long)fix64(v);
+    return (unsigned long long)fix64_raw(v);
 }
 
 // Calls add(q, k, b, v) for every non-zero fixed-point contribution v of
@@ -504,6 +533,11 @@ __device__ __forceinline__ unsigned long long esk_fix(double v) {
 template <class Add>
 __device__ __forceinline__ void esk_emit(const EskShape& m, double cJx, double cJy, double cJz,
                                          Add&& add) {
+    // every prefix sum and every Jz weight lies in [-1, 1]: clamping the
+    // coefficients keeps each product inside the range of fix64_raw
+    cJx = clamp_fix_f64(cJx);
+    cJy = clamp_fix_f64(cJy);
+    cJz = clamp_fix_f64(cJz);
 #pragma unroll
     for (int b = 1; b < 4; ++b) {
         const double sy = m.Sy0[b] + 0.5 * m.DSy[b];

@@ -38,7 +38,8 @@ int ke_partials(const aw_particles& p, double* part, hipStream_t s);
 int grid_partials(const float* Ex, const float* Ey, const float* rho, int64_t nc, double* part,
                   hipStream_t s);
 void finalize_diag(const double* kep, int nke, double w_scale, const double* gp, int ng,
-                   double* out4, hipStream_t s);
+                   double* out4, hipStream_t s, double* stage);
+size_t ke_stage_capacity(size_t nke);
 int reduce_partial_capacity();
 int reduce_grid_offset();
 int reduce_out_offset();
@@ -46,19 +47,21 @@ int tile_bpt(const aw_ctx& c, int64_t n);
 bool uz_evolves(const aw_run_params& rp, bool has_b0);
 void enable_em_fft(aw_ctx& c);
 void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
-                      const float B0[3], hipStream_t s, bool stag);
+                      const float B0[3], hipStream_t s, bool stag, int runs);
 void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
-                     double n0, hipStream_t s);
+                     double n0, hipStream_t s, int runs);
+void enable_em_batch_fft(aw_ctx& c, int R);
+int dcu_part_capacity();
 void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
                 unsigned long long* acc8, const float* cq, int bpt, unsigned long long* strays,
-                hipStream_t s);
+                hipStream_t s, const EmBatch& b);
 void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc3, const float* c_dcu, int bpt,
-            unsigned long long* strays, hipStream_t s);
+            unsigned long long* strays, hipStream_t s, const EmBatch& b);
 int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc8, const float* cq, double qm,
             double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s,
-            bool esk);
+            bool esk, const EmBatch& b);
 void deposit_rho_j_amu(aw_ctx& c, const aw_particles& p, double qm, float* const out[8],
                        hipStream_t s);
 void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const* F,
@@ -69,12 +72,13 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,
                        hipStream_t s);
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
-                   const double* inv, hipStream_t s);
-void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t,
-            hipStream_t s);
+                   const double* inv, hipStream_t s, int runs, long long sa, long long so,
+                   long long so2, int si);
+void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t, hipStream_t s,
+            int runs);
 void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_abs,
                double coef_signed, double umax, const double* dev, float* c_dcu, double* inv_dcu,
-               hipStream_t s);
+               hipStream_t s, int runs, long long sE, long long sB, int si);
 
 namespace {
 
@@ -151,8 +155,18 @@ struct aw_sim {
     DevBuf<unsigned long long> strays;
     DevBuf<double> ke_part, grid_part, diag_out;
     int diag_cap = 256;
-    std::vector<int64_t> diag_pending;  // steps with a slot in diag_out
-    std::vector<aw_diag_sample> history;
+    std::vector<int64_t> diag_pending;  // steps with a slot in diag_out (runs slots each)
+    std::vector<std::vector<aw_diag_sample>> hist;  // per run
+
+    // Darwin batch (aw_ensemble_create with the Darwin model): runs of one
+    // shape (grid, particle count, weights, step parameters) stored run after
+    // run in every particle and grid buffer; the kernels take the run from
+    // blockIdx.y.  runs = 1 is a plain simulation.
+    int runs = 1;
+    std::vector<std::vector<aw_species>> rspecies;  // species of run r (runs > 1)
+    std::vector<aw_run_params> rrp;                 // load parameters (seed) of run r
+    std::vector<std::vector<double>> rmodes;        // mode amplitudes of run r
+    size_t kcap = 0;                                // KE partial slots per run
     bool initialized = false, fields_ready = false, sort_requested = false;
     int64_t steps_since_sort = 0, nsorts = 0;
     unsigned long long* stray_host = nullptr;
@@ -182,10 +196,28 @@ struct aw_sim {
     double dcu_abs = 0.0, dcu_signed = 0.0, umax = 1.0;
     bool moments_ready = false;
 
-    float* Ex() const { return em ? e3.get() : E.get(); }
-    float* Ey() const { return em ? e3.get() + ncell(*ctx) : E.get() + ncell(*ctx); }
-    float* rho_ptr() const { return em ? src.get() : rho.get(); }
+    float* Ex(int r = 0) const {
+        return em ? e3.get() + (size_t)r * 3 * ncell(*ctx) : E.get();
+    }
+    float* Ey(int r = 0) const { return Ex(r) + ncell(*ctx); }
+    float* rho_ptr(int r = 0) const {
+        return em ? src.get() + (size_t)r * 11 * ncell(*ctx) : rho.get();
+    }
     float* B3() const { return elb.get() + 2 * ncell(*ctx); }
+    // the uniform batch description of the particle kernels (astride: the
+    // per-run size of the accumulator they write)
+    EmBatch batch(long long astride) const {
+        const long long nc = ncell(*ctx);
+        EmBatch b;
+        b.runs = runs;
+        b.pstride = n;
+        b.estride = 3 * nc;
+        b.bstride = 5 * nc;
+        b.astride = astride;
+        b.kstride = (long long)kcap;
+        b.ostride = ctx->ntiles + 1;
+        return b;
+    }
     // E (3 grids) and B as the six component pointers of the particle kernels
     void fields6(const float* E3, const float* F[6]) const {
         const int64_t nc = ncell(*ctx);
@@ -199,17 +231,19 @@ struct aw_sim {
         em = true;
         const int64_t nc = ncell(*ctx);
         enable_em_fft(*ctx);
-        acc8.resize((size_t)8 * nc);
-        acc3.resize((size_t)3 * nc);
-        src.resize((size_t)11 * nc);
-        fftin.resize((size_t)11 * nc);
-        elb.resize((size_t)5 * nc);
-        et.resize((size_t)3 * nc);
-        e3.resize((size_t)3 * nc);
-        ed3.resize((size_t)3 * nc);
-        inv.resize(11);
-        cdcu.resize(1);
-        fmax_part.resize(512);
+        const size_t R = (size_t)runs;
+        if (runs > 1) enable_em_batch_fft(*ctx, runs);
+        acc8.resize(R * 8 * nc);
+        acc3.resize(R * 3 * nc);
+        src.resize(R * 11 * nc);
+        fftin.resize(R * 11 * nc);  // [run][4] R2C inputs, then [run][7]
+        elb.resize(R * 5 * nc);
+        et.resize(R * 3 * nc);
+        e3.resize(R * 3 * nc);
+        ed3.resize(R * 3 * nc);
+        inv.resize(R * 11);
+        cdcu.resize(R);
+        fmax_part.resize(R * (size_t)dcu_part_capacity());
         // static per-quantity fixed-point scales: the per-contribution bound maps
         // to 2^48, leaving 2^14 bound-sized contributions of headroom per cell
         double wmax = w_uniform;
@@ -229,7 +263,8 @@ struct aw_sim {
         double h[11] = {std::ldexp(1.0, -Sr), std::ldexp(1.0, -Sj), std::ldexp(1.0, -Sj),
                         std::ldexp(1.0, -Sj), std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa),
                         std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa), 1.0, 1.0, 1.0};
-        AW_HIP(hipMemcpy(inv.get(), h, sizeof(h), hipMemcpyHostToDevice));
+        for (int r = 0; r < runs; ++r)
+            AW_HIP(hipMemcpy(inv.get() + (size_t)r * 11, h, sizeof(h), hipMemcpyHostToDevice));
         cqd.resize(3);
         AW_HIP(hipMemcpy(cqd.get(), cq, sizeof(cq), hipMemcpyHostToDevice));
         dcu_abs = rp.qm * rp.qm * wmax / area;
@@ -240,8 +275,10 @@ struct aw_sim {
     }
 
     void init_em() {
+        acc8.zero(stream);
+        acc3.zero(stream);
         em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
-                   nullptr, stream);
+                   nullptr, stream, batch(8 * ncell(*ctx)));
         moments_ready = true;
         j_staggered = false;
     }
@@ -251,35 +288,46 @@ struct aw_sim {
         const hipStream_t s = stream;
         const int64_t nc = ncell(*ctx);
         const double t = (double)nstep * rp.dt;
+        const int R = runs;
         if (!moments_ready) {
             em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
-                       nullptr, s);
+                       nullptr, s, batch(8 * nc));
             j_staggered = false;
         }
         moments_ready = false;
-        // rho, J -> src4; amu -> src7[3..6]; fftin holds the copies rocFFT may clobber
-        float* fin = fftin.get();
-        convert_multi(acc8.get(), src.get(), fin, nc, 4, inv.get(), s);
-        convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin + 7 * nc, nc, 4, inv.get() + 4,
-                      s);
-        solve_darwin_elb(*ctx, fin, elb.get(), rp.eps0, rp.c, rp.B0, s, j_staggered);
-        form_e(FormE{{ed3.get(), ed3.get() + nc, ed3.get() + 2 * nc},
-                     {elb.get(), elb.get() + nc},
-                     {nullptr, nullptr, nullptr}},
-               g, rp, t, s);
+        // per run: rho, J -> src[0..3] and the R2C input block [run][4]; amu ->
+        // src[7..10] and [run][7] entries 3..6 (fftin holds the copies rocFFT
+        // may overwrite: [R x 4 grids][R x 7 grids])
+        float* fin4 = fftin.get();
+        float* fin7 = fftin.get() + (size_t)R * 4 * nc;
+        convert_multi(acc8.get(), src.get(), fin4, nc, 4, inv.get(), s, R, 8 * nc, 11 * nc,
+                      4 * nc, 11);
+        convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin7 + 3 * nc, nc, 4,
+                      inv.get() + 4, s, R, 8 * nc, 11 * nc, 7 * nc, 11);
+        solve_darwin_elb(*ctx, fin4, elb.get(), rp.eps0, rp.c, rp.B0, s, j_staggered, R);
+        FormE fd{{ed3.get(), ed3.get() + nc, ed3.get() + 2 * nc},
+                 {elb.get(), elb.get() + nc},
+                 {nullptr, nullptr, nullptr}};
+        fd.sE = 3 * nc;
+        fd.sL = 5 * nc;
+        form_e(fd, g, rp, t, s, R);
         if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) do_sort();
         const float* Fd[6];
         fields6(ed3.get(), Fd);
         dcu_scale(Fd, nc, fmax_part.get(), dcu_abs, dcu_signed, umax, nullptr, cdcu.get(),
-                  inv.get() + 8, s);
+                  inv.get() + 8, s, R, 3 * nc, 5 * nc, 11);
         em_dcu(*ctx, particles(), tile_off.get(), nullptr, Fd, acc3.get(), cdcu.get(), bpt,
-               strays.get(), s);
-        convert_multi(acc3.get(), src.get() + 4 * nc, fin + 4 * nc, nc, 3, inv.get() + 8, s);
-        solve_darwin_et(*ctx, fin + 4 * nc, et.get(), rp.eps0, rp.c, rp.n0, s);
-        form_e(FormE{{e3.get(), e3.get() + nc, e3.get() + 2 * nc},
-                     {elb.get(), elb.get() + nc},
-                     {et.get(), et.get() + nc, et.get() + 2 * nc}},
-               g, rp, t, s);
+               strays.get(), s, batch(3 * nc));
+        convert_multi(acc3.get(), src.get() + 4 * nc, fin7, nc, 3, inv.get() + 8, s, R, 3 * nc,
+                      11 * nc, 7 * nc, 11);
+        solve_darwin_et(*ctx, fin7, et.get(), rp.eps0, rp.c, rp.n0, s, R);
+        FormE fe{{e3.get(), e3.get() + nc, e3.get() + 2 * nc},
+                 {elb.get(), elb.get() + nc},
+                 {et.get(), et.get() + nc, et.get() + 2 * nc}};
+        fe.sE = 3 * nc;
+        fe.sL = 5 * nc;
+        fe.sT = 3 * nc;
+        form_e(fe, g, rp, t, s, R);
         const bool dump = rp.dump_every <= 0 || (nstep % rp.dump_every) == 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
@@ -292,7 +340,7 @@ struct aw_sim {
         const bool esk = cfg.current_deposit == AW_CURRENT_ESIRKEPOV;
         const int nblk = em_push(*ctx, particles(), tile_off.get(), nullptr, Fp, acc8.get(),
                                  cqd.get(), rp.qm, rp.dt, bpt, strays.get(),
-                                 dump ? ke_part.get() : nullptr, s, esk);
+                                 dump ? ke_part.get() : nullptr, s, esk, batch(8 * nc));
         j_staggered = esk;
         if (prof) {
             AW_HIP(hipEventRecord(e1, s));
@@ -311,22 +359,31 @@ struct aw_sim {
         ++steps_since_sort;
     }
 
+    static constexpr size_t kGridPart = 3 * 512 + 3;  // grid partial slots per run
+
     void record_diag(int64_t nstep, int nblk) {
         if ((int)diag_pending.size() >= diag_cap) flush_diag();
-        const int ng = grid_partials(Ex(), Ey(), rho_ptr(), ncell(*ctx), grid_part.get(), stream);
-        finalize_diag(ke_part.get(), nblk, per_w ? 1.0 : w_uniform, grid_part.get(), ng,
-                      diag_out.get() + 4 * diag_pending.size(), stream);
+        for (int r = 0; r < runs; ++r) {
+            double* gp = grid_part.get() + (size_t)r * kGridPart;
+            double* kp = ke_part.get() + (size_t)r * kcap;
+            const int ng = grid_partials(Ex(r), Ey(r), rho_ptr(r), ncell(*ctx), gp, stream);
+            finalize_diag(kp, nblk, per_w ? 1.0 : w_uniform, gp, ng,
+                          diag_out.get() + 4 * (diag_pending.size() * runs + r), stream,
+                          kp + nblk);
+        }
         diag_pending.push_back(nstep);
     }
 
-    aw_particles particles(int which) const {
+    // buffer set `which`, run r (n particles; run 0 also heads a batch)
+    aw_particles particles(int which, int r = 0) const {
+        const size_t o = (size_t)r * n;
         aw_particles p{};
-        p.x = P[which][0].get();
-        p.y = P[which][1].get();
-        p.ux = P[which][2].get();
-        p.uy = P[which][3].get();
-        p.uz = P[which][4].get();
-        p.w = per_w ? P[which][5].get() : nullptr;
+        p.x = P[which][0].get() + o;
+        p.y = P[which][1].get() + o;
+        p.ux = P[which][2].get() + o;
+        p.uy = P[which][3].get() + o;
+        p.uz = P[which][4].get() + o;
+        p.w = per_w ? P[which][5].get() + o : nullptr;
         p.n = n;
         p.w_uniform = w_uniform;
         return p;
@@ -345,8 +402,9 @@ struct aw_sim {
     }
 
     void do_sort() {
-        const aw_particles src = particles(cur), dst = particles(cur ^ 1);
-        sort_tiles(*ctx, src, dst, perm.get(), tile_off.get(), stream);
+        for (int r = 0; r < runs; ++r)
+            sort_tiles(*ctx, particles(cur, r), particles(cur ^ 1, r), perm.get(),
+                       tile_off.get() + (size_t)r * (ctx->ntiles + 1), stream);
         cur ^= 1;
         steps_since_sort = 0;
         ++nsorts;
@@ -355,22 +413,24 @@ struct aw_sim {
 
     void flush_diag() {
         if (diag_pending.empty()) return;
-        std::vector<double> h(diag_pending.size() * 4);
+        std::vector<double> h(diag_pending.size() * 4 * runs);
         AW_HIP(hipMemcpyAsync(h.data(), diag_out.get(), h.size() * sizeof(double),
                               hipMemcpyDeviceToHost, stream));
         AW_HIP(hipStreamSynchronize(stream));
         const double dxdy = g.dx * g.dy;
-        for (size_t k = 0; k < diag_pending.size(); ++k) {
-            aw_diag_sample d{};
-            d.step = diag_pending[k];
-            d.time = (double)d.step * rp.dt;
-            d.ke = 0.5 * h[4 * k];
-            d.ee = 0.5 * rp.eps0 * h[4 * k + 1] * dxdy;
-            d.charge = h[4 * k + 3] * dxdy;
-            d.max_e = std::sqrt(h[4 * k + 2]);
-            d.total = d.ke + d.ee;
-            history.push_back(d);
-        }
+        for (size_t k = 0; k < diag_pending.size(); ++k)
+            for (int r = 0; r < runs; ++r) {
+                const double* v = h.data() + 4 * (k * runs + r);
+                aw_diag_sample d{};
+                d.step = diag_pending[k];
+                d.time = (double)d.step * rp.dt;
+                d.ke = 0.5 * v[0];
+                d.ee = 0.5 * rp.eps0 * v[1] * dxdy;
+                d.charge = v[3] * dxdy;
+                d.max_e = std::sqrt(v[2]);
+                d.total = d.ke + d.ee;
+                hist[r].push_back(d);
+            }
         diag_pending.clear();
     }
 
@@ -387,7 +447,7 @@ struct aw_sim {
         else if (hipEventQuery(stray_ev) != hipSuccess) return;
         stray_pending = false;
         const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
-        if ((double)*stray_host > frac * (double)n) sort_requested = true;
+        if ((double)*stray_host > frac * (double)n * runs) sort_requested = true;
     }
 
     // Tile geometry and workgroups per tile of the particle kernels.
@@ -395,21 +455,31 @@ struct aw_sim {
         AW_HIP(hipStreamSynchronize(stream));
         set_ctx_tiles(*ctx, tile, tile);
         bpt = nbpt > 0 ? nbpt : tile_bpt(*ctx, n);
-        tile_off.resize((size_t)ctx->ntiles + 1);
-        ke_part.resize((size_t)ctx->ntiles * bpt + 1);
+        size_buffers();
     }
 
-    // Autotuner (ES): for tile 8/16/32 the state is loaded and initialised,
-    // then the fused push+deposit is timed (HIP events, best of 3, each
-    // trial on a copy of the particles) for 1..16 workgroups per tile; the
-    // fastest pair is kept and cached for the process per (device, grid,
-    // particle count, weights, uz invariance).  Every deposit is fixed-point
-    // and order-free, so the choice changes no result.
+    // tile offsets and KE partials (per run) for the current tiles and bpt
+    void size_buffers() {
+        tile_off.resize((size_t)runs * (ctx->ntiles + 1));
+        kcap = ke_stage_capacity((size_t)ctx->ntiles * bpt);
+        ke_part.resize((size_t)runs * kcap);
+    }
+
+    // Autotuner: for tile 8/16/32 the state is loaded and initialised, then the
+    // particle kernels of one step are timed (HIP events, best of three, each on
+    // a copy of the particles) for 1..16 workgroups per tile: the fused
+    // push+deposit (ES), or the dcu deposit plus the EM push (Darwin, on zero
+    // fields; a batch times its first run).  The fastest pair is kept and cached
+    // per device, grid, particle count, weights and kernel variant for the
+    // process.  Every deposit is fixed-point and order-free, so the choice
+    // changes no result.
     void autotune() {
-        using Key = std::tuple<int, int, int, int64_t, bool, bool>;
+        const bool esk = cfg.current_deposit == AW_CURRENT_ESIRKEPOV;
+        using Key = std::tuple<int, int, int, int64_t, bool, bool, bool, bool>;
         static std::mutex mu;
         static std::map<Key, std::pair<int, int>> cache;
-        const Key key{ctx->device, g.nx, g.ny, n, per_w, uz_evolves(rp, cfg.has_b0 != 0)};
+        const Key key{ctx->device, g.nx, g.ny,  n,  per_w, uz_evolves(rp, cfg.has_b0 != 0),
+                      em,          esk};
         {
             std::lock_guard<std::mutex> lk(mu);
             const auto it = cache.find(key);
@@ -428,10 +498,19 @@ struct aw_sim {
         AW_HIP(hipEventCreate(&ev.a));
         AW_HIP(hipEventCreate(&ev.b));
         const hipStream_t s = stream;
+        if (em) {  // the timed kernels read zero fields and a zero dcu coefficient
+            e3.zero(s);
+            elb.zero(s);
+            cdcu.zero(s);
+        }
         float best = 0.f;
         int best_tile = 0, best_bpt = 0;
         for (const int tile : {8, 16, 32}) {
             if (tile > g.nx || tile > g.ny) continue;
+            if (em) {  // the EM push's LDS frame: 8 int64 moments + 6 field floats per node
+                const size_t side = (size_t)tile + 2 * ctx->apron + 1;
+                if (side * side * (8 * 8 + 24) > ctx->lds_per_block) continue;
+            }
             apply_exec(tile, 1);
             init_state();
             const int64_t per_tile = n / ctx->ntiles;
@@ -447,8 +526,17 @@ struct aw_sim {
                     AW_HIP(hipMemcpyAsync(dst.uy, src.uy, bytes, hipMemcpyDeviceToDevice, s));
                     AW_HIP(hipMemcpyAsync(dst.uz, src.uz, bytes, hipMemcpyDeviceToDevice, s));
                     AW_HIP(hipEventRecord(ev.a, s));
-                    push_deposit_es(*ctx, dst, tile_off.get(), Ex(), Ey(), rp, cfg.has_b0 != 0,
-                                    strays.get(), nullptr, b, s);
+                    if (em) {
+                        const float* F[6];
+                        fields6(e3.get(), F);
+                        em_dcu(*ctx, dst, tile_off.get(), nullptr, F, acc3.get(), cdcu.get(), b,
+                               strays.get(), s, EmBatch{});
+                        em_push(*ctx, dst, tile_off.get(), nullptr, F, acc8.get(), cqd.get(),
+                                rp.qm, rp.dt, b, strays.get(), nullptr, s, esk, EmBatch{});
+                    } else {
+                        push_deposit_es(*ctx, dst, tile_off.get(), Ex(), Ey(), rp,
+                                        cfg.has_b0 != 0, strays.get(), nullptr, b, s);
+                    }
                     AW_HIP(hipEventRecord(ev.b, s));
                     AW_HIP(hipEventSynchronize(ev.b));
                     float ms = 0.f;
@@ -471,7 +559,7 @@ struct aw_sim {
     }
 
     void init() {
-        if (cfg.autotune && !em && n > 0) autotune();
+        if (cfg.autotune && n > 0) autotune();
         init_state();
     }
 
@@ -480,10 +568,17 @@ struct aw_sim {
         const aw_particles p = particles(cur);
         if (cfg.legacy_load) {
             load_legacy(*ctx, rp, p, s);
-        } else {
+        } else if (rspecies.empty()) {
             std::vector<double> wts(species.size());
             load_species(*ctx, species.data(), (int)species.size(), rp, cfg.mode_amp,
                          cfg.n_modes, p, wts.data(), s);
+        } else {
+            for (int r = 0; r < runs; ++r) {
+                std::vector<double> wts(rspecies[r].size());
+                load_species(*ctx, rspecies[r].data(), (int)rspecies[r].size(), rrp[r],
+                             rmodes[r].empty() ? nullptr : rmodes[r].data(),
+                             (int)rmodes[r].size(), particles(cur, r), wts.data(), s);
+            }
         }
         do_sort();
         if (em) {  // Darwin: t=0 moments; fields are solved inside each step
@@ -869,7 +964,7 @@ aw_status aw_solve_darwin_el_b(aw_ctx* ctx, const float* src4, float* out5, doub
         AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
         const float zero[3] = {0.f, 0.f, 0.f};
         enable_em_fft(*ctx);
-        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s), false);
+        solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s), false, 1);
     });
 }
 
@@ -882,7 +977,7 @@ aw_status aw_solve_darwin_el_b_ex(aw_ctx* ctx, const float* src4, float* out5, d
         const float zero[3] = {0.f, 0.f, 0.f};
         enable_em_fft(*ctx);
         solve_darwin_elb(*ctx, src4, out5, eps0, c, B0 ? B0 : zero, S(s),
-                         (flags & AW_J_STAGGERED) != 0);
+                         (flags & AW_J_STAGGERED) != 0, 1);
     });
 }
 
@@ -892,7 +987,7 @@ aw_status aw_solve_darwin_et(aw_ctx* ctx, const float* src7, float* out3, double
         AW_REQUIRE(ctx && src7 && out3, "NULL argument");
         AW_REQUIRE(eps0 > 0 && c > 0, "eps0 and c must be positive");
         enable_em_fft(*ctx);
-        solve_darwin_et(*ctx, src7, out3, eps0, c, n0, S(s));
+        solve_darwin_et(*ctx, src7, out3, eps0, c, n0, S(s), 1);
     });
 }
 
@@ -903,7 +998,7 @@ aw_status aw_form_e(aw_ctx* ctx, aw_fields* f, const aw_run_params* rp, double t
         AW_REQUIRE(f->Ex && f->Ey && f->Ez && f->ELx && f->ELy, "E / E_L grids must be non-NULL");
         AW_REQUIRE(!f->ETx || (f->ETy && f->ETz), "E_T must be all NULL or all non-NULL");
         form_e(FormE{{f->Ex, f->Ey, f->Ez}, {f->ELx, f->ELy}, {f->ETx, f->ETy, f->ETz}},
-               ctx->g, *rp, t, S(s));
+               ctx->g, *rp, t, S(s), 1);
     });
 }
 
@@ -967,7 +1062,7 @@ aw_status aw_reduce_energies(aw_ctx* ctx, const aw_particles* p, const float* Ex
         double* gp = part + reduce_grid_offset();
         double* o = part + reduce_out_offset();
         const int ng = grid_partials(Ex, Ey, rho, ncell(*ctx), gp, S(s));
-        finalize_diag(part, nke, p->w ? 1.0 : p->w_uniform, gp, ng, o, S(s));
+        finalize_diag(part, nke, p->w ? 1.0 : p->w_uniform, gp, ng, o, S(s), nullptr);
         double h[4];
         AW_HIP(hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, S(s)));
         AW_HIP(hipStreamSynchronize(S(s)));
@@ -984,80 +1079,97 @@ aw_status aw_reduce_energies(aw_ctx* ctx, const aw_particles* p, const float* Ex
 
 // ---- simulation ------------------------------------------------------------------
 
+// A validated simulation with its context and stream (buffers: finish_sim).
+static std::unique_ptr<aw_sim> make_sim(const aw_grid* g, const aw_run_params* rp,
+                                        const aw_species* sp, int32_t nspecies,
+                                        const aw_sim_config* cfg) {
+    AW_REQUIRE(rp && cfg, "rp/cfg is NULL");
+    check_grid(g);
+    AW_REQUIRE(cfg->field_model == AW_MODEL_ES || cfg->field_model == AW_MODEL_DARWIN,
+               "field_model must be AW_MODEL_ES or AW_MODEL_DARWIN");
+    if (cfg->field_model == AW_MODEL_DARWIN)
+        AW_REQUIRE(rp->c > 0 && rp->n0 >= 0, "Darwin needs rp.c > 0 and rp.n0 >= 0");
+    AW_REQUIRE(rp->dt > 0, "rp.dt must be positive");
+    AW_REQUIRE(rp->eps0 > 0, "rp.eps0 must be positive");
+    AW_REQUIRE(cfg->current_deposit == AW_CURRENT_CIC ||
+                   (cfg->current_deposit == AW_CURRENT_ESIRKEPOV &&
+                    cfg->field_model == AW_MODEL_DARWIN),
+               "current_deposit must be AW_CURRENT_CIC, or AW_CURRENT_ESIRKEPOV with "
+               "the Darwin model");
+    auto sim = std::make_unique<aw_sim>();
+    sim->g = *g;
+    if (sim->g.Lx <= 0) sim->g.Lx = g->nx * g->dx;
+    if (sim->g.Ly <= 0) sim->g.Ly = g->ny * g->dy;
+    sim->rp = *rp;
+    sim->cfg = *cfg;
+    AW_REQUIRE(cfg->n_modes >= 0 && cfg->n_modes <= 32, "n_modes must be in [0, 32]");
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    if (cfg->legacy_load) {
+        AW_REQUIRE(rp->ppc > 0, "legacy load needs rp.ppc > 0");
+        sim->n = (int64_t)rp->ppc * nc;
+        sim->w_uniform = rp->weight > 0 ? rp->weight : rp->n0 * g->dx * g->dy / rp->ppc;
+        sim->rp.weight = sim->w_uniform;
+    } else {
+        AW_REQUIRE(sp && nspecies > 0, "at least one species is required");
+        sim->species.assign(sp, sp + nspecies);
+        int64_t total = 0;
+        double w0 = -1.0;
+        bool uniform = true;
+        for (int k = 0; k < nspecies; ++k) {
+            AW_REQUIRE(sp[k].ppc > 0, "species ppc must be positive");
+            total += (int64_t)sp[k].ppc * nc;
+            const double wk = sp[k].density * g->dx * g->dy / sp[k].ppc;
+            if (w0 < 0) w0 = wk;
+            else if (wk != w0) uniform = false;
+        }
+        sim->n = total;
+        sim->w_uniform = w0;
+        sim->per_w = !uniform;
+    }
+    AW_REQUIRE(sim->n < (int64_t)INT32_MAX, "particle count must fit int32");
+    AW_HIP(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
+    // default ES tile: 8x8 at >= 64 particles per cell (more, smaller
+    // workgroups; the apron overhead stays small), else 16x16
+    int tx = cfg->tile_x, ty = cfg->tile_y;
+    if (tx <= 0 && ty <= 0 && cfg->field_model != AW_MODEL_DARWIN)
+        tx = ty = sim->n >= 64 * nc ? 8 : 16;
+    sim->ctx.reset(make_ctx(&sim->g, tx, ty));
+    sim->ctx->fix_log2 = cfg->fixed_scale > 0
+                             ? cfg->fixed_scale
+                             : fixed_scale_for(sim->species, *rp, cfg->legacy_load != 0);
+    return sim;
+}
+
+// Buffers of a validated simulation for `runs` runs of its shape.
+static std::unique_ptr<aw_sim> finish_sim(std::unique_ptr<aw_sim> sim, int runs) {
+    const int64_t nc = ncell(*sim->ctx);
+    sim->runs = runs;
+    const size_t n = (size_t)std::max<int64_t>(1, sim->n);
+    const size_t R = (size_t)runs;
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < (sim->per_w ? 6 : 5); ++a) sim->P[b][a].resize(R * n);
+    sim->perm.resize(n);
+    sim->rho.resize((size_t)nc);
+    sim->E.resize((size_t)2 * nc);
+    sim->strays.resize(1);
+    sim->bpt = sim->cfg.blocks_per_tile > 0 ? sim->cfg.blocks_per_tile
+                                            : tile_bpt(*sim->ctx, sim->n);
+    sim->size_buffers();
+    sim->grid_part.resize(R * aw_sim::kGridPart);
+    sim->diag_out.resize(R * 4 * sim->diag_cap);
+    sim->hist.assign(R, {});
+    if (sim->cfg.field_model == AW_MODEL_DARWIN) sim->setup_em();
+    AW_HIP(hipHostMalloc((void**)&sim->stray_host, sizeof(unsigned long long),
+                         hipHostMallocDefault));
+    AW_HIP(hipEventCreateWithFlags(&sim->stray_ev, hipEventDisableTiming));
+    return sim;
+}
+
 aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
                         int32_t nspecies, const aw_sim_config* cfg, aw_sim** out) {
     return guard("aw_sim_create", [&] {
-        AW_REQUIRE(out && rp && cfg, "out/rp/cfg is NULL");
-        check_grid(g);
-        AW_REQUIRE(cfg->field_model == AW_MODEL_ES || cfg->field_model == AW_MODEL_DARWIN,
-                   "field_model must be AW_MODEL_ES or AW_MODEL_DARWIN");
-        if (cfg->field_model == AW_MODEL_DARWIN)
-            AW_REQUIRE(rp->c > 0 && rp->n0 >= 0, "Darwin needs rp.c > 0 and rp.n0 >= 0");
-        AW_REQUIRE(rp->dt > 0, "rp.dt must be positive");
-        AW_REQUIRE(rp->eps0 > 0, "rp.eps0 must be positive");
-        AW_REQUIRE(cfg->current_deposit == AW_CURRENT_CIC ||
-                       (cfg->current_deposit == AW_CURRENT_ESIRKEPOV &&
-                        cfg->field_model == AW_MODEL_DARWIN),
-                   "current_deposit must be AW_CURRENT_CIC, or AW_CURRENT_ESIRKEPOV with "
-                   "the Darwin model");
-        auto sim = std::make_unique<aw_sim>();
-        sim->g = *g;
-        if (sim->g.Lx <= 0) sim->g.Lx = g->nx * g->dx;
-        if (sim->g.Ly <= 0) sim->g.Ly = g->ny * g->dy;
-        sim->rp = *rp;
-        sim->cfg = *cfg;
-        AW_REQUIRE(cfg->n_modes >= 0 && cfg->n_modes <= 32, "n_modes must be in [0, 32]");
-        const int64_t nc = (int64_t)g->nx * g->ny;
-        if (cfg->legacy_load) {
-            AW_REQUIRE(rp->ppc > 0, "legacy load needs rp.ppc > 0");
-            sim->n = (int64_t)rp->ppc * nc;
-            sim->w_uniform = rp->weight > 0 ? rp->weight : rp->n0 * g->dx * g->dy / rp->ppc;
-            sim->rp.weight = sim->w_uniform;
-        } else {
-            AW_REQUIRE(sp && nspecies > 0, "at least one species is required");
-            sim->species.assign(sp, sp + nspecies);
-            int64_t total = 0;
-            double w0 = -1.0;
-            bool uniform = true;
-            for (int k = 0; k < nspecies; ++k) {
-                AW_REQUIRE(sp[k].ppc > 0, "species ppc must be positive");
-                total += (int64_t)sp[k].ppc * nc;
-                const double wk = sp[k].density * g->dx * g->dy / sp[k].ppc;
-                if (w0 < 0) w0 = wk;
-                else if (wk != w0) uniform = false;
-            }
-            sim->n = total;
-            sim->w_uniform = w0;
-            sim->per_w = !uniform;
-        }
-        AW_REQUIRE(sim->n < (int64_t)INT32_MAX, "particle count must fit int32");
-        AW_HIP(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
-        // default ES tile: 8x8 at >= 64 particles per cell (more, smaller
-        // workgroups; the apron overhead stays small), else 16x16
-        int tx = cfg->tile_x, ty = cfg->tile_y;
-        if (tx <= 0 && ty <= 0 && cfg->field_model != AW_MODEL_DARWIN)
-            tx = ty = sim->n >= 64 * nc ? 8 : 16;
-        sim->ctx.reset(make_ctx(&sim->g, tx, ty));
-        sim->ctx->fix_log2 = cfg->fixed_scale > 0
-                                 ? cfg->fixed_scale
-                                 : fixed_scale_for(sim->species, *rp, cfg->legacy_load != 0);
-        const size_t n = (size_t)std::max<int64_t>(1, sim->n);
-        for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < (sim->per_w ? 6 : 5); ++a) sim->P[b][a].resize(n);
-        sim->tile_off.resize((size_t)sim->ctx->ntiles + 1);
-        sim->perm.resize(n);
-        sim->rho.resize((size_t)nc);
-        sim->E.resize((size_t)2 * nc);
-        sim->strays.resize(1);
-        sim->bpt = cfg->blocks_per_tile > 0 ? cfg->blocks_per_tile : tile_bpt(*sim->ctx, sim->n);
-        sim->ke_part.resize((size_t)sim->ctx->ntiles * sim->bpt + 1);
-        sim->grid_part.resize((size_t)3 * 512 + 3);
-        sim->diag_out.resize((size_t)4 * sim->diag_cap);
-        if (cfg->field_model == AW_MODEL_DARWIN) sim->setup_em();
-        AW_HIP(hipHostMalloc((void**)&sim->stray_host, sizeof(unsigned long long),
-                             hipHostMallocDefault));
-        AW_HIP(hipEventCreateWithFlags(&sim->stray_ev, hipEventDisableTiming));
-        *out = sim.release();
+        AW_REQUIRE(out, "out is NULL");
+        *out = finish_sim(make_sim(g, rp, sp, nspecies, cfg), 1).release();
     });
 }
 
@@ -1162,9 +1274,9 @@ aw_status aw_sim_history(aw_sim* sim, aw_diag_sample* out, int64_t cap, int64_t*
     return guard("aw_sim_history", [&] {
         AW_REQUIRE(sim && count, "NULL argument");
         sim->flush_diag();
-        const int64_t nh = (int64_t)sim->history.size();
+        const int64_t nh = (int64_t)sim->hist[0].size();
         if (out)
-            for (int64_t k = 0; k < std::min(cap, nh); ++k) out[k] = sim->history[k];
+            for (int64_t k = 0; k < std::min(cap, nh); ++k) out[k] = sim->hist[0][k];
         *count = nh;
     });
 }
@@ -1180,7 +1292,7 @@ aw_status aw_sim_diag_now(aw_sim* sim, int64_t step, aw_diag_sample* out) {
         double* o = part + reduce_out_offset();
         const int ng = grid_partials(sim->Ex(), sim->Ey(), sim->rho_ptr(), ncell(c), gp,
                                      sim->stream);
-        finalize_diag(part, nke, p.w ? 1.0 : p.w_uniform, gp, ng, o, sim->stream);
+        finalize_diag(part, nke, p.w ? 1.0 : p.w_uniform, gp, ng, o, sim->stream, nullptr);
         double h[4];
         AW_HIP(hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, sim->stream));
         AW_HIP(hipStreamSynchronize(sim->stream));
@@ -1234,3 +1346,108 @@ aw_status aw_sim_exec_config(aw_sim* sim, int32_t* tile_x, int32_t* tile_y,
 
 }  // extern "C"
 
+// ============================================================================
+// Darwin ensembles: one aw_sim stepping a uniform batch of runs
+// ============================================================================
+
+namespace aw {
+
+// Runs of one Darwin batch share the grid, the species layout (count, ppc,
+// density: equal weights) and every step parameter; they differ in their
+// loads (thermal and drift velocities, seed, mode amplitudes).
+aw_sim* darwin_batch_create(const aw_run_desc* runs, int nruns, const aw_sim_config* cfg) {
+    AW_REQUIRE(runs && cfg, "NULL argument");
+    AW_REQUIRE(nruns >= 1 && nruns <= 64, "a Darwin ensemble holds 1..64 runs");
+    AW_REQUIRE(cfg->field_model == AW_MODEL_DARWIN, "not a Darwin configuration");
+    AW_REQUIRE(!cfg->legacy_load, "Darwin ensembles load species (legacy_load = 0)");
+    const aw_run_desc& r0 = runs[0];
+    for (int k = 0; k < nruns; ++k) {
+        const aw_run_desc& r = runs[k];
+        AW_REQUIRE(r.species && r.nspecies > 0, "every run needs species");
+        AW_REQUIRE(r.n_modes >= 0 && r.n_modes <= 32, "n_modes must be in [0, 32]");
+        AW_REQUIRE(r.grid.nx == r0.grid.nx && r.grid.ny == r0.grid.ny &&
+                       r.grid.dx == r0.grid.dx && r.grid.dy == r0.grid.dy,
+                   "Darwin ensemble runs must share the grid");
+        AW_REQUIRE(r.nspecies == r0.nspecies && r.has_b0 == r0.has_b0,
+                   "Darwin ensemble runs must share the species layout and has_b0");
+        for (int q = 0; q < r.nspecies; ++q)
+            AW_REQUIRE(r.species[q].ppc == r0.species[q].ppc &&
+                           r.species[q].density == r0.species[q].density,
+                       "Darwin ensemble runs must share species ppc and density");
+        const aw_run_params &a = r.rp, &b = r0.rp;
+        AW_REQUIRE(a.dt == b.dt && a.qm == b.qm && a.eps0 == b.eps0 && a.c == b.c &&
+                       a.n0 == b.n0 && a.B0[0] == b.B0[0] && a.B0[1] == b.B0[1] &&
+                       a.B0[2] == b.B0[2] && a.pump == b.pump && a.pump_ex == b.pump_ex &&
+                       a.pump_ey == b.pump_ey && a.pump_ez == b.pump_ez &&
+                       a.pump_k0 == b.pump_k0 && a.pump_w0 == b.pump_w0 &&
+                       a.pump_trmp == b.pump_trmp && a.pump_toff == b.pump_toff &&
+                       a.dump_every == b.dump_every,
+                   "Darwin ensemble runs must share dt, qm, eps0, c, n0, B0, the pump and "
+                   "dump_every");
+    }
+    aw_sim_config c = *cfg;
+    c.has_b0 = r0.has_b0;
+    c.n_modes = 0;
+    auto sim = make_sim(&r0.grid, &r0.rp, r0.species, r0.nspecies, &c);
+    AW_REQUIRE((int64_t)sim->n * nruns < (int64_t)INT32_MAX,
+               "ensemble particle count must fit int32");
+    sim->rspecies.resize(nruns);
+    sim->rrp.resize(nruns);
+    sim->rmodes.resize(nruns);
+    for (int k = 0; k < nruns; ++k) {
+        sim->rspecies[k].assign(runs[k].species, runs[k].species + runs[k].nspecies);
+        sim->rrp[k] = runs[k].rp;
+        sim->rmodes[k].assign(runs[k].mode_amp, runs[k].mode_amp + runs[k].n_modes);
+    }
+    return finish_sim(std::move(sim), nruns).release();
+}
+
+void darwin_batch_destroy(aw_sim* s) noexcept { delete s; }
+void darwin_batch_init(aw_sim* s) { s->init(); }
+void darwin_batch_step(aw_sim* s, int64_t n) { s->step(n); }
+void darwin_batch_sync(aw_sim* s) { AW_HIP(hipStreamSynchronize(s->stream)); }
+
+void darwin_batch_info(aw_sim* s, aw_ensemble_stats* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->nruns = s->runs;
+    out->ngroups = 1;
+    out->particles = s->n * s->runs;
+    out->cells = ncell(*s->ctx) * s->runs;
+    out->workgroups = s->ctx->ntiles * s->bpt * s->runs;
+    out->sorts = s->nsorts;
+}
+
+static int check_run(const aw_sim* s, int run) {
+    AW_REQUIRE(run >= 0 && run < s->runs, "run index out of range");
+    return run;
+}
+
+aw_particles darwin_batch_particles(const aw_sim* s, int run) {
+    return s->particles(s->cur, check_run(s, run));
+}
+
+aw_fields darwin_batch_fields(const aw_sim* s, int run) {
+    const size_t nc = (size_t)ncell(*s->ctx), r = (size_t)check_run(s, run);
+    aw_fields f{};
+    f.Ex = s->e3.get() + r * 3 * nc;
+    f.Ey = f.Ex + nc;
+    f.Ez = f.Ex + 2 * nc;
+    f.ELx = s->elb.get() + r * 5 * nc;
+    f.ELy = f.ELx + nc;
+    f.Bx = f.ELx + 2 * nc;
+    f.By = f.ELx + 3 * nc;
+    f.Bz = f.ELx + 4 * nc;
+    f.ETx = s->et.get() + r * 3 * nc;
+    f.ETy = f.ETx + nc;
+    f.ETz = f.ETx + 2 * nc;
+    return f;
+}
+
+float* darwin_batch_rho(const aw_sim* s, int run) { return s->rho_ptr(check_run(s, run)); }
+
+const std::vector<aw_diag_sample>& darwin_batch_history(aw_sim* s, int run) {
+    s->flush_diag();
+    return s->hist[check_run(s, run)];
+}
+
+}  // namespace aw

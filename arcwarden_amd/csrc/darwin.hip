@@ -48,7 +48,34 @@ struct EmArgs {
     double dxdt, dydt;           // dx/dt, dy/dt (Esirkepov current coefficients)
     unsigned long long* strays;
     double* ke_part;
+    // uniform batch (EmBatch): run blockIdx.y offsets its pointers by these
+    long long bp, be, bb, ba, bk;
+    int bo;
 };
+
+// Moves every per-run pointer of a to run blockIdx.y of a uniform batch.
+__device__ __forceinline__ void select_run(EmArgs& a) {
+    const long long r = blockIdx.y;
+    if (r == 0) return;
+    a.x += r * a.bp;
+    a.y += r * a.bp;
+    a.ux += r * a.bp;
+    a.uy += r * a.bp;
+    a.uz += r * a.bp;
+    if (a.w) a.w += r * a.bp;
+    a.off += r * a.bo;
+    if (a.Ex) {
+        a.Ex += r * a.be;
+        a.Ey += r * a.be;
+        a.Ez += r * a.be;
+        a.Bx += r * a.bb;
+        a.By += r * a.bb;
+        a.Bz += r * a.bb;
+    }
+    if (a.acc) a.acc += r * a.ba;
+    if (a.c_dcu) a.c_dcu += r;
+    if (a.ke_part) a.ke_part += r * a.bk;
+}
 
 // the 8 moment values of one particle (already scaled); cq = a.cq
 __device__ __forceinline__ void moments8(const float cq[3], float mw, float ux, float uy, float uz,
@@ -65,12 +92,16 @@ __device__ __forceinline__ void moments8(const float cq[3], float mw, float ux, 
 }
 
 // Deposits v[q] for the q < NQ selected by MASK (grid q of the tile / global
-// accumulators).
+// accumulators): 4 CIC contributions rint(v[q] * W_k) per moment, with the
+// FP64 weights of cic_d (exact products) and one fma per contribution.
 template <int NQ, unsigned MASK = (1u << NQ) - 1u>
 __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int W, int H,
                                           unsigned long long* g_acc, int ncell, int nx, int ny,
-                                          int orgx, int orgy, const dev::CicF& d,
-                                          const float* v, unsigned int& nstray) {
+                                          int orgx, int orgy, const dev::CicD& d,
+                                          const float* v_in, unsigned int& nstray) {
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = dev::clamp_fix_f64((double)v_in[q]);
     const int mi = local_index(d.i0, orgx, nx), mj = local_index(d.j0, orgy, ny);
     if (dev::in_tile(mi, W - 2) && dev::in_tile(mj, H - 2)) {
         const int b = mj * W + mi;
@@ -78,10 +109,10 @@ __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int 
         for (int q = 0; q < NQ; ++q) {
             if (!((MASK >> q) & 1u)) continue;
             unsigned long long* s = s_acc + q * R;
-            atomicAdd(&s[b], dev::fixq32(v[q], d.w0));
-            atomicAdd(&s[b + 1], dev::fixq32(v[q], d.w1));
-            atomicAdd(&s[b + W], dev::fixq32(v[q], d.w2));
-            atomicAdd(&s[b + W + 1], dev::fixq32(v[q], d.w3));
+            atomicAdd(&s[b], dev::fix_mul(v[q], d.w0));
+            atomicAdd(&s[b + 1], dev::fix_mul(v[q], d.w1));
+            atomicAdd(&s[b + W], dev::fix_mul(v[q], d.w2));
+            atomicAdd(&s[b + W + 1], dev::fix_mul(v[q], d.w3));
         }
     } else {
         int i0, i1, j0, j1;
@@ -91,10 +122,10 @@ __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int 
         for (int q = 0; q < NQ; ++q) {
             if (!((MASK >> q) & 1u)) continue;
             unsigned long long* g = g_acc + (size_t)q * ncell;
-            atomicAdd(&g[j0 * nx + i0], dev::fixq32(v[q], d.w0));
-            atomicAdd(&g[j0 * nx + i1], dev::fixq32(v[q], d.w1));
-            atomicAdd(&g[j1 * nx + i0], dev::fixq32(v[q], d.w2));
-            atomicAdd(&g[j1 * nx + i1], dev::fixq32(v[q], d.w3));
+            atomicAdd(&g[j0 * nx + i0], dev::fix_mul(v[q], d.w0));
+            atomicAdd(&g[j0 * nx + i1], dev::fix_mul(v[q], d.w1));
+            atomicAdd(&g[j1 * nx + i0], dev::fix_mul(v[q], d.w2));
+            atomicAdd(&g[j1 * nx + i1], dev::fix_mul(v[q], d.w3));
         }
         ++nstray;
     }
@@ -205,6 +236,7 @@ __device__ __forceinline__ void esk_deposit(const EmArgs& a, const Frame& f,
 
 template <bool PER_W>
 __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
+    select_run(a);
     extern __shared__ unsigned long long smem[];
     const Frame f = frame(a);
     for (int c = threadIdx.x; c < 8 * f.R; c += kThreads) smem[c] = 0ull;
@@ -217,7 +249,7 @@ __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
         float v[8];
         moments8(cq, mw, a.ux[k], a.uy[k], a.uz[k], v);
         deposit_q<8>(smem, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
-                     dev::cic_f(a.x[k], a.y[k]), v, nstray);
+                     dev::cic_d(a.x[k], a.y[k]), v, nstray);
     }
     __syncthreads();
     flush_q<8>(smem, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
@@ -226,6 +258,7 @@ __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
 
 template <bool PER_W>
 __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
+    select_run(a);
     extern __shared__ unsigned long long smem[];
     const Frame f = frame(a);
     unsigned long long* s_acc = smem;
@@ -246,8 +279,8 @@ __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
         v[0] = base * (e[0] + (vy * b[2] - vz * b[1]));
         v[1] = base * (e[1] + (vz * b[0] - vx * b[2]));
         v[2] = base * (e[2] + (vx * b[1] - vy * b[0]));
-        deposit_q<3>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy, s, v,
-                     nstray);
+        deposit_q<3>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
+                     dev::cic_d(x, y), v, nstray);
     };
     int r = f.r0 + (int)threadIdx.x;
     for (; r + kThreads < f.r1; r += 2 * kThreads) {
@@ -293,11 +326,11 @@ __device__ __forceinline__ void em_one(const EmArgs& a, const Frame& f, const fl
         moments8(cq, mw, ux, uy, uz, v);
         if constexpr (ESK) {
             deposit_q<8, 0xF1u>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx,
-                                f.orgy, dev::cic_f(xn, yn), v, nstray);
+                                f.orgy, dev::cic_d(xn, yn), v, nstray);
             esk_deposit(a, f, s_acc, x, y, xn, yn, uz, (double)cq[1] * (double)mw);
         } else {
             deposit_q<8>(s_acc, f.R, f.W, f.H, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy,
-                         dev::cic_f(xn, yn), v, nstray);
+                         dev::cic_d(xn, yn), v, nstray);
         }
     }
 }
@@ -309,6 +342,7 @@ __device__ __forceinline__ void em_one(const EmArgs& a, const Frame& f, const fl
 // per thread per iteration, both loaded before either is pushed.
 template <bool PER_W, bool KE, bool DEP, bool ESK = false>
 __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
+    select_run(a);
     extern __shared__ unsigned long long smem[];
     const Frame f = frame(a);
     constexpr int kNq = DEP ? 8 : 0;
@@ -364,11 +398,19 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
 // from device memory (the dcu scale changes every step).  out2 receives a
 // second copy: rocFFT may overwrite the input of an out-of-place R2C, so the
 // transform reads the copy and the source grids stay intact.
+// Batched (gridDim.y runs): run r uses acc + r*sa, out + r*so, out2 + r*so2,
+// inv + r*si.
 __global__ void convert_multi_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
                                      float* __restrict__ out2, int ncell, int nq,
-                                     const double* __restrict__ inv) {
+                                     const double* __restrict__ inv, long long sa, long long so,
+                                     long long so2, int si) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
+    const long long r = blockIdx.y;
+    acc += r * sa;
+    out += r * so;
+    if (out2) out2 += r * so2;
+    inv += r * si;
     for (int q = 0; q < nq; ++q) {
         const size_t i = (size_t)q * ncell + c;
         const float v = (float)((double)(long long)acc[i] * inv[q]);
@@ -395,6 +437,14 @@ struct PumpArgs {
 __global__ void form_e_kernel(FormE f, int nx, int ncell, PumpArgs p) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
+    if (const long long r = blockIdx.y) {  // run r of a batch
+        for (int q = 0; q < 3; ++q) {
+            f.E[q] += r * f.sE;
+            if (f.ET[q]) f.ET[q] += r * f.sT;
+        }
+        f.EL[0] += r * f.sL;
+        f.EL[1] += r * f.sL;
+    }
     float ex = f.EL[0][c], ey = f.EL[1][c], ez = 0.0f;
     if (f.ET[0]) {
         ex = ex + 1.0f * f.ET[0][c];
@@ -421,8 +471,15 @@ struct Field6 {
 
 // block maxima of |E|, |B| components -> part[blockIdx]
 __global__ void __launch_bounds__(kThreads)
-field_max_kernel(Field6 F, int ncell, float* __restrict__ part) {
+field_max_kernel(Field6 F, int ncell, float* __restrict__ part, long long sE, long long sB) {
     __shared__ float sh[2][kThreads];
+    if (const long long r = blockIdx.y) {  // run r of a batch
+        for (int q = 0; q < 3; ++q) {
+            F.f[q] += r * sE;
+            F.f[3 + q] += r * sB;
+        }
+        part += r * 2 * gridDim.x;
+    }
     float me = 0.f, mb = 0.f;
     for (int c = blockIdx.x * kThreads + threadIdx.x; c < ncell; c += gridDim.x * kThreads) {
         for (int q = 0; q < 3; ++q) {
@@ -450,10 +507,17 @@ field_max_kernel(Field6 F, int ncell, float* __restrict__ part) {
 // S = 48 - ceil(log2(bound)) maps the bound to 2^48.  Writes the float
 // coefficient used by dcu_kernel and the inverse scale used by conversion.
 // dev (optional, device): {coef_abs, umax} measured from the particles.
+// Batched: one thread per run (threadIdx.x), run r reads part + 2*nb*r and
+// writes c_dcu[r], inv_dcu + si*r.
 __global__ void dcu_scale_kernel(const float* __restrict__ part, int nb, double coef_abs,
                                  double coef_signed, double umax, const double* __restrict__ dev,
-                                 float* __restrict__ c_dcu, double* __restrict__ inv_dcu) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+                                 float* __restrict__ c_dcu, double* __restrict__ inv_dcu, int runs,
+                                 int si) {
+    const int r = threadIdx.x;
+    if (r >= runs || blockIdx.x != 0) return;
+    part += (size_t)2 * nb * r;
+    c_dcu += r;
+    inv_dcu += (size_t)si * r;
     if (dev) {
         coef_abs = dev[0];
         umax = dev[1];
@@ -590,28 +654,43 @@ static EmArgs em_args(aw_ctx& c, const aw_particles& p, const int32_t* off, cons
     return a;
 }
 
+static void set_batch(EmArgs& a, const EmBatch& b) {
+    a.bp = b.pstride;
+    a.be = b.estride;
+    a.bb = b.bstride;
+    a.ba = b.astride;
+    a.bk = b.kstride;
+    a.bo = b.ostride;
+}
+
+// Batched forms (b.runs > 1): p describes run 0 (p.n particles per run);
+// runs 1.. lie at the strides of b.
 void em_moments(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
                 unsigned long long* acc8, const float* cq, int bpt, unsigned long long* strays,
-                hipStream_t s) {
+                hipStream_t s, const EmBatch& b) {
     if (p.n == 0) return;
     EmArgs a = em_args(c, p, off, idx, nullptr, acc8, cq, nullptr, 0.0, 0.0, bpt, strays,
                        nullptr);
+    set_batch(a, b);
     const size_t sh = (size_t)a.W * a.H * 8 * 8;
-    const int nblk = c.ntiles * bpt;
-    if (p.w) em_moments_kernel<true><<<nblk, kThreads, sh, s>>>(a);
-    else em_moments_kernel<false><<<nblk, kThreads, sh, s>>>(a);
+    AW_REQUIRE(sh <= c.lds_per_block, "tile too large for the LDS of the moment deposit");
+    const dim3 grid(c.ntiles * bpt, b.runs);
+    if (p.w) em_moments_kernel<true><<<grid, kThreads, sh, s>>>(a);
+    else em_moments_kernel<false><<<grid, kThreads, sh, s>>>(a);
     AW_LAUNCH_CHECK();
 }
 
 void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc3, const float* c_dcu, int bpt,
-            unsigned long long* strays, hipStream_t s) {
+            unsigned long long* strays, hipStream_t s, const EmBatch& b) {
     if (p.n == 0) return;
     EmArgs a = em_args(c, p, off, idx, F, acc3, nullptr, c_dcu, 0.0, 0.0, bpt, strays, nullptr);
+    set_batch(a, b);
     const size_t sh = (size_t)a.W * a.H * (3 * 8 + 16 + 8);
-    const int nblk = c.ntiles * bpt;
-    if (p.w) dcu_kernel<true><<<nblk, kThreads, sh, s>>>(a);
-    else dcu_kernel<false><<<nblk, kThreads, sh, s>>>(a);
+    AW_REQUIRE(sh <= c.lds_per_block, "tile too large for the LDS of the dcu deposit");
+    const dim3 grid(c.ntiles * bpt, b.runs);
+    if (p.w) dcu_kernel<true><<<grid, kThreads, sh, s>>>(a);
+    else dcu_kernel<false><<<grid, kThreads, sh, s>>>(a);
     AW_LAUNCH_CHECK();
 }
 
@@ -620,15 +699,17 @@ void em_dcu(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
 int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t* idx,
             const float* const* F, unsigned long long* acc8, const float* cq, double qm,
             double dt, int bpt, unsigned long long* strays, double* ke_part, hipStream_t s,
-            bool esk) {
+            bool esk, const EmBatch& b) {
     if (p.n == 0) return 0;
     EmArgs a = em_args(c, p, off, idx, F, acc8, cq, nullptr, qm, dt, bpt, strays, ke_part);
+    set_batch(a, b);
     const bool dep = acc8 != nullptr;
     const size_t sh = (size_t)a.W * a.H * ((dep ? 8 * 8 : 0) + 16 + 8);
+    AW_REQUIRE(sh <= c.lds_per_block, "tile too large for the LDS of the EM push");
     const int nblk = c.ntiles * bpt;
     const bool pw = p.w != nullptr, ke = ke_part != nullptr;
     auto launch = [&](auto kernel) {
-        kernel<<<nblk, kThreads, sh, s>>>(a);
+        kernel<<<dim3(nblk, b.runs), kThreads, sh, s>>>(a);
         AW_LAUNCH_CHECK();
     };
     if (dep && esk) {
@@ -650,15 +731,19 @@ int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
     return nblk;
 }
 
+// Batched (runs > 1): run r converts acc + r*sa into out + r*so (and
+// out2 + r*so2) with the scales at inv + r*si.
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
-                   const double* inv, hipStream_t s) {
-    convert_multi_kernel<<<blocks_for(ncell), kThreads, 0, s>>>(acc, out, out2, (int)ncell, nq,
-                                                                inv);
+                   const double* inv, hipStream_t s, int runs, long long sa, long long so,
+                   long long so2, int si) {
+    convert_multi_kernel<<<dim3(blocks_for(ncell), runs), kThreads, 0, s>>>(
+        acc, out, out2, (int)ncell, nq, inv, sa, so, so2, si);
     AW_LAUNCH_CHECK();
 }
 
-void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t,
-            hipStream_t s) {
+// Batched (runs > 1): f carries run 0's pointers and the per-run strides.
+void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t, hipStream_t s,
+            int runs) {
     PumpArgs p{};
     p.on = rp.pump ? 1 : 0;
     p.ex = rp.pump_ex;
@@ -671,20 +756,27 @@ void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t,
     p.t = t;
     p.dx = g.dx;
     const int64_t nc = (int64_t)g.nx * g.ny;
-    form_e_kernel<<<blocks_for(nc), kThreads, 0, s>>>(f, g.nx, (int)nc, p);
+    form_e_kernel<<<dim3(blocks_for(nc), runs), kThreads, 0, s>>>(f, g.nx, (int)nc, p);
     AW_LAUNCH_CHECK();
 }
 
+// Batched (runs > 1): F holds run 0's fields, E components at stride sE and
+// B at stride sB per run; part holds dcu_part_capacity(ncell) floats per run,
+// c_dcu one float and inv_dcu si doubles per run.
 void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_abs,
                double coef_signed, double umax, const double* dev, float* c_dcu, double* inv_dcu,
-               hipStream_t s) {
+               hipStream_t s, int runs, long long sE, long long sB, int si) {
+    AW_REQUIRE(runs >= 1 && runs <= 64, "dcu scale: 1..64 runs");
     const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (ncell + 4095) / 4096));
     Field6 f6{{F[0], F[1], F[2], F[3], F[4], F[5]}};
-    field_max_kernel<<<nb, kThreads, 0, s>>>(f6, (int)ncell, part);
+    field_max_kernel<<<dim3(nb, runs), kThreads, 0, s>>>(f6, (int)ncell, part, sE, sB);
     AW_LAUNCH_CHECK();
-    dcu_scale_kernel<<<1, 64, 0, s>>>(part, nb, coef_abs, coef_signed, umax, dev, c_dcu, inv_dcu);
+    dcu_scale_kernel<<<1, 64, 0, s>>>(part, nb, coef_abs, coef_signed, umax, dev, c_dcu, inv_dcu,
+                                      runs, si);
     AW_LAUNCH_CHECK();
 }
+
+int dcu_part_capacity() { return 2 * 256; }
 
 // ---- standalone operators (any particle order) ------------------------------------------
 //
@@ -729,7 +821,7 @@ void deposit_rho_j_amu(aw_ctx& c, const aw_particles& p, double qm, float* const
     bin_particles(c, p, s);
     measure_scales(c, p, qm, s);
     em_moments(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), e.acc.get(), e.cq.get(),
-               tile_bpt(c, p.n), nullptr, s);
+               tile_bpt(c, p.n), nullptr, s, EmBatch{});
     Out8 o{};
     for (int q = 0; q < 8; ++q) o.o[q] = out[q];
     convert_add_kernel<<<blocks_for(nc), kThreads, 0, s>>>(e.acc.get(), o, (int)nc, 8, e.inv.get());
@@ -750,9 +842,9 @@ void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const
     measure_scales(c, p, qm, s);
     const double mult = p.w ? 1.0 : p.w_uniform;
     dcu_scale(F, nc, e.fpart.get(), 0.0, qm * qm * mult / (c.g.dx * c.g.dy), 0.0, e.dv.get(),
-              e.cdcu.get(), e.inv.get() + 8, s);
+              e.cdcu.get(), e.inv.get() + 8, s, 1, 0, 0, 0);
     em_dcu(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), F, e.acc.get(), e.cdcu.get(),
-           tile_bpt(c, p.n), nullptr, s);
+           tile_bpt(c, p.n), nullptr, s, EmBatch{});
     Out8 o{};
     for (int q = 0; q < 3; ++q) o.o[q] = out[q];
     convert_add_kernel<<<blocks_for(nc), kThreads, 0, s>>>(e.acc.get(), o, (int)nc, 3,
@@ -766,7 +858,7 @@ void push_boris_em(aw_ctx& c, const aw_particles& p, double qm, double dt, const
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
     bin_particles(c, p, s);
     em_push(c, p, c.sort.bin_off.get(), c.sort.bin_perm.get(), F, nullptr, nullptr, qm, dt,
-            tile_bpt(c, p.n), nullptr, nullptr, s, false);
+            tile_bpt(c, p.n), nullptr, nullptr, s, false, EmBatch{});
 }
 
 }  // namespace aw

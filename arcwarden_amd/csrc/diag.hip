@@ -12,6 +12,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 512;
+constexpr int kChunk = 1024;  // partials per workgroup of the first finalize stage
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -74,6 +75,17 @@ grid_kernel(const float* __restrict__ Ex, const float* __restrict__ Ey,
     }
 }
 
+// out[c] = sum of in[c*kChunk .. (c+1)*kChunk) in a fixed order.
+__global__ void __launch_bounds__(kThreads)
+chunk_sum_kernel(const double* __restrict__ in, int n, double* __restrict__ out) {
+    __shared__ double sh[kThreads / 64];
+    const int b = blockIdx.x * kChunk, e = min(n, b + kChunk);
+    double a = 0.0;
+    for (int i = b + threadIdx.x; i < e; i += kThreads) a += in[i];
+    const double t = block_sum(a, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 // out[0] = sum(w u^2) (times w_scale), out[1] = sum |E|^2, out[2] = max |E|^2,
 // out[3] = sum rho.
 __global__ void __launch_bounds__(kThreads)
@@ -117,11 +129,23 @@ int grid_partials(const float* Ex, const float* Ey, const float* rho, int64_t nc
     return nb;
 }
 
+// stage (may be NULL) holds ke_stage_capacity(nke) - nke doubles: with it, a
+// long list of KE partials (one per fused workgroup) is first summed in
+// chunks by many workgroups, so the single-workgroup pass stays short.
 void finalize_diag(const double* kep, int nke, double w_scale, const double* gp, int ng,
-                   double* out4, hipStream_t s) {
+                   double* out4, hipStream_t s, double* stage) {
+    if (stage && nke > 2 * kChunk) {
+        const int nb = (nke + kChunk - 1) / kChunk;
+        chunk_sum_kernel<<<nb, kThreads, 0, s>>>(kep, nke, stage);
+        AW_LAUNCH_CHECK();
+        kep = stage;
+        nke = nb;
+    }
     finalize_kernel<<<1, kThreads, 0, s>>>(kep, nke, w_scale, gp, ng, out4);
     AW_LAUNCH_CHECK();
 }
+
+size_t ke_stage_capacity(size_t nke) { return nke + (nke + kChunk - 1) / kChunk + 1; }
 
 // layout of a reduction scratch buffer: [ke partials | grid partials | out4]
 int reduce_partial_capacity() { return 4 * kRedBlocks + 8; }

@@ -84,7 +84,26 @@ ens_diag_kernel(const DiagRun* __restrict__ runs, const double* __restrict__ ke_
 
 using namespace aw;
 
+namespace aw {
+// Darwin ensembles (capi.hip): a batched aw_sim
+aw_sim* darwin_batch_create(const aw_run_desc* runs, int nruns, const aw_sim_config* cfg);
+void darwin_batch_destroy(aw_sim* s) noexcept;
+void darwin_batch_init(aw_sim* s);
+void darwin_batch_step(aw_sim* s, int64_t n);
+void darwin_batch_sync(aw_sim* s);
+void darwin_batch_info(aw_sim* s, aw_ensemble_stats* out);
+aw_particles darwin_batch_particles(const aw_sim* s, int run);
+aw_fields darwin_batch_fields(const aw_sim* s, int run);
+float* darwin_batch_rho(const aw_sim* s, int run);
+const std::vector<aw_diag_sample>& darwin_batch_history(aw_sim* s, int run);
+}  // namespace aw
+
 struct aw_ensemble {
+    // the Darwin model: a uniform batch stepped by one aw_sim (every call
+    // below forwards to it); the members after it serve the ES model
+    aw_sim* dar = nullptr;
+    bool dar_initialized = false;
+
     struct Run {
         int user = 0;  // index in the caller's order
         aw_grid g{};
@@ -134,6 +153,7 @@ struct aw_ensemble {
     int stray_age = 0;  // steps since the pending stray read-back was issued
 
     ~aw_ensemble() {
+        if (dar) darwin_batch_destroy(dar);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& gx : graph)
             if (gx) (void)hipGraphExecDestroy(gx);
@@ -358,7 +378,13 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         *out = nullptr;
         auto e = std::make_unique<aw_ensemble>();
         if (cfg) e->cfg = *cfg;
-        AW_REQUIRE(e->cfg.field_model == AW_MODEL_ES, "ensembles run the electrostatic model");
+        if (e->cfg.field_model == AW_MODEL_DARWIN) {
+            e->dar = darwin_batch_create(descs, nruns, &e->cfg);
+            *out = e.release();
+            return;
+        }
+        AW_REQUIRE(e->cfg.field_model == AW_MODEL_ES,
+                   "field_model must be AW_MODEL_ES or AW_MODEL_DARWIN");
         // order runs by grid size so same-size runs form contiguous FFT batches
         std::vector<int> order(nruns);
         std::iota(order.begin(), order.end(), 0);
@@ -477,6 +503,10 @@ aw_status aw_ensemble_destroy(aw_ensemble* e) {
 aw_status aw_ensemble_set_graph(aw_ensemble* e, int32_t on) {
     return guard("aw_ensemble_set_graph", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
+        if (e->dar) {
+            AW_REQUIRE(!on, "Darwin ensembles step without HIP graphs");
+            return;
+        }
         AW_REQUIRE(!e->initialized, "set the graph mode before aw_ensemble_init");
         e->use_graph = on != 0;
     });
@@ -485,6 +515,11 @@ aw_status aw_ensemble_set_graph(aw_ensemble* e, int32_t on) {
 aw_status aw_ensemble_init(aw_ensemble* e) {
     return guard("aw_ensemble_init", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
+        if (e->dar) {
+            darwin_batch_init(e->dar);
+            e->dar_initialized = true;
+            return;
+        }
         e->init();
     });
 }
@@ -492,13 +527,19 @@ aw_status aw_ensemble_init(aw_ensemble* e) {
 aw_status aw_ensemble_step(aw_ensemble* e, int64_t n) {
     return guard("aw_ensemble_step", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
-        e->step(n);
+        if (e->dar) darwin_batch_step(e->dar, n);
+        else e->step(n);
     });
 }
 
 aw_status aw_ensemble_run(aw_ensemble* e, int64_t first, int64_t nsteps) {
     return guard("aw_ensemble_run", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
+        if (e->dar) {
+            for (int64_t k = 0; k < nsteps; ++k) darwin_batch_step(e->dar, first + k);
+            darwin_batch_sync(e->dar);
+            return;
+        }
         for (int64_t k = 0; k < nsteps; ++k) e->step(first + k);
         AW_HIP(hipStreamSynchronize(e->stream));
     });
@@ -507,13 +548,18 @@ aw_status aw_ensemble_run(aw_ensemble* e, int64_t first, int64_t nsteps) {
 aw_status aw_ensemble_synchronize(aw_ensemble* e) {
     return guard("aw_ensemble_synchronize", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
-        AW_HIP(hipStreamSynchronize(e->stream));
+        if (e->dar) darwin_batch_sync(e->dar);
+        else AW_HIP(hipStreamSynchronize(e->stream));
     });
 }
 
 aw_status aw_ensemble_info(aw_ensemble* e, aw_ensemble_stats* out) {
     return guard("aw_ensemble_info", [&] {
         AW_REQUIRE(e && out, "NULL argument");
+        if (e->dar) {
+            darwin_batch_info(e->dar, out);
+            return;
+        }
         out->nruns = (int32_t)e->runs.size();
         out->ngroups = (int32_t)e->groups.size();
         out->particles = e->ntot;
@@ -528,6 +574,10 @@ aw_status aw_ensemble_info(aw_ensemble* e, aw_ensemble_stats* out) {
 aw_status aw_ensemble_particles(aw_ensemble* e, int32_t run, aw_particles* out) {
     return guard("aw_ensemble_particles", [&] {
         AW_REQUIRE(e && out, "NULL argument");
+        if (e->dar) {
+            *out = darwin_batch_particles(e->dar, run);
+            return;
+        }
         *out = e->run_particles(e->by_user(run), e->cur);
     });
 }
@@ -535,6 +585,10 @@ aw_status aw_ensemble_particles(aw_ensemble* e, int32_t run, aw_particles* out) 
 aw_status aw_ensemble_fields(aw_ensemble* e, int32_t run, aw_fields* out) {
     return guard("aw_ensemble_fields", [&] {
         AW_REQUIRE(e && out, "NULL argument");
+        if (e->dar) {
+            *out = darwin_batch_fields(e->dar, run);
+            return;
+        }
         const auto& r = e->by_user(run);
         std::memset(out, 0, sizeof(*out));
         out->Ex = e->E.get() + 2 * r.cbase;
@@ -545,6 +599,10 @@ aw_status aw_ensemble_fields(aw_ensemble* e, int32_t run, aw_fields* out) {
 aw_status aw_ensemble_rho(aw_ensemble* e, int32_t run, float** out) {
     return guard("aw_ensemble_rho", [&] {
         AW_REQUIRE(e && out, "NULL argument");
+        if (e->dar) {
+            *out = darwin_batch_rho(e->dar, run);
+            return;
+        }
         *out = e->rho.get() + e->by_user(run).cbase;
     });
 }
@@ -553,8 +611,8 @@ aw_status aw_ensemble_history(aw_ensemble* e, int32_t run, aw_diag_sample* out, 
                               int64_t* count) {
     return guard("aw_ensemble_history", [&] {
         AW_REQUIRE(e && count, "NULL argument");
-        e->flush_diag();
-        const auto& h = e->by_user(run).history;
+        if (!e->dar) e->flush_diag();
+        const auto& h = e->dar ? darwin_batch_history(e->dar, run) : e->by_user(run).history;
         const int64_t nh = (int64_t)h.size();
         if (out)
             for (int64_t k = 0; k < std::min(cap, nh); ++k) out[k] = h[k];

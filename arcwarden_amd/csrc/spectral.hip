@@ -124,9 +124,29 @@ void enable_em_fft(aw_ctx& c) {
     if (work > c.fft_work.size()) c.fft_work.resize(work);
 }
 
+// Darwin batches of R runs: R2C x4R, C2R x5R, R2C x7R, C2R x3R.
+void enable_em_batch_fft(aw_ctx& c, int R) {
+    if (c.em_batch == R) return;
+    AW_REQUIRE(c.em_batch == 0, "Darwin batch plans already prepared for another size");
+    enable_em_fft(c);
+    const double inv = 1.0 / ((double)c.g.nx * (double)c.g.ny);
+    c.r2c4B = make_plan(c, true, 4 * R, 1.0);
+    c.c2r5B = make_plan(c, false, 5 * R, inv);
+    c.r2c7B = make_plan(c, true, 7 * R, 1.0);
+    c.c2r3B = make_plan(c, false, 3 * R, inv);
+    size_t work = c.fft_work.size();
+    for (rocfft_plan p : {c.r2c4B, c.c2r5B, c.r2c7B, c.c2r3B}) {
+        size_t w = 0;
+        AW_FFT(rocfft_plan_get_work_buffer_size(p, &w));
+        work = std::max(work, w);
+    }
+    if (work > c.fft_work.size()) c.fft_work.resize(work);
+    c.em_batch = R;
+}
+
 void destroy_fft(aw_ctx& c) noexcept {
     for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7, &c.c2r5,
-                           &c.r2cB, &c.c2rB})
+                           &c.r2cB, &c.c2rB, &c.r2c4B, &c.c2r5B, &c.r2c7B, &c.c2r3B})
         if (*p) {
             rocfft_plan_destroy(*p);
             *p = nullptr;
@@ -253,6 +273,8 @@ __global__ void darwin_elb_kernel(const float2* __restrict__ src, float2* __rest
     const int nk = kg.nkx * kg.ny;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nk) return;
+    src += (size_t)blockIdx.y * 4 * nk;  // run blockIdx.y of a batch
+    out += (size_t)blockIdx.y * 5 * nk;
     const int i = t % kg.nkx, j = t / kg.nkx;
     double kx, ky;
     kvec(kg, i, j, kx, ky);
@@ -309,6 +331,8 @@ __global__ void darwin_et_kernel(const float2* __restrict__ src, float2* __restr
     const int nk = kg.nkx * kg.ny;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nk) return;
+    src += (size_t)blockIdx.y * 7 * nk;  // run blockIdx.y of a batch
+    out += (size_t)blockIdx.y * 3 * nk;
     const int i = t % kg.nkx, j = t / kg.nkx;
     double kx, ky;
     kvec(kg, i, j, kx, ky);
@@ -343,35 +367,38 @@ __global__ void darwin_et_kernel(const float2* __restrict__ src, float2* __restr
 }  // namespace
 
 // src4 = [rho|Jx|Jy|Jz] (float, contiguous) -> out5 = [ELx|ELy|Bx|By|Bz];
-// stag: Jx, Jy staggered half a cell (Esirkepov current).
+// stag: Jx, Jy staggered half a cell (Esirkepov current).  runs > 1: a batch
+// of runs, src4 = [run][4 grids], out5 = [run][5 grids] (enable_em_batch_fft).
 void solve_darwin_elb(aw_ctx& c, const float* src4, float* out5, double eps0, double cl,
-                      const float B0[3], hipStream_t s, bool stag) {
+                      const float B0[3], hipStream_t s, bool stag, int runs) {
+    AW_REQUIRE(runs == 1 || runs == c.em_batch, "Darwin batch plans not prepared");
     const int64_t nk = nkmodes(c);
-    c.ktmp.reserve((size_t)2 * 12 * nk);
+    c.ktmp.reserve((size_t)2 * 12 * nk * runs);
     float2* k4 = reinterpret_cast<float2*>(c.ktmp.get());
-    float2* k5 = k4 + 4 * nk;
-    fft_exec(c, c.r2c4, const_cast<float*>(src4), k4, s);
+    float2* k5 = k4 + 4 * nk * runs;
+    fft_exec(c, runs == 1 ? c.r2c4 : c.r2c4B, const_cast<float*>(src4), k4, s);
     KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
              6.28318530717958647692 / c.g.Ly};
-    darwin_elb_kernel<<<blocks_for(nk), kThreads, 0, s>>>(k4, k5, kg, eps0, cl, B0[0], B0[1],
-                                                         B0[2], stag ? 1 : 0);
+    darwin_elb_kernel<<<dim3(blocks_for(nk), runs), kThreads, 0, s>>>(
+        k4, k5, kg, eps0, cl, B0[0], B0[1], B0[2], stag ? 1 : 0);
     AW_LAUNCH_CHECK();
-    fft_exec(c, c.c2r5, k5, out5, s);
+    fft_exec(c, runs == 1 ? c.c2r5 : c.c2r5B, k5, out5, s);
 }
 
-// src7 = [dcux|dcuy|dcuz|amu0..3] -> out3 = [ETx|ETy|ETz].
+// src7 = [dcux|dcuy|dcuz|amu0..3] -> out3 = [ETx|ETy|ETz]; runs > 1 as above.
 void solve_darwin_et(aw_ctx& c, const float* src7, float* out3, double eps0, double cl,
-                     double n0, hipStream_t s) {
+                     double n0, hipStream_t s, int runs) {
+    AW_REQUIRE(runs == 1 || runs == c.em_batch, "Darwin batch plans not prepared");
     const int64_t nk = nkmodes(c);
-    c.ktmp.reserve((size_t)2 * 12 * nk);
+    c.ktmp.reserve((size_t)2 * 12 * nk * runs);
     float2* k7 = reinterpret_cast<float2*>(c.ktmp.get());
-    float2* k3 = k7 + 7 * nk;
-    fft_exec(c, c.r2c7, const_cast<float*>(src7), k7, s);
+    float2* k3 = k7 + 7 * nk * runs;
+    fft_exec(c, runs == 1 ? c.r2c7 : c.r2c7B, const_cast<float*>(src7), k7, s);
     KGeom kg{c.g.nx, c.g.ny, c.g.nx / 2 + 1, 6.28318530717958647692 / c.g.Lx,
              6.28318530717958647692 / c.g.Ly};
-    darwin_et_kernel<<<blocks_for(nk), kThreads, 0, s>>>(k7, k3, kg, eps0, cl, n0);
+    darwin_et_kernel<<<dim3(blocks_for(nk), runs), kThreads, 0, s>>>(k7, k3, kg, eps0, cl, n0);
     AW_LAUNCH_CHECK();
-    fft_exec(c, c.c2r3, k3, out3, s);
+    fft_exec(c, runs == 1 ? c.c2r3 : c.c2r3B, k3, out3, s);
 }
 
 // General form with separate output buffers.

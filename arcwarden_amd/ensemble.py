@@ -1,7 +1,8 @@
-"""Batched electrostatic ensembles over aw_ensemble_* (include/aw.h), and the
-randomised ensemble workload: runs of a two-beam plasma whose grid side,
-particles per cell, thermal speed and beam speed are drawn from a seeded
-generator."""
+"""Batched ensembles over aw_ensemble_* (include/aw.h): electrostatic runs of
+any mix of grids, or a uniform batch of Darwin runs (SimConfig with
+field_model = AW_MODEL_DARWIN).  Also the randomised ES ensemble workload:
+runs of a two-beam plasma whose grid side, particles per cell, thermal speed
+and beam speed are drawn from a seeded generator."""
 from __future__ import annotations
 
 import ctypes as C
@@ -121,11 +122,13 @@ class Ensemble:
         return d2h(ptr.value, g.nx * g.ny).reshape(g.ny, g.nx)
 
     def fields(self, run: int) -> dict:
+        """Host copies of the run's field grids (ES: Ex, Ey; Darwin: all)."""
         f = L.AwFields()
         L.call("aw_ensemble_fields", self._h, run, C.byref(f))
         self.synchronize()
         g = self.runs[run].grid
-        return {k: d2h(getattr(f, k), g.nx * g.ny).reshape(g.ny, g.nx) for k in ("Ex", "Ey")}
+        return {k: d2h(getattr(f, k), g.nx * g.ny).reshape(g.ny, g.nx)
+                for k, _ in L.AwFields._fields_ if getattr(f, k)}
 
     def history(self, run: int) -> list[dict]:
         cnt = C.c_int64()

@@ -186,8 +186,9 @@ def whistler_run(vr_over_vth: float, nx: int = 512, ny: int = 128, ppc: int = 25
 def darwin_bench(steps: int, current: int):
     """The Darwin whistler-pump scan (v_r/v_th in {3.2, 2.1, 1.0}, 512x128,
     256 ppc = 16,777,216 electrons per run) stepped run after run by aw_sim
-    (spectral Darwin model with the given current deposit, re-sort on 2%
-    strays, diagnostics every 100 steps).  Loading is outside the timed
+    (spectral Darwin model with the given current deposit, autotuned tile
+    and workgroups per tile, re-sort on 2% strays, diagnostics every 100
+    steps).  Loading is outside the timed
     region.  The EM push kernel is timed with HIP events on the sim stream
     and priced at 40 B per particle (x, y, ux, uy, uz read and written)."""
     from arcwarden_amd import _lib as L
@@ -198,9 +199,11 @@ def darwin_bench(steps: int, current: int):
     for r in ratios:
         g, rp, sp = whistler_run(r)
         sim = Simulation(g, rp, sp, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN,
-                                              sort_every=0, current_deposit=current))
+                                              sort_every=0, current_deposit=current,
+                                              autotune=True))
         sim.init()
         n = sim.particle_views().n
+        exec_cfg = sim.exec_config()
         sim.synchronize()
         sim.set_profiling(True)
         t0 = time.perf_counter()
@@ -220,6 +223,24 @@ def darwin_bench(steps: int, current: int):
               file=sys.stderr, flush=True)
     avg_k = kms / 1e3 / max(kl, 1)
     achieved = 40.0 * n_last / avg_k / 1e9
+    # the same three runs as one batched Darwin ensemble (one launch per
+    # particle kernel and per transform for all runs)
+    from arcwarden_amd.ensemble import Ensemble, RunSpec
+    specs = []
+    for r in ratios:
+        g, rp, sp = whistler_run(r)
+        specs.append(RunSpec(g, rp, sp, has_b0=True))
+    ens = Ensemble(specs, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN, sort_every=0,
+                                    current_deposit=current, autotune=True))
+    ens.init()
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run(0, steps)
+    bat = time.perf_counter() - t0
+    ens_sorts = ens.info()["sorts"]
+    ens.close()
+    print(f"# darwin current={current} batched: {bat / steps * 1e3:.3f} ms/step (3 runs)",
+          file=sys.stderr, flush=True)
     return {"workload": "Darwin whistler pump, 3 runs v_r/v_th 3.2/2.1/1.0, 512x128, dx 0.5, "
                         "256 ppc (16,777,216 each), c 10, w_ce 0.5 at 30 deg, pump w0 0.15 "
                         "M=18 dB=1e-3 B0, dt 0.05, run after run",
@@ -228,7 +249,9 @@ def darwin_bench(steps: int, current: int):
             "ms_per_step": tot_s / (len(ratios) * steps) * 1e3, "sorts": sorts,
             "em_push": {"avg_kernel_ms": avg_k * 1e3, "bytes_per_particle": 40.0,
                         "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS},
-            "final_ee": ee}
+            "final_ee": ee, "exec": exec_cfg,
+            "batched": {"value": tot_ps / bat, "ms_per_step_all_runs": bat / steps * 1e3,
+                        "speedup_vs_run_after_run": tot_s / bat, "sorts": ens_sorts}}
 
 
 def policy_bench(nx: int, ppc: int, reps: int = 5):

@@ -398,10 +398,12 @@ typedef struct aw_sim_config {
     int32_t legacy_load;    /* 1: Particles::initialize(rp) path */
     int32_t fixed_scale;    /* 0 -> chosen from the species densities */
     int32_t blocks_per_tile; /* workgroups sharing one tile (0 -> heuristic) */
-    int32_t autotune;       /* 1 (ES model): at aw_sim_init, time the fused
-                               push+deposit for tile 8/16/32 x 1..16 workgroups
-                               per tile and keep the fastest (cached per grid
-                               and particle count for the process) */
+    int32_t autotune;       /* 1: at aw_sim_init, time one step's particle
+                               kernels (ES: the fused push+deposit; Darwin: the
+                               dcu deposit + EM push) for tile 8/16/32 x 1..16
+                               workgroups per tile and keep the fastest (cached
+                               per grid, particle count and kernel variant for
+                               the process); results are unchanged */
     int32_t current_deposit; /* Darwin model: AW_CURRENT_CIC (the reference) or
                                AW_CURRENT_ESIRKEPOV -- the next state's J is the
                                charge-conserving current of the push's move
@@ -475,7 +477,14 @@ typedef struct aw_ensemble_stats {
 
 /* cfg: tile_x (0 -> 8 at >= 64 particles per cell, else 16), blocks_per_tile,
  * sort_every (0: re-sort only when strays exceed stray_frac), stray_frac
- * (0 -> 0.02); the other members are ignored. */
+ * (0 -> 0.02); the other members are ignored.
+ * cfg->field_model = AW_MODEL_DARWIN builds a Darwin ensemble: a uniform
+ * batch whose runs share the grid, the species layout (count, ppc, density)
+ * and every step parameter (dt, qm, eps0, c, n0, B0, pump, dump_every) and
+ * differ in their loads (velocities, seed, mode amplitudes).  Each particle
+ * kernel and each batched rocFFT call spans all runs (blockIdx.y = run); cfg
+ * then also takes current_deposit and autotune, and graphs are not used.
+ * Results equal an aw_sim of the same run bit for bit. */
 aw_status aw_ensemble_create(const aw_run_desc* runs, int32_t nruns, const aw_sim_config* cfg,
                              aw_ensemble** out);
 aw_status aw_ensemble_destroy(aw_ensemble* e);
@@ -489,8 +498,8 @@ aw_status aw_ensemble_step(aw_ensemble* e, int64_t n);
 aw_status aw_ensemble_run(aw_ensemble* e, int64_t first, int64_t nsteps);
 aw_status aw_ensemble_synchronize(aw_ensemble* e);
 aw_status aw_ensemble_info(aw_ensemble* e, aw_ensemble_stats* out);
-/* Per run (index in the caller's order): its particles, Ex/Ey, rho^n and
- * DiagSample history (blocking). */
+/* Per run (index in the caller's order): its particles, fields (ES: Ex/Ey;
+ * Darwin: every aw_fields grid), rho^n and DiagSample history (blocking). */
 aw_status aw_ensemble_particles(aw_ensemble* e, int32_t run, aw_particles* out);
 aw_status aw_ensemble_fields(aw_ensemble* e, int32_t run, aw_fields* out);
 aw_status aw_ensemble_rho(aw_ensemble* e, int32_t run, float** out);

@@ -66,7 +66,7 @@ def test_darwin_et_closed_form(gpu):
 
 def darwin_sim(nx=64, ny=32, ppc=32, pump=True, sort_every=0, stray_frac=1.0, dump_every=1,
                c=5.0, B0=(0.4, 0.0, 0.23), seed=11, v0=0.3, uth=0.05,
-               current=L.AW_CURRENT_CIC):
+               current=L.AW_CURRENT_CIC, **exec_kw):
     g = L.grid(nx, ny, nx * 0.5, ny * 0.5)
     rp = L.run_params(dt=0.05, qm=-1.0, eps0=1.0, c=c, n0=1.0, noisy_load=1, rng_seed=seed,
                       B0=B0, dump_every=dump_every)
@@ -79,7 +79,7 @@ def darwin_sim(nx=64, ny=32, ppc=32, pump=True, sort_every=0, stray_frac=1.0, du
     sp = [L.species("beam_p", 0.5, ppc // 2, (uth,) * 3, (0, 0, v0)),
           L.species("beam_m", 0.5, ppc // 2, (uth,) * 3, (0, 0, -v0))]
     cfg = SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN, sort_every=sort_every,
-                    stray_frac=stray_frac, current_deposit=current)
+                    stray_frac=stray_frac, current_deposit=current, **exec_kw)
     return Simulation(g, rp, sp, cfg), g, rp
 
 
@@ -321,3 +321,41 @@ def test_esirkepov_current_needs_darwin(gpu):
     sp = [L.species("e", 1.0, 4, (0.05,) * 3, (0, 0, 0))]
     with pytest.raises(L.AwError):
         Simulation(g, rp, sp, SimConfig(current_deposit=L.AW_CURRENT_ESIRKEPOV))
+
+
+def _multiset(sim):
+    st = sim.particles()
+    cols = [st[k] for k in ("x", "y", "ux", "uy", "uz")]
+    order = np.lexsort(cols[::-1])
+    return np.stack([c[order] for c in cols])
+
+
+@pytest.mark.parametrize("tile,bpt,autotune,current", [
+    (8, 3, False, L.AW_CURRENT_CIC), (0, 0, True, L.AW_CURRENT_CIC),
+    (16, 5, False, L.AW_CURRENT_ESIRKEPOV), (0, 0, True, L.AW_CURRENT_ESIRKEPOV)])
+def test_darwin_exec_config_changes_no_result(gpu, tile, bpt, autotune, current):
+    """Tile size, workgroups per tile and the autotuner change only the
+    physical particle order of a Darwin run: every source and field grid is
+    bit-identical to the default configuration's, and so is the particle
+    multiset (deterministic fixed-point deposits)."""
+    a, g, rp = darwin_sim(sort_every=5, current=current)
+    b, _, _ = darwin_sim(sort_every=5, current=current, tile_x=tile, tile_y=tile,
+                         blocks_per_tile=bpt, autotune=autotune)
+    a.init()
+    b.init()
+    ec = b.exec_config()
+    assert ec["tile_x"] in (8, 16, 32) and ec["tile_y"] == ec["tile_x"]
+    if tile:
+        assert ec["tile_x"] == tile and ec["blocks_per_tile"] == bpt
+    for n in range(8):
+        a.step(n)
+        b.step(n)
+    sa, sb = a.sources_all(), b.sources_all()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    fa, fb = a.fields(), b.fields()
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+    assert np.array_equal(_multiset(a), _multiset(b))
+    a.close()
+    b.close()
