@@ -183,7 +183,7 @@ def whistler_run(vr_over_vth: float, nx: int = 512, ny: int = 128, ppc: int = 25
     return g, rp, sp
 
 
-def darwin_bench(steps: int, current: int):
+def darwin_bench(steps: int, current: int, autotune: bool = True, batch: bool = True):
     """The Darwin whistler-pump scan (v_r/v_th in {3.2, 2.1, 1.0}, 512x128,
     256 ppc = 16,777,216 electrons per run) stepped run after run by aw_sim
     (spectral Darwin model with the given current deposit, autotuned tile
@@ -200,7 +200,7 @@ def darwin_bench(steps: int, current: int):
         g, rp, sp = whistler_run(r)
         sim = Simulation(g, rp, sp, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN,
                                               sort_every=0, current_deposit=current,
-                                              autotune=True))
+                                              autotune=autotune))
         sim.init()
         n = sim.particle_views().n
         exec_cfg = sim.exec_config()
@@ -226,12 +226,23 @@ def darwin_bench(steps: int, current: int):
     # the same three runs as one batched Darwin ensemble (one launch per
     # particle kernel and per transform for all runs)
     from arcwarden_amd.ensemble import Ensemble, RunSpec
+    out = {"workload": "Darwin whistler pump, 3 runs v_r/v_th 3.2/2.1/1.0, 512x128, dx 0.5, "
+                       "256 ppc (16,777,216 each), c 10, w_ce 0.5 at 30 deg, pump w0 0.15 "
+                       "M=18 dB=1e-3 B0, dt 0.05, run after run",
+           "current_deposit": "esirkepov" if current == L.AW_CURRENT_ESIRKEPOV else "cic",
+           "steps": steps, "value": tot_ps / tot_s, "unit": "particle-steps/s",
+           "ms_per_step": tot_s / (len(ratios) * steps) * 1e3, "sorts": sorts,
+           "em_push": {"avg_kernel_ms": avg_k * 1e3, "bytes_per_particle": 40.0,
+                       "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS},
+           "final_ee": ee, "exec": exec_cfg}
+    if not batch:
+        return out
     specs = []
     for r in ratios:
         g, rp, sp = whistler_run(r)
         specs.append(RunSpec(g, rp, sp, has_b0=True))
     ens = Ensemble(specs, SimConfig(has_b0=True, field_model=L.AW_MODEL_DARWIN, sort_every=0,
-                                    current_deposit=current, autotune=True))
+                                    current_deposit=current, autotune=autotune))
     ens.init()
     ens.synchronize()
     t0 = time.perf_counter()
@@ -241,17 +252,9 @@ def darwin_bench(steps: int, current: int):
     ens.close()
     print(f"# darwin current={current} batched: {bat / steps * 1e3:.3f} ms/step (3 runs)",
           file=sys.stderr, flush=True)
-    return {"workload": "Darwin whistler pump, 3 runs v_r/v_th 3.2/2.1/1.0, 512x128, dx 0.5, "
-                        "256 ppc (16,777,216 each), c 10, w_ce 0.5 at 30 deg, pump w0 0.15 "
-                        "M=18 dB=1e-3 B0, dt 0.05, run after run",
-            "current_deposit": "esirkepov" if current == L.AW_CURRENT_ESIRKEPOV else "cic",
-            "steps": steps, "value": tot_ps / tot_s, "unit": "particle-steps/s",
-            "ms_per_step": tot_s / (len(ratios) * steps) * 1e3, "sorts": sorts,
-            "em_push": {"avg_kernel_ms": avg_k * 1e3, "bytes_per_particle": 40.0,
-                        "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS},
-            "final_ee": ee, "exec": exec_cfg,
-            "batched": {"value": tot_ps / bat, "ms_per_step_all_runs": bat / steps * 1e3,
-                        "speedup_vs_run_after_run": tot_s / bat, "sorts": ens_sorts}}
+    out["batched"] = {"value": tot_ps / bat, "ms_per_step_all_runs": bat / steps * 1e3,
+                      "speedup_vs_run_after_run": tot_s / bat, "sorts": ens_sorts}
+    return out
 
 
 def policy_bench(nx: int, ppc: int, reps: int = 5):
@@ -375,6 +378,10 @@ def main():
                     help="1: time the 128^2 two-stream adjoint (0: skip)")
     ap.add_argument("--darwin-steps", type=int, default=2000,
                     help="steps per run of the Darwin whistler-pump scan (0: skip)")
+    ap.add_argument("--darwin-autotune", type=int, default=1,
+                    help="1: autotune the Darwin runs' tile / workgroups per tile")
+    ap.add_argument("--no-darwin-batch", action="store_true",
+                    help="skip the batched Darwin ensemble of the three runs")
     ap.add_argument("--no-policies", action="store_true",
                     help="skip the deposit/push policy microbench")
     ap.add_argument("--no-autotune", action="store_true",
@@ -468,8 +475,9 @@ def main():
     dar = dar_cic = None
     if args.darwin_steps > 0:
         from arcwarden_amd import _lib as L
-        dar = darwin_bench(args.darwin_steps, L.AW_CURRENT_ESIRKEPOV)
-        dar_cic = darwin_bench(args.darwin_steps, L.AW_CURRENT_CIC)
+        kw = dict(autotune=bool(args.darwin_autotune), batch=not args.no_darwin_batch)
+        dar = darwin_bench(args.darwin_steps, L.AW_CURRENT_ESIRKEPOV, **kw)
+        dar_cic = darwin_bench(args.darwin_steps, L.AW_CURRENT_CIC, **kw)
     pol = None if args.no_policies else policy_bench(nx, args.ppc)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
