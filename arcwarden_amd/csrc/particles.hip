@@ -198,6 +198,7 @@ __global__ void key_offsets_kernel(const uint32_t* __restrict__ skeys, int n, ui
 // workgroups stay resident per CU to hide the memory latency.
 
 constexpr int kTsWave = 64;
+constexpr int kTsLoads = 4;  // particles per lane per histogram iteration
 
 // rounds of 64 particles per chunk: at least 16, more when the chunk x tile
 // count table would exceed 2^24 entries
@@ -224,18 +225,30 @@ ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, 
     __syncthreads();
     const int base = blockIdx.x * kTsWave * rounds;
     const int end = min(n, base + kTsWave * rounds);
-    for (int i0 = base; i0 < end; i0 += kTsWave) {
-        const int i = i0 + lane;
-        const bool valid = i < end;
-        const int t = valid ? tile_id(x[i], y[i], nx, ny, tx, ty, ntx) : -1;
-        // one LDS update per distinct tile of the wave (the wave owns h)
-        unsigned long long todo = __ballot(valid);
-        while (todo) {
-            const int leader = __ffsll((long long)todo) - 1;
-            const int lt = __shfl(t, leader, 64);
-            const unsigned long long mk = __ballot(valid && t == lt) & todo;
-            if (lane == leader) h[lt] += (uint32_t)__popcll(mk);
-            todo &= ~mk;
+    // kTsLoads particles per lane per iteration, all loaded before any is
+    // counted (rounds is a multiple of kTsLoads; the counts do not depend on
+    // the order)
+    for (int i0 = base; i0 < end; i0 += kTsWave * kTsLoads) {
+        float px[kTsLoads], py[kTsLoads];
+#pragma unroll
+        for (int k = 0; k < kTsLoads; ++k) {
+            const int i = i0 + k * kTsWave + lane;
+            px[k] = i < end ? x[i] : 0.f;
+            py[k] = i < end ? y[i] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kTsLoads; ++k) {
+            const bool valid = i0 + k * kTsWave + lane < end;
+            const int t = valid ? tile_id(px[k], py[k], nx, ny, tx, ty, ntx) : -1;
+            // one LDS update per distinct tile of the wave (the wave owns h)
+            unsigned long long todo = __ballot(valid);
+            while (todo) {
+                const int leader = __ffsll((long long)todo) - 1;
+                const int lt = __shfl(t, leader, 64);
+                const unsigned long long mk = __ballot(valid && t == lt) & todo;
+                if (lane == leader) h[lt] += (uint32_t)__popcll(mk);
+                todo &= ~mk;
+            }
         }
     }
     __syncthreads();
@@ -258,20 +271,30 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
     const int cbase = blockIdx.x * kTsWave * rounds;
     const int nr = min(rounds, (n - cbase + kTsWave - 1) / kTsWave);
     const unsigned long long below = (1ull << lane) - 1ull;
+    // software pipeline: round r+1's particle is loaded before round r is
+    // ranked and stored (the rounds still complete in order, so the
+    // permutation is the stable one)
+    float nxv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    auto load = [&](int r, float* v) {
+        const int i = cbase + r * kTsWave + lane;
+        if (r < nr && i < n) {
+            v[0] = m.x[i];
+            v[1] = m.y[i];
+            v[2] = m.ux[i];
+            v[3] = m.uy[i];
+            v[4] = m.uz[i];
+            if (m.w) v[5] = m.w[i];
+        }
+    };
+    load(0, nxv);
     for (int r = 0; r < nr; ++r) {
         const int i = cbase + r * kTsWave + lane;
         const bool valid = i < n;
-        float x = 0.f, y = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, w = 0.f;
+        const float x = nxv[0], y = nxv[1], ux = nxv[2], uy = nxv[3], uz = nxv[4],
+                    w = nxv[5];
+        load(r + 1, nxv);
         int t = 0;
-        if (valid) {
-            x = m.x[i];
-            y = m.y[i];
-            ux = m.ux[i];
-            uy = m.uy[i];
-            uz = m.uz[i];
-            if (m.w) w = m.w[i];
-            t = tile_id(x, y, nx, ny, tx, ty, ntx);
-        }
+        if (valid) t = tile_id(x, y, nx, ny, tx, ty, ntx);
         // lanes with the same tile, and this lane's rank among them
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < tbits; ++b) {

@@ -34,6 +34,14 @@ def weibel_growth(k: float, v0: float, c: float) -> float:
     return k * v0 / math.sqrt(1.0 + (k * c) ** 2)
 
 
+def weibel_growth_warm(k: float, v0: float, c: float, vth: float) -> float:
+    """Filamentation of beams with thermal spread vth, fluid closure (the
+    beams' pressure along k opposes the bunching):
+    gamma^2 = k^2 v0^2 / (1 + k^2 c^2) - k^2 vth^2 (0 if stable)."""
+    g2 = (k * v0) ** 2 / (1.0 + (k * c) ** 2) - (k * vth) ** 2
+    return math.sqrt(g2) if g2 > 0 else 0.0
+
+
 def peak_times(t: np.ndarray, y: np.ndarray, thresh_frac: float = 0.25) -> np.ndarray:
     """Local maxima above thresh_frac*max, refined by a parabola through the
     three samples around each maximum."""
