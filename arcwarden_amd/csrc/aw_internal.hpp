@@ -194,6 +194,7 @@ struct SortScratch {
     DevBuf<uint32_t> keyA, keyB;
     DevBuf<int32_t> valA, valB;
     DevBuf<uint32_t> table, sums;
+    DevBuf<uint32_t> colsum;  // tile sort: per (chunk block, tile) sums, then tile bases
     DevBuf<int32_t> bin_off, bin_perm;  // tile bins of the last binned deposit
 };
 

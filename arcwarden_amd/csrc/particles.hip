@@ -252,7 +252,57 @@ ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, 
         }
     }
     __syncthreads();
-    for (int t = lane; t < ntiles; t += kTsWave) table[(size_t)t * nchunks + blockIdx.x] = h[t];
+    // chunk-major table: this chunk's counts are one contiguous row
+    uint32_t* row = table + (size_t)blockIdx.x * ntiles;
+    for (int t = lane; t < ntiles; t += kTsWave) row[t] = h[t];
+}
+
+// Offsets of the chunk-major count table T[chunk][tile] in tile-major order:
+// off(t, c) = base(t) + sum_{c' < c} T[c'][t], base(t) = sum_{t' < t} total(t').
+// Chunks are taken kTsColBlock at a time: ts_colsum sums each (block, tile)
+// column segment, ts_colscan scans the block sums per tile (and yields the
+// tile totals), the totals are scanned into base(t), and ts_colapply rewrites
+// T in place with the offsets.  Every pass reads rows of T, coalesced across
+// the tiles.
+constexpr int kTsColBlock = 64;
+
+__global__ void ts_colsum_kernel(const uint32_t* __restrict__ table, int nchunks, int ntiles,
+                                 uint32_t* __restrict__ bsum) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int c0 = blockIdx.y * kTsColBlock, c1 = min(nchunks, c0 + kTsColBlock);
+    uint32_t v = 0;
+    for (int c = c0; c < c1; ++c) v += table[(size_t)c * ntiles + t];
+    bsum[(size_t)blockIdx.y * ntiles + t] = v;
+}
+
+// bsum[b][t] -> exclusive prefix over b; total[t] = column total
+__global__ void ts_colscan_kernel(uint32_t* __restrict__ bsum, int nblocks, int ntiles,
+                                  uint32_t* __restrict__ total) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    uint32_t run = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const uint32_t v = bsum[(size_t)b * ntiles + t];
+        bsum[(size_t)b * ntiles + t] = run;
+        run += v;
+    }
+    total[t] = run;
+}
+
+__global__ void ts_colapply_kernel(uint32_t* __restrict__ table, int nchunks, int ntiles,
+                                   const uint32_t* __restrict__ bpre,
+                                   const uint32_t* __restrict__ base) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int c0 = blockIdx.y * kTsColBlock, c1 = min(nchunks, c0 + kTsColBlock);
+    uint32_t run = base[t] + bpre[(size_t)blockIdx.y * ntiles + t];
+    for (int c = c0; c < c1; ++c) {
+        uint32_t* e = table + (size_t)c * ntiles + t;
+        const uint32_t v = *e;
+        *e = run;
+        run += v;
+    }
 }
 
 struct TsMove {
@@ -266,7 +316,8 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
                   int nchunks, int rounds, int tbits, const uint32_t* __restrict__ table) {
     extern __shared__ uint32_t base[];  // running output offset per tile
     const int lane = threadIdx.x;
-    for (int t = lane; t < ntiles; t += kTsWave) base[t] = table[(size_t)t * nchunks + blockIdx.x];
+    const uint32_t* row = table + (size_t)blockIdx.x * ntiles;
+    for (int t = lane; t < ntiles; t += kTsWave) base[t] = row[t];
     __syncthreads();
     const int cbase = blockIdx.x * kTsWave * rounds;
     const int nr = min(rounds, (n - cbase + kTsWave - 1) / kTsWave);
@@ -319,10 +370,10 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
     }
 }
 
-__global__ void ts_offsets_kernel(const uint32_t* __restrict__ table, int nchunks, int ntiles,
-                                  int n, int32_t* __restrict__ off) {
+__global__ void ts_offsets_kernel(const uint32_t* __restrict__ base, int ntiles, int n,
+                                  int32_t* __restrict__ off) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ntiles) off[t] = (int32_t)table[(size_t)t * nchunks];
+    if (t < ntiles) off[t] = (int32_t)base[t];
     if (t == ntiles) off[t] = n;
 }
 
@@ -575,9 +626,24 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
                                                  c.ntx, c.ntiles, nchunks, rounds,
                                                  c.sort.table.get());
     AW_LAUNCH_CHECK();
-    exclusive_scan(c, c.sort.table.get(), (int64_t)c.ntiles * nchunks, s);
-    ts_offsets_kernel<<<blocks_for((int64_t)c.ntiles + 1), kThreads, 0, s>>>(
-        c.sort.table.get(), nchunks, c.ntiles, n, off);
+    // offsets of the chunk-major table (see ts_colsum_kernel)
+    const int nblk = (nchunks + kTsColBlock - 1) / kTsColBlock;
+    const size_t nt = (size_t)c.ntiles;
+    c.sort.colsum.reserve((size_t)nblk * nt + nt + 1);
+    uint32_t* bsum = c.sort.colsum.get();
+    uint32_t* tbase = bsum + (size_t)nblk * nt;  // tile totals, scanned into tile bases
+    const dim3 cg(blocks_for((int64_t)c.ntiles), nblk);
+    ts_colsum_kernel<<<cg, kThreads, 0, s>>>(c.sort.table.get(), nchunks, c.ntiles, bsum);
+    AW_LAUNCH_CHECK();
+    ts_colscan_kernel<<<blocks_for((int64_t)c.ntiles), kThreads, 0, s>>>(bsum, nblk, c.ntiles,
+                                                                         tbase);
+    AW_LAUNCH_CHECK();
+    exclusive_scan(c, tbase, (int64_t)c.ntiles, s);
+    ts_colapply_kernel<<<cg, kThreads, 0, s>>>(c.sort.table.get(), nchunks, c.ntiles, bsum,
+                                               tbase);
+    AW_LAUNCH_CHECK();
+    ts_offsets_kernel<<<blocks_for((int64_t)c.ntiles + 1), kThreads, 0, s>>>(tbase, c.ntiles, n,
+                                                                           off);
     AW_LAUNCH_CHECK();
     TsMove m{src.x, src.y, src.ux, src.uy, src.uz, src.w,
              dst.x, dst.y, dst.ux, dst.uy, dst.uz, dst.w, perm};

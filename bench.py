@@ -81,6 +81,52 @@ def cpu_baseline(state, g_nx, g_ny, dx, w_uniform, rho, budget_s):
                       f"({n} particles) taken from the GPU state after init"}
 
 
+def two_stream_bench(steps: int = 200, cpu: bool = True):
+    """The plain ES two-stream run (64x64, dx 0.5, beams +-0.2 with vth 0.01,
+    64 ppc = 262,144 particles, x-displacement 1e-3 sin(k1 x), dt 0.1, seed
+    20260627): 200 GPU steps, then the oracle's scalar FP32 step (one thread,
+    the same algorithm) for the same 200 steps from the same initial state."""
+    import numpy as np
+    from arcwarden_amd import _lib as L
+    from arcwarden_amd.sim import SimConfig, Simulation
+    g = L.grid(64, 64, 32.0, 32.0)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260627,
+                      dump_every=steps)
+    sp = [L.species("beam_plus", 0.5, 32, (0.01, 0.01, 0.01), (0.2, 0, 0)),
+          L.species("beam_minus", 0.5, 32, (0.01, 0.01, 0.01), (-0.2, 0, 0))]
+    sim = Simulation(g, rp, sp, SimConfig(has_b0=False, mode_amp=[1e-3], sort_every=0))
+    sim.init()
+    state, rho = sim.particles(), sim.rho()
+    n = state["x"].size
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    el = time.perf_counter() - t0
+    ee_gpu = sim.history()[-1]["ee"]
+    sim.close()
+    out = {"workload": "ES two-stream 64x64, 64 ppc (262,144), 200 steps",
+           "value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pic_oracle as orc
+        orc.build()
+        og = orc.grid(64, 64, 0.5, 0.5)
+        pp = orc.push_params(-1.0, 0.1, (0.0, 0.0, 0.0), False)
+        st = {k: np.ascontiguousarray(state[k], np.float32) for k in ("x", "y", "ux", "uy", "uz")}
+        r = np.ascontiguousarray(rho, np.float32)
+        c0 = time.perf_counter()
+        for _ in range(steps):
+            orc.es_step_f32(og, pp, st, state["w_uniform"], 40, r, nthreads=1)
+        cel = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": n * steps / cel, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle scalar FP32 ES step, one thread, the full "
+                                         "200-step run from the GPU's initial state"}
+        out["gpu_vs_cpu_scalar"] = (n * steps / el) / (n * steps / cel)
+    out["ee_final"] = ee_gpu
+    return out
+
+
 def ensemble_bench(steps: int, graph: bool = True):
     """The 24-run batched ensemble (two-beam plasmas, seed 20260629: side in
     {64..256}, ppc in {16, 64, 256}, vth log-uniform [0.005, 0.05], v0 in
@@ -118,12 +164,14 @@ def ensemble_bench(steps: int, graph: bool = True):
                         "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
 
 
-def adjoint_bench(nsteps: int = 100):
+def adjoint_bench(nsteps: int = 100, cpu_steps: int = 0):
     """Field-energy gradient of the differentiable two-stream run (128^2, dx
     0.5, 64 ppc as beams +-0.2 with vth 0.02, 16 x-displacement modes
     A_m ~ 1e-3 U[-1,1] seed 20260630, dt 0.1, 100 steps, checkpoints every 10):
     forward with checkpoints, backward sweep, and the plain forward run of
-    the same state by aw_sim, each timed with a device synchronisation."""
+    the same state by aw_sim, each timed with a device synchronisation.
+    cpu_steps > 0 also times the oracle's FP64 host adjoint (forward + reverse
+    sweep, OpenMP) of the same initial state over that many steps."""
     from arcwarden_amd import _lib as L
     from arcwarden_amd.adjoint import two_stream_adjoint
     from arcwarden_amd.sim import SimConfig, Simulation
@@ -143,14 +191,35 @@ def adjoint_bench(nsteps: int = 100):
     sim.run(nsteps)
     t4 = time.perf_counter()
     sim.close()
+    cpu = None
+    if cpu_steps > 0:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pic_oracle as orc
+        orc.build()
+        st = run.initial_state()
+        g = run.grid
+        og = orc.grid(g.nx, g.ny, g.dx, g.dy)
+        w = run.species[0].density * g.dx * g.dy / run.species[0].ppc
+        threads = min(16, os.cpu_count() or 1)
+        c0 = time.perf_counter()
+        orc.es_adjoint_f64(og, st, w, cpu_steps, nthreads=threads)
+        cel = time.perf_counter() - c0
+        cpu = {"value": run.n * cpu_steps / cel, "unit": "particle-steps/s (forward+reverse)",
+               "cores": threads, "kind": "port",
+               "sample": f"oracle FP64 host adjoint (forward + reverse sweep), OpenMP "
+                         f"x{threads}, {cpu_steps} steps of the same 1,048,576-particle "
+                         f"initial state"}
     run.close()
     plain = t4 - t3
+    n = run.n
     return {"workload": "two-stream 128^2, 64 ppc (1,048,576), 16 modes, 100 steps, "
                         "checkpoint every 10, loss EE(100)",
             "loss": loss, "forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
             "plain_forward_ms": 1e3 * plain,
             "gradient_cost_vs_plain": (t2 - t0) / plain,
             "backward_vs_plain": (t2 - t1) / plain,
+            "value": n * nsteps / (t2 - t0), "unit": "particle-steps/s (forward+reverse)",
+            "cpu_baseline": cpu,
             "dL_dv0": float(gU[0, 0] - gU[1, 0]), "dL_dA1": float(gA[0])}
 
 
@@ -378,6 +447,8 @@ def main():
                     help="1: time the 128^2 two-stream adjoint (0: skip)")
     ap.add_argument("--darwin-steps", type=int, default=2000,
                     help="steps per run of the Darwin whistler-pump scan (0: skip)")
+    ap.add_argument("--two-stream", type=int, default=1,
+                    help="1: time the 64x64 two-stream run against the scalar host step")
     ap.add_argument("--darwin-autotune", type=int, default=1,
                     help="1: autotune the Darwin runs' tile / workgroups per tile")
     ap.add_argument("--no-darwin-batch", action="store_true",
@@ -471,7 +542,8 @@ def main():
         print(f"# sort every 20: {fixed['ms_per_step']:.4f} ms/step", file=sys.stderr,
               flush=True)
     ens = ensemble_bench(args.ensemble_steps) if args.ensemble_steps > 0 else None
-    adj = adjoint_bench() if args.adjoint else None
+    adj = adjoint_bench(cpu_steps=0 if args.no_cpu else 10) if args.adjoint else None
+    ts2 = two_stream_bench(cpu=not args.no_cpu) if args.two_stream else None
     dar = dar_cic = None
     if args.darwin_steps > 0:
         from arcwarden_amd import _lib as L
@@ -514,6 +586,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,
+        "two_stream": ts2,
         "darwin": dar,
         "darwin_cic_current": dar_cic,
         "policies": pol,
