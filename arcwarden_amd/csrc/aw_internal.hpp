@@ -573,4 +573,22 @@ __device__ __forceinline__ void esk_emit(const EskShape& m, double cJx, double c
         }
 }
 
+// Particle range [r0, r1) of workgroup `part` of the bpt workgroups sharing
+// the tile range [beg, end): the split points are rounded up to multiples of
+// 32 particles (128 B of each FP32 array), so neighbouring workgroups never
+// read or write the same cache line of a particle array (only tile ends stay
+// unaligned).
+__device__ __forceinline__ void split_range(int beg, int end, int part, int bpt, int& r0,
+                                            int& r1) {
+    auto at = [&](int p) {
+        if (p <= 0) return beg;
+        if (p >= bpt) return end;
+        int v = beg + (int)((long long)(end - beg) * p / bpt);
+        v = (v + 31) & ~31;
+        return v < beg ? beg : (v > end ? end : v);
+    };
+    r0 = at(part);
+    r1 = at(part + 1);
+}
+
 }  // namespace aw::dev

@@ -157,10 +157,7 @@ __device__ __forceinline__ Frame frame(const EmArgs& a) {
     const int tile = blockIdx.x / a.bpt, part = blockIdx.x % a.bpt;
     f.orgx = (tile % a.ntx) * a.tx - a.apron;
     f.orgy = (tile / a.ntx) * a.ty - a.apron;
-    const int beg = a.off[tile], end = a.off[tile + 1];
-    const int chunk = (end - beg + a.bpt - 1) / a.bpt;
-    f.r0 = beg + part * chunk;
-    f.r1 = min(end, f.r0 + chunk);
+    dev::split_range(a.off[tile], a.off[tile + 1], part, a.bpt, f.r0, f.r1);
     return f;
 }
 

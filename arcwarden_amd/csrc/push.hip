@@ -182,9 +182,8 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
     }
     __syncthreads();
 
-    const int beg = a.off[tile], end = a.off[tile + 1];
-    const int chunk = (end - beg + a.bpt - 1) / a.bpt;
-    const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
+    int r0, r1;
+    dev::split_range(a.off[tile], a.off[tile + 1], part, a.bpt, r0, r1);
     unsigned int nstray = 0;
     double ke = 0.0;
 
