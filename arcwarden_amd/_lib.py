@@ -110,7 +110,7 @@ class AwRunDesc(C.Structure):
 class AwEnsembleStats(C.Structure):
     _fields_ = [("nruns", C.c_int32), ("ngroups", C.c_int32), ("particles", C.c_int64),
                 ("cells", C.c_int64), ("workgroups", C.c_int32), ("graph", C.c_int32),
-                ("sorts", C.c_int64), ("graph_launches", C.c_int64)]
+                ("sorts", C.c_int64), ("graph_launches", C.c_int64), ("run_sorts", C.c_int64)]
 
 
 AW_DECK_MAX_SPECIES = 16

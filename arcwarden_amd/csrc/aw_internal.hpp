@@ -250,9 +250,11 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
                        int32_t* perm, hipStream_t s);
 
 // Deposit helpers (deposit.hip)
+// counter: ncounter counters cleared by the same launch (stray counts)
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign = false,
-                      unsigned long long* counter = nullptr, float* out2 = nullptr);
+                      unsigned long long* counter = nullptr, float* out2 = nullptr,
+                      int ncounter = 1);
 // acc == nullptr: the context's accumulator
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
                        const int32_t* sorted_off, hipStream_t s,

@@ -1415,6 +1415,7 @@ void darwin_batch_info(aw_sim* s, aw_ensemble_stats* out) {
     out->cells = ncell(*s->ctx) * s->runs;
     out->workgroups = s->ctx->ntiles * s->bpt * s->runs;
     out->sorts = s->nsorts;
+    out->run_sorts = s->nsorts * s->runs;  // a uniform batch re-sorts every run
 }
 
 static int check_run(const aw_sim* s, int run) {

@@ -121,9 +121,10 @@ deposit_tiles_kernel(DepArgs a, TileGeom t, unsigned long long* __restrict__ acc
 // accumulation and one 64-bit counter (saves separate memset launches).
 __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
                                     float* __restrict__ out2, int64_t n, double inv, int zero_acc,
-                                    int assign, unsigned long long* __restrict__ counter) {
+                                    int assign, unsigned long long* __restrict__ counter,
+                                    int ncounter) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0 && counter) *counter = 0ull;
+    if (counter && c < ncounter) counter[c] = 0ull;
     if (c >= n) return;
     const long long v = (long long)acc[c];
     const float add = (float)((double)v * inv);
@@ -137,11 +138,12 @@ __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float*
 
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign, unsigned long long* counter,
-                      float* out2) {
+                      float* out2, int ncounter) {
     if (n == 0) return;
-    fix_to_float_kernel<<<blocks_for(n), kThreads, 0, s>>>(
+    AW_REQUIRE(ncounter >= 0, "ncounter must be non-negative");
+    fix_to_float_kernel<<<blocks_for(std::max<int64_t>(n, ncounter)), kThreads, 0, s>>>(
         const_cast<unsigned long long*>(acc), out, out2, n, std::ldexp(1.0, -scale_log2),
-        zero_acc ? 1 : 0, assign ? 1 : 0, counter);
+        zero_acc ? 1 : 0, assign ? 1 : 0, counter, ncounter);
     AW_LAUNCH_CHECK();
 }
 

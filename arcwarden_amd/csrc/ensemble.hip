@@ -7,11 +7,15 @@
 // every run keeps its own grid, RunParams, species and tile geometry, and a
 // step of the whole ensemble is
 //   1 launch   fixed-point rho^n -> float for every run (concatenated grids),
-//   3 launches per grid size: R2C over the size's runs, k-space, C2R (Ex|Ey),
+//   3 launches per grid size: R2C over the size's runs, k-space, C2R (Ex|Ey);
+//              the sizes are independent and run concurrently, on the main
+//              stream and up to kSideStreams side streams (fork/join events),
 //   1 launch   fused gather + Boris + move + wrap + deposit over every run
 //              (each workgroup finds its run in a device-resident run table),
-// with the stable tile re-sort of every run on a cadence, or when more than a
-// fraction of the particles strayed from their tile apron.  The step can
+// with the stable tile re-sort of every run on a cadence, and of each run
+// whose own particles strayed from their tile apron beyond a fraction of its
+// particle count (per-run stray counters; the other runs keep their order
+// and buffers).  The step can
 // be captured once into a HIP graph and replayed (two graphs: with and
 // without the diagnostics reduction); the run table lives in device memory,
 // so the re-sort's buffer swap only rewrites the table, not the graphs.
@@ -29,6 +33,12 @@ namespace aw {
 namespace {
 
 constexpr int kThreads = 256;
+// streams beside the main one for the per-size solves (the process has four
+// hardware queues)
+#ifndef AW_ENSEMBLE_SIDE_STREAMS
+#define AW_ENSEMBLE_SIDE_STREAMS 2
+#endif
+constexpr int kSideStreams = AW_ENSEMBLE_SIDE_STREAMS;
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
@@ -115,6 +125,8 @@ struct aw_ensemble {
         double w_uniform = 0.0;
         bool has_b0 = false;
         int bpt = 0, wg_begin = 0, nwg = 0;
+        int cur = 0;            // particle buffer (P[cur]) holding the run
+        bool sort_req = false;  // strays beyond the threshold: re-sort
         std::vector<aw_diag_sample> history;
     };
     struct Group {
@@ -132,7 +144,8 @@ struct aw_ensemble {
     bool uz = false;
     size_t lds = 0;
     DevBuf<float> P[2][5];
-    int cur = 0;
+    std::vector<hipStream_t> side;         // concurrent per-size solves
+    std::vector<hipEvent_t> fork_join;     // [0] fork, [1 + i] join of side[i]
     DevBuf<int32_t> tile_off, perm;
     DevBuf<unsigned long long> acc, strays;
     DevBuf<float> rho, rin, E;
@@ -142,11 +155,12 @@ struct aw_ensemble {
     int diag_cap = 64;
     std::vector<int64_t> diag_pending;
     bool initialized = false, fields_ready = false;
-    int64_t steps_since_sort = 0, nsorts = 0;
+    int64_t steps_since_sort = 0, nsorts = 0, run_sorts = 0;
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // [dump]
     int64_t graph_launches = 0;
-    // stray feedback (read back asynchronously, at most kStrayLag steps late)
+    // per-run stray counts, read back asynchronously, at most kStrayLag
+    // steps late
     unsigned long long* stray_host = nullptr;
     hipEvent_t stray_ev = nullptr;
     bool stray_pending = false, sort_requested = false;
@@ -159,6 +173,11 @@ struct aw_ensemble {
             if (gx) (void)hipGraphExecDestroy(gx);
         if (stray_ev) (void)hipEventDestroy(stray_ev);
         if (stray_host) (void)hipHostFree(stray_host);
+        for (hipEvent_t ev : fork_join) (void)hipEventDestroy(ev);
+        for (hipStream_t ss : side) {
+            (void)hipStreamSynchronize(ss);
+            (void)hipStreamDestroy(ss);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -172,7 +191,11 @@ struct aw_ensemble {
         else if (hipEventQuery(stray_ev) != hipSuccess) return;
         stray_pending = false;
         const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
-        if ((double)*stray_host > frac * (double)ntot) sort_requested = true;
+        for (size_t k = 0; k < runs.size(); ++k)
+            if ((double)stray_host[k] > frac * (double)runs[k].n) {
+                runs[k].sort_req = true;
+                sort_requested = true;
+            }
     }
 
     aw_particles run_particles(const Run& r, int which) const {
@@ -197,27 +220,40 @@ struct aw_ensemble {
                 fill_run_slot(h.data() + sb * k, *r.ctx, run_particles(r, which),
                               tile_off.get() + r.obase, E.get() + 2 * r.cbase,
                               E.get() + 2 * r.cbase + ncell(*r.ctx), r.rp, r.has_b0,
-                              acc.get() + r.cbase, strays.get(), ke_part.get() + r.wg_begin,
+                              acc.get() + r.cbase, strays.get() + k, ke_part.get() + r.wg_begin,
                               r.bpt, r.wg_begin);
             }
             slots[which].resize(h.size());
             AW_HIP(hipMemcpy(slots[which].get(), h.data(), h.size(), hipMemcpyHostToDevice));
         }
         active.resize(sb * runs.size());
-        activate();
+        for (size_t k = 0; k < runs.size(); ++k) activate(k);
     }
 
-    void activate() {
-        AW_HIP(hipMemcpyAsync(active.get(), slots[cur].get(), active.size(),
+    // the run table entry of run k <- its slot for the buffer that holds it
+    void activate(size_t k) {
+        const size_t sb = run_slot_bytes();
+        AW_HIP(hipMemcpyAsync(active.get() + sb * k, slots[runs[k].cur].get() + sb * k, sb,
                               hipMemcpyDeviceToDevice, stream));
     }
 
-    void sort_all() {
-        for (Run& r : runs)
-            sort_tiles(*r.ctx, run_particles(r, cur), run_particles(r, cur ^ 1), perm.get(),
+    // Stable tile re-sort of every run (all) or of the runs whose strays
+    // crossed the threshold; a sorted run moves to its other buffer.
+    void sort_runs(bool all) {
+        bool any = false;
+        for (size_t k = 0; k < runs.size(); ++k) {
+            Run& r = runs[k];
+            if (!all && !r.sort_req) continue;
+            sort_tiles(*r.ctx, run_particles(r, r.cur), run_particles(r, r.cur ^ 1), perm.get(),
                        tile_off.get() + r.obase, stream);
-        cur ^= 1;
-        activate();
+            r.cur ^= 1;
+            r.sort_req = false;
+            activate(k);
+            ++run_sorts;
+            any = true;
+        }
+        sort_requested = false;
+        if (!any) return;
         steps_since_sort = 0;
         ++nsorts;
     }
@@ -225,10 +261,26 @@ struct aw_ensemble {
     void convert_and_solve() {
         // rho^n for every run; rin receives the R2C input copy, strays reset
         fix_to_float_add(acc.get(), rho.get(), nctot, fix_log2, true, stream, true, strays.get(),
-                         rin.get());
-        for (const Group& gr : groups)
+                         rin.get(), (int)runs.size());
+        // one batched solve per grid size; the sizes touch disjoint buffers
+        // (each its own plans, work area and k-space scratch), so they fork
+        // onto the side streams and join before the push
+        const size_t nside = std::min(side.size(), groups.size() - 1);
+        if (nside) {
+            AW_HIP(hipEventRecord(fork_join[0], stream));
+            for (size_t i = 0; i < nside; ++i)
+                AW_HIP(hipStreamWaitEvent(side[i], fork_join[0], 0));
+        }
+        for (size_t g = 0; g < groups.size(); ++g) {
+            const Group& gr = groups[g];
+            const size_t lane = nside ? g % (nside + 1) : 0;
             solve_es_batch(*runs[gr.first].ctx, gr.count, rin.get() + gr.cbase,
-                           E.get() + 2 * gr.cbase, eps0, stream);
+                           E.get() + 2 * gr.cbase, eps0, lane ? side[lane - 1] : stream);
+        }
+        for (size_t i = 0; i < nside; ++i) {
+            AW_HIP(hipEventRecord(fork_join[1 + i], side[i]));
+            AW_HIP(hipStreamWaitEvent(stream, fork_join[1 + i], 0));
+        }
     }
 
     void enqueue_step(bool dump) {
@@ -264,22 +316,22 @@ struct aw_ensemble {
     void init() {
         const hipStream_t s = stream;
         for (Run& r : runs) {
-            const aw_particles p = run_particles(r, cur);
+            const aw_particles p = run_particles(r, r.cur);
             std::vector<double> wts(r.species.size());
             load_species(*r.ctx, r.species.data(), (int)r.species.size(), r.rp,
                          r.modes.empty() ? nullptr : r.modes.data(), (int)r.modes.size(), p,
                          wts.data(), s);
         }
-        sort_all();
-        nsorts = 0;
+        sort_runs(true);
+        nsorts = run_sorts = 0;
         for (Run& r : runs) {
-            deposit_rho_fixed(*r.ctx, run_particles(r, cur), r.rp.qm, AW_DEPOSIT_BINNED,
+            deposit_rho_fixed(*r.ctx, run_particles(r, r.cur), r.rp.qm, AW_DEPOSIT_BINNED,
                               tile_off.get() + r.obase, s, acc.get() + r.cbase);
         }
         convert_and_solve();
         for (Run& r : runs) {
             const int64_t nc = ncell(*r.ctx);
-            push_es(*r.ctx, run_particles(r, cur), E.get() + 2 * r.cbase,
+            push_es(*r.ctx, run_particles(r, r.cur), E.get() + 2 * r.cbase,
                     E.get() + 2 * r.cbase + nc, r.rp, r.has_b0, true, s);
         }
         fields_ready = true;
@@ -293,10 +345,8 @@ struct aw_ensemble {
                                __FILE__, __LINE__);
         const hipStream_t s = stream;
         poll_strays();
-        if (sort_requested || (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every)) {
-            sort_all();
-            sort_requested = false;
-        }
+        if (cfg.sort_every > 0 && steps_since_sort >= cfg.sort_every) sort_runs(true);
+        else if (sort_requested) sort_runs(false);
         const bool dump = cfg_dump(nstep);
         if (fields_ready) {
             // the init solve already holds E^0: push only (first step)
@@ -313,8 +363,9 @@ struct aw_ensemble {
         } else {
             enqueue_step(dump);
         }
-        if (!stray_pending) {  // this step's stray count (reset by the next conversion)
-            AW_HIP(hipMemcpyAsync(stray_host, strays.get(), sizeof(unsigned long long),
+        if (!stray_pending) {  // this step's stray counts (reset by the next conversion)
+            AW_HIP(hipMemcpyAsync(stray_host, strays.get(),
+                                  runs.size() * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s));
             AW_HIP(hipEventRecord(stray_ev, s));
             stray_pending = true;
@@ -471,7 +522,8 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         e->perm.resize((size_t)nmax);
         e->acc.resize((size_t)cb);
         AW_HIP(hipMemset(e->acc.get(), 0, (size_t)cb * sizeof(unsigned long long)));
-        e->strays.resize(1);
+        e->strays.resize(e->runs.size());
+        AW_HIP(hipMemset(e->strays.get(), 0, e->runs.size() * sizeof(unsigned long long)));
         e->rho.resize((size_t)cb);
         e->rin.resize((size_t)cb);
         e->E.resize((size_t)2 * cb);
@@ -489,8 +541,19 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         AW_HIP(hipMemcpy(e->diag_runs.get(), dr.data(), R * sizeof(DiagRun),
                          hipMemcpyHostToDevice));
         e->build_slots();
-        AW_HIP(hipHostMalloc((void**)&e->stray_host, sizeof(unsigned long long),
-                             hipHostMallocDefault));
+        AW_HIP(hipHostMalloc((void**)&e->stray_host,
+                             e->runs.size() * sizeof(unsigned long long), hipHostMallocDefault));
+        const size_t nside = std::min<size_t>(kSideStreams, e->groups.size() - 1);
+        for (size_t i = 0; i < nside; ++i) {
+            hipStream_t ss = nullptr;
+            AW_HIP(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
+            e->side.push_back(ss);
+        }
+        for (size_t i = 0; i < (nside ? nside + 1 : 0); ++i) {
+            hipEvent_t ev = nullptr;
+            AW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            e->fork_join.push_back(ev);
+        }
         AW_HIP(hipEventCreateWithFlags(&e->stray_ev, hipEventDisableTiming));
         *out = e.release();
     });
@@ -566,6 +629,7 @@ aw_status aw_ensemble_info(aw_ensemble* e, aw_ensemble_stats* out) {
         out->cells = e->nctot;
         out->workgroups = e->total_wg;
         out->sorts = e->nsorts;
+        out->run_sorts = e->run_sorts;
         out->graph_launches = e->graph_launches;
         out->graph = e->use_graph ? 1 : 0;
     });
@@ -578,7 +642,8 @@ aw_status aw_ensemble_particles(aw_ensemble* e, int32_t run, aw_particles* out) 
             *out = darwin_batch_particles(e->dar, run);
             return;
         }
-        *out = e->run_particles(e->by_user(run), e->cur);
+        const auto& r = e->by_user(run);
+        *out = e->run_particles(r, r.cur);
     });
 }
 

@@ -193,12 +193,17 @@ __global__ void key_offsets_kernel(const uint32_t* __restrict__ skeys, int n, ui
 // radix + permute path: one histogram pass over (x, y), one scan of the
 // chunk x tile count table, and one scatter that ranks each particle inside
 // its chunk with wavefront ballots and writes it straight to its final slot.
-// A chunk belongs to one single-wave workgroup: ranking needs no cross-wave
-// bookkeeping, and the LDS table is one 32-bit counter per tile, so many
-// workgroups stay resident per CU to hide the memory latency.
+// In the scatter a chunk belongs to one single-wave workgroup: ranking needs
+// no cross-wave bookkeeping, and the LDS table is one 32-bit counter per
+// tile, so many workgroups stay resident per CU; each wave loads kTsAhead
+// rounds ahead of the one it ranks to keep enough reads in flight.  The
+// histogram only counts, so four waves share a chunk and its LDS table.
 
 constexpr int kTsWave = 64;
-constexpr int kTsLoads = 4;  // particles per lane per histogram iteration
+constexpr int kTsHistThreads = 256;
+constexpr int kTsLoads = 4;  // particles per thread per histogram iteration
+constexpr int kTsAhead = 8;  // rounds of 64 particles in flight per scatter wave
+constexpr int kTsMatch = 4;  // tiles matched one by one per scatter round
 
 // rounds of 64 particles per chunk: at least 16, more when the chunk x tile
 // count table would exceed 2^24 entries
@@ -209,44 +214,58 @@ inline int ts_rounds(int64_t n, int ntiles) {
     return (int)r;
 }
 
-__device__ __forceinline__ int tile_id(float x, float y, int nx, int ny, int tx, int ty,
-                                       int ntx) {
-    const int c = dev::cell_of(x, y, nx, ny);
-    return ((c / nx) / ty) * ntx + (c % nx) / tx;
+// Tile geometry for tile ids; rtx, rty = 1/tx, 1/ty in FP32.
+struct TileMap {
+    int nx, ny, tx, ty, ntx;
+    float rtx, rty;
+};
+
+// a / b for 0 <= a < 2^24, b >= 1: the FP32 estimate a * (1/b) is within one
+// of the quotient, and one integer check each way makes it exact
+__device__ __forceinline__ int div_small(int a, int b, float rb) {
+    int q = (int)((float)a * rb);
+    if (q * b > a) --q;
+    else if ((q + 1) * b <= a) ++q;
+    return q;
 }
 
-__global__ void __launch_bounds__(kTsWave)
-ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, int nx, int ny,
-               int tx, int ty, int ntx, int ntiles, int nchunks, int rounds,
-               uint32_t* __restrict__ table) {
+// tile of the cell holding (x, y), cells clamped to the grid as cell_of does
+__device__ __forceinline__ int tile_id(float x, float y, const TileMap& m) {
+    const int ci = min(max((int)floorf(x), 0), m.nx - 1);
+    const int cj = min(max((int)floorf(y), 0), m.ny - 1);
+    return div_small(cj, m.ty, m.rty) * m.ntx + div_small(ci, m.tx, m.rtx);
+}
+
+__global__ void __launch_bounds__(kTsHistThreads)
+ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, TileMap tm,
+               int ntiles, int rounds, uint32_t* __restrict__ table) {
     extern __shared__ uint32_t h[];
-    const int lane = threadIdx.x;
-    for (int t = lane; t < ntiles; t += kTsWave) h[t] = 0;
+    for (int t = threadIdx.x; t < ntiles; t += kTsHistThreads) h[t] = 0;
     __syncthreads();
+    const int lane = threadIdx.x & (kTsWave - 1);
     const int base = blockIdx.x * kTsWave * rounds;
     const int end = min(n, base + kTsWave * rounds);
-    // kTsLoads particles per lane per iteration, all loaded before any is
-    // counted (rounds is a multiple of kTsLoads; the counts do not depend on
-    // the order)
-    for (int i0 = base; i0 < end; i0 += kTsWave * kTsLoads) {
+    // kTsLoads particles per thread per iteration, all loaded before any is
+    // counted (the counts do not depend on the order)
+    for (int i0 = base; i0 < end; i0 += kTsHistThreads * kTsLoads) {
         float px[kTsLoads], py[kTsLoads];
 #pragma unroll
         for (int k = 0; k < kTsLoads; ++k) {
-            const int i = i0 + k * kTsWave + lane;
+            const int i = i0 + k * kTsHistThreads + (int)threadIdx.x;
             px[k] = i < end ? x[i] : 0.f;
             py[k] = i < end ? y[i] : 0.f;
         }
 #pragma unroll
         for (int k = 0; k < kTsLoads; ++k) {
-            const bool valid = i0 + k * kTsWave + lane < end;
-            const int t = valid ? tile_id(px[k], py[k], nx, ny, tx, ty, ntx) : -1;
-            // one LDS update per distinct tile of the wave (the wave owns h)
+            const bool valid = i0 + k * kTsHistThreads + (int)threadIdx.x < end;
+            const int t = valid ? tile_id(px[k], py[k], tm) : -1;
+            // one LDS atomic per distinct tile of the wave
             unsigned long long todo = __ballot(valid);
             while (todo) {
                 const int leader = __ffsll((long long)todo) - 1;
-                const int lt = __shfl(t, leader, 64);
-                const unsigned long long mk = __ballot(valid && t == lt) & todo;
-                if (lane == leader) h[lt] += (uint32_t)__popcll(mk);
+                const int lt = __builtin_amdgcn_readlane(t, leader);
+                const unsigned long long mk = __ballot(t == lt) & todo;
+                if (lane == leader) atomicAdd(&h[lt], (uint32_t)__popcll(mk));
                 todo &= ~mk;
             }
         }
@@ -254,7 +273,7 @@ ts_hist_kernel(const float* __restrict__ x, const float* __restrict__ y, int n, 
     __syncthreads();
     // chunk-major table: this chunk's counts are one contiguous row
     uint32_t* row = table + (size_t)blockIdx.x * ntiles;
-    for (int t = lane; t < ntiles; t += kTsWave) row[t] = h[t];
+    for (int t = threadIdx.x; t < ntiles; t += kTsHistThreads) row[t] = h[t];
 }
 
 // Offsets of the chunk-major count table T[chunk][tile] in tile-major order:
@@ -312,8 +331,8 @@ struct TsMove {
 };
 
 __global__ void __launch_bounds__(kTsWave)
-ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int ntiles,
-                  int nchunks, int rounds, int tbits, const uint32_t* __restrict__ table) {
+ts_scatter_kernel(TsMove m, int n, TileMap tm, int ntiles, int rounds, int tbits,
+                  const uint32_t* __restrict__ table) {
     extern __shared__ uint32_t base[];  // running output offset per tile
     const int lane = threadIdx.x;
     const uint32_t* row = table + (size_t)blockIdx.x * ntiles;
@@ -322,10 +341,10 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
     const int cbase = blockIdx.x * kTsWave * rounds;
     const int nr = min(rounds, (n - cbase + kTsWave - 1) / kTsWave);
     const unsigned long long below = (1ull << lane) - 1ull;
-    // software pipeline: round r+1's particle is loaded before round r is
-    // ranked and stored (the rounds still complete in order, so the
+    // software pipeline: rounds r+1 .. r+kTsAhead are loaded before round r
+    // is ranked and stored (the rounds still complete in order, so the
     // permutation is the stable one)
-    float nxv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float ring[kTsAhead][6];
     auto load = [&](int r, float* v) {
         const int i = cbase + r * kTsWave + lane;
         if (r < nr && i < n) {
@@ -334,39 +353,60 @@ ts_scatter_kernel(TsMove m, int n, int nx, int ny, int tx, int ty, int ntx, int 
             v[2] = m.ux[i];
             v[3] = m.uy[i];
             v[4] = m.uz[i];
-            if (m.w) v[5] = m.w[i];
+            v[5] = m.w ? m.w[i] : 0.f;
+        } else {
+            for (int q = 0; q < 6; ++q) v[q] = 0.f;
         }
     };
-    load(0, nxv);
-    for (int r = 0; r < nr; ++r) {
-        const int i = cbase + r * kTsWave + lane;
-        const bool valid = i < n;
-        const float x = nxv[0], y = nxv[1], ux = nxv[2], uy = nxv[3], uz = nxv[4],
-                    w = nxv[5];
-        load(r + 1, nxv);
-        int t = 0;
-        if (valid) t = tile_id(x, y, nx, ny, tx, ty, ntx);
-        // lanes with the same tile, and this lane's rank among them
-        unsigned long long peers = __ballot(valid);
-        for (int b = 0; b < tbits; ++b) {
-            const bool bit = (t >> b) & 1;
-            const unsigned long long bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+#pragma unroll
+    for (int p = 0; p < kTsAhead; ++p) load(p, ring[p]);
+    for (int r0 = 0; r0 < nr; r0 += kTsAhead) {
+#pragma unroll
+        for (int p = 0; p < kTsAhead; ++p) {
+            const int r = r0 + p;
+            if (r >= nr) break;
+            const int i = cbase + r * kTsWave + lane;
+            const bool valid = i < n;
+            const float x = ring[p][0], y = ring[p][1], ux = ring[p][2], uy = ring[p][3],
+                        uz = ring[p][4], w = ring[p][5];
+            load(r + kTsAhead, ring[p]);
+            const int t = valid ? tile_id(x, y, tm) : -1;
+            // peers = the lanes with this lane's tile.  A nearly sorted wave
+            // spans few tiles: match them one at a time (the lowest pending
+            // lane leads); past kTsMatch tiles, the rest split on tile-id
+            // bits (a tile's lanes are all pending or all matched).
+            unsigned long long todo = __ballot(valid), peers = 0;
+            for (int it = 0; it < kTsMatch && todo; ++it) {
+                const int lt = __builtin_amdgcn_readlane(t, __ffsll((long long)todo) - 1);
+                const unsigned long long mk = __ballot(t == lt);
+                if (t == lt) peers = mk;
+                todo &= ~mk;
+            }
+            if (todo) {
+                const bool mine = (todo >> lane) & 1ull;
+                unsigned long long pk = todo;
+                for (int b = 0; b < tbits; ++b) {
+                    const bool bit = (t >> b) & 1;
+                    const unsigned long long bal = __ballot(bit);
+                    pk &= bit ? bal : ~bal;
+                }
+                if (mine) peers = pk;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & below);
+            // one wave per workgroup: its LDS accesses complete in program
+            // order, so every lane reads base[t] before its leader advances it
+            if (valid) {
+                const uint32_t pos = base[t] + rank;
+                m.xo[pos] = x;
+                m.yo[pos] = y;
+                m.uxo[pos] = ux;
+                m.uyo[pos] = uy;
+                m.uzo[pos] = uz;
+                if (m.wo) m.wo[pos] = w;
+                if (m.perm) m.perm[pos] = i;
+                if (rank == 0) base[t] += (uint32_t)__popcll(peers);
+            }
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & below);
-        if (valid) {
-            const uint32_t pos = base[t] + rank;
-            m.xo[pos] = x;
-            m.yo[pos] = y;
-            m.uxo[pos] = ux;
-            m.uyo[pos] = uy;
-            m.uzo[pos] = uz;
-            if (m.wo) m.wo[pos] = w;
-            if (m.perm) m.perm[pos] = i;
-        }
-        __syncthreads();  // every lane has read base[] before the leaders advance it
-        if (valid && rank == 0) base[t] += (uint32_t)__popcll(peers);
-        __syncthreads();
     }
 }
 
@@ -622,9 +662,10 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
     int tbits = 0;
     while ((1 << tbits) < c.ntiles) ++tbits;
     c.sort.table.reserve((size_t)c.ntiles * nchunks);
-    ts_hist_kernel<<<nchunks, kTsWave, lds, s>>>(src.x, src.y, n, c.g.nx, c.g.ny, c.tx, c.ty,
-                                                 c.ntx, c.ntiles, nchunks, rounds,
-                                                 c.sort.table.get());
+    AW_REQUIRE(c.g.nx < (1 << 24) && c.g.ny < (1 << 24), "grid sides must be below 2^24");
+    const TileMap tm{c.g.nx, c.g.ny, c.tx, c.ty, c.ntx, 1.0f / (float)c.tx, 1.0f / (float)c.ty};
+    ts_hist_kernel<<<nchunks, kTsHistThreads, lds, s>>>(src.x, src.y, n, tm, c.ntiles, rounds,
+                                                        c.sort.table.get());
     AW_LAUNCH_CHECK();
     // offsets of the chunk-major table (see ts_colsum_kernel)
     const int nblk = (nchunks + kTsColBlock - 1) / kTsColBlock;
@@ -647,8 +688,7 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
     AW_LAUNCH_CHECK();
     TsMove m{src.x, src.y, src.ux, src.uy, src.uz, src.w,
              dst.x, dst.y, dst.ux, dst.uy, dst.uz, dst.w, perm};
-    ts_scatter_kernel<<<nchunks, kTsWave, lds, s>>>(m, n, c.g.nx, c.g.ny, c.tx, c.ty, c.ntx,
-                                                    c.ntiles, nchunks, rounds, tbits,
+    ts_scatter_kernel<<<nchunks, kTsWave, lds, s>>>(m, n, tm, c.ntiles, rounds, tbits,
                                                     c.sort.table.get());
     AW_LAUNCH_CHECK();
     return true;

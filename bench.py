@@ -160,6 +160,7 @@ def ensemble_bench(steps: int, graph: bool = True):
             "value": ntot * steps / bat, "unit": "particle-steps/s",
             "sequential_value": ntot * steps / seq, "speedup_vs_sequential": seq / bat,
             "graph": bool(graph), "size_groups": info["ngroups"],
+            "sort_events": info["sorts"], "run_sorts": info["run_sorts"],
             "workload": "24 two-beam ES runs, seed 20260629, sides 64-256, ppc 16/64/256, "
                         "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
 

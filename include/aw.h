@@ -473,6 +473,8 @@ typedef struct aw_ensemble_stats {
     int64_t particles, cells;
     int32_t workgroups, graph;
     int64_t sorts, graph_launches;
+    int64_t run_sorts; /* runs re-sorted, summed over the sorts (a sort on the
+                          stray threshold re-sorts only the runs beyond it) */
 } aw_ensemble_stats;
 
 /* cfg: tile_x (0 -> 8 at >= 64 particles per cell, else 16), blocks_per_tile,

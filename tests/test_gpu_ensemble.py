@@ -65,6 +65,39 @@ def test_ensemble_matches_single_runs(gpu, graph):
     ens.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_stray_sort_is_per_run(gpu, graph):
+    """Only the run whose particles strayed beyond the threshold is re-sorted
+    (fast beams: 0.2 cells per step); the slow runs keep their order.  Every
+    run still equals the same run stepped alone, and the single runs' own
+    stray-driven sorts reorder nothing that rho, E or the particle multiset
+    can see."""
+    runs = [two_beam_run(32, 16, 0.01, 0.01, seed=301), two_beam_run(32, 16, 0.01, 1.0, seed=302),
+            two_beam_run(48, 16, 0.01, 0.01, seed=303), two_beam_run(64, 16, 0.01, 0.01, seed=304)]
+    nsteps = 30
+    cfg = dict(sort_every=0, stray_frac=0.01)
+    ens = Ensemble(runs, SimConfig(**cfg), graph=graph)
+    ens.init()
+    ens.run(0, nsteps)
+    info = ens.info()
+    assert info["ngroups"] == 3
+    assert info["sorts"] >= 1 and info["run_sorts"] == info["sorts"], info
+    for i, r in enumerate(runs):
+        s = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, **cfg))
+        s.init()
+        s.run(nsteps)
+        assert np.array_equal(ens.rho(i), s.rho()), i
+        f, fs = ens.fields(i), s.fields()
+        assert np.array_equal(f["Ex"], fs["Ex"]) and np.array_equal(f["Ey"], fs["Ey"]), i
+        pe, ps = ens.particles(i), s.particles()
+        assert np.array_equal(_multiset(pe), _multiset(ps)), i
+        if i != 1:  # never re-sorted, here or alone: the same order
+            for k in ("x", "y", "ux", "uy", "uz"):
+                assert np.array_equal(pe[k], ps[k]), (i, k)
+        s.close()
+    ens.close()
+
+
 def test_random_ensemble_generator_and_invariants(gpu):
     """The 24-run workload generator (seeded) and a short run of a scaled-down
     draw: charge conserved in every run."""
