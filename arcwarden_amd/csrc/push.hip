@@ -174,6 +174,31 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
     const int tile = blk / a.bpt, part = blk % a.bpt;
     f.orgx = (tile % a.ntx) * a.tx - a.apron;
     f.orgy = (tile / a.ntx) * a.ty - a.apron;
+    int r0, r1;
+    dev::split_range(a.off[tile], a.off[tile + 1], part, a.bpt, r0, r1);
+
+    // a thread's particles k, k + kThreads, ...: loaded two at a time, every
+    // load of a pair issued before either particle is consumed
+    struct Part {
+        float x, y, ux, uy, uz, w;
+    };
+    auto fetch = [&](int i) {
+        Part p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (i < r1) {
+            p.x = a.x[i];
+            p.y = a.y[i];
+            p.ux = a.ux[i];
+            p.uy = a.uy[i];
+            if constexpr (kLoadUz) p.uz = a.uz[i];
+            if constexpr (PER_W) p.w = a.w[i];
+        }
+        return p;
+    };
+    // the first pair is requested before the field tile, so that the
+    // workgroup's particle reads overlap its tile load
+    int k = r0 + (int)threadIdx.x;
+    Part c0 = fetch(k), c1 = fetch(k + kThreads);
+
     for (int c = threadIdx.x; c < R; c += kThreads) {
         const int gx = dev::wrapfar(f.orgx + c % W, a.nx);
         const int gy = dev::wrapfar(f.orgy + c / W, a.ny);
@@ -182,36 +207,16 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
     }
     __syncthreads();
 
-    int r0, r1;
-    dev::split_range(a.off[tile], a.off[tile + 1], part, a.bpt, r0, r1);
     unsigned int nstray = 0;
     double ke = 0.0;
-
-    // two particles per thread per iteration: both sets of loads are issued
-    // before either is consumed
-    int k = r0 + (int)threadIdx.x;
     for (; k + kThreads < r1; k += 2 * kThreads) {
-        const int k2 = k + kThreads;
-        const float x0 = a.x[k], y0 = a.y[k], ux0 = a.ux[k], uy0 = a.uy[k];
-        const float x1 = a.x[k2], y1 = a.y[k2], ux1 = a.ux[k2], uy1 = a.uy[k2];
-        float uz0 = 0.f, uz1 = 0.f, w0 = 0.f, w1 = 0.f;
-        if constexpr (kLoadUz) {
-            uz0 = a.uz[k];
-            uz1 = a.uz[k2];
-        }
-        if constexpr (PER_W) {
-            w0 = a.w[k];
-            w1 = a.w[k2];
-        }
-        push_one<PER_W, KE, UZ>(a, f, k, x0, y0, ux0, uy0, uz0, w0, nstray, ke);
-        push_one<PER_W, KE, UZ>(a, f, k2, x1, y1, ux1, uy1, uz1, w1, nstray, ke);
+        push_one<PER_W, KE, UZ>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray, ke);
+        push_one<PER_W, KE, UZ>(a, f, k + kThreads, c1.x, c1.y, c1.ux, c1.uy, c1.uz, c1.w,
+                                nstray, ke);
+        c0 = fetch(k + 2 * kThreads);
+        c1 = fetch(k + 3 * kThreads);
     }
-    if (k < r1) {
-        float uz0 = 0.f, w0 = 0.f;
-        if constexpr (kLoadUz) uz0 = a.uz[k];
-        if constexpr (PER_W) w0 = a.w[k];
-        push_one<PER_W, KE, UZ>(a, f, k, a.x[k], a.y[k], a.ux[k], a.uy[k], uz0, w0, nstray, ke);
-    }
+    if (k < r1) push_one<PER_W, KE, UZ>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray, ke);
 
     // per-workgroup stray count and kinetic-energy partial
     __shared__ unsigned int s_stray;
