@@ -127,12 +127,15 @@ def two_stream_bench(steps: int = 200, cpu: bool = True):
     return out
 
 
-def ensemble_bench(steps: int, graph: bool = True):
+def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps: int = 5):
     """The 24-run batched ensemble (two-beam plasmas, seed 20260629: side in
     {64..256}, ppc in {16, 64, 256}, vth log-uniform [0.005, 0.05], v0 in
     [0.1, 0.3], dx 0.5, dt 0.1) stepped as one aw_ensemble versus the same
     runs stepped one after another by aw_sim.  Loading is outside both timed
-    regions; each timed region ends with a device synchronisation."""
+    regions; each timed region ends with a device synchronisation.  cpu: the
+    oracle's FP32 ES step (OpenMP) over cpu_steps steps of each of four runs
+    (the first four distinct (side, ppc) shapes in draw order), from their GPU
+    initial states."""
     from arcwarden_amd.ensemble import Ensemble, random_ensemble
     from arcwarden_amd.sim import SimConfig, Simulation
     runs = random_ensemble(dump_every=100)
@@ -156,11 +159,47 @@ def ensemble_bench(steps: int, graph: bool = True):
     info = ens.info()
     ens.close()
     print(f"# ensemble: batched {bat:.3f}s (graph={graph})", file=sys.stderr, flush=True)
+    cpu_out = None
+    if cpu:
+        import numpy as np
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pic_oracle as orc
+        orc.build()
+        threads = min(16, os.cpu_count() or 1)
+        cps, cel, desc, shapes, subset = 0, 0.0, [], set(), []
+        for r in runs:
+            shape = (r.grid.nx, sum(sp.ppc for sp in r.species))
+            if shape not in shapes and len(subset) < 4:
+                shapes.add(shape)
+                subset.append(r)
+        for r in subset:
+            sim = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=0))
+            sim.init()
+            state, rho = sim.particles(), sim.rho()
+            sim.close()
+            g = r.grid
+            og = orc.grid(g.nx, g.ny, g.dx, g.dy)
+            pp = orc.push_params(r.rp.qm, r.rp.dt, (0.0, 0.0, 0.0), False)
+            st = {k: np.ascontiguousarray(state[k], np.float32)
+                  for k in ("x", "y", "ux", "uy", "uz")}
+            rr = np.ascontiguousarray(rho, np.float32)
+            c0 = time.perf_counter()
+            for _ in range(cpu_steps):
+                orc.es_step_f32(og, pp, st, state["w_uniform"], 40, rr, nthreads=threads)
+            cel += time.perf_counter() - c0
+            cps += st["x"].size * cpu_steps
+            desc.append(f"{g.nx}^2 x {sum(sp.ppc for sp in r.species)} ppc")
+        cpu_out = {"value": cps / cel, "unit": "particle-steps/s", "cores": threads,
+                   "kind": "port",
+                   "sample": f"oracle FP32 ES step (solve+push+wrap+deposit), OpenMP x{threads}, "
+                             f"{cpu_steps} steps of each of four runs of the ensemble "
+                             f"({', '.join(desc)}) from their GPU initial states"}
     return {"runs": len(runs), "particles": ntot, "steps": steps,
             "value": ntot * steps / bat, "unit": "particle-steps/s",
             "sequential_value": ntot * steps / seq, "speedup_vs_sequential": seq / bat,
             "graph": bool(graph), "size_groups": info["ngroups"],
             "sort_events": info["sorts"], "run_sorts": info["run_sorts"],
+            "cpu_baseline": cpu_out,
             "workload": "24 two-beam ES runs, seed 20260629, sides 64-256, ppc 16/64/256, "
                         "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
 
@@ -542,7 +581,8 @@ def main():
                  "ms_per_step": r20["elapsed"] / args.steps * 1e3, "sorts": r20["sorts"]}
         print(f"# sort every 20: {fixed['ms_per_step']:.4f} ms/step", file=sys.stderr,
               flush=True)
-    ens = ensemble_bench(args.ensemble_steps) if args.ensemble_steps > 0 else None
+    ens = (ensemble_bench(args.ensemble_steps, cpu=not args.no_cpu)
+           if args.ensemble_steps > 0 else None)
     adj = adjoint_bench(cpu_steps=0 if args.no_cpu else 10) if args.adjoint else None
     ts2 = two_stream_bench(cpu=not args.no_cpu) if args.two_stream else None
     dar = dar_cic = None

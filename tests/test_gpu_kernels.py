@@ -68,6 +68,46 @@ def test_cells_bins_and_sorts_bit_exact(gpu, oracle, nx, ny, n):
             assert np.array_equal(moved[k], st[k][operm])
 
 
+@pytest.mark.parametrize("nx,ny,tx,ty", [(1000, 24, 7, 5), (250, 250, 3, 3), (96, 40, 12, 40),
+                                         (64, 64, 1, 64)])
+def test_tile_sort_odd_tiles_bit_exact(gpu, oracle, nx, ny, tx, ty):
+    """The one-pass tile sort (tile ids from an FP32 reciprocal corrected to
+    the exact quotient) on tile sides that are not powers of two, with
+    particles on both edges of every cell: the same stable permutation and
+    offsets as the oracle's counting sort."""
+    i = np.arange(nx, dtype=np.float32)
+    j = np.arange(ny, dtype=np.float32)
+    lo_x, hi_x = i, np.nextafter(i + 1, np.float32(0)).astype(np.float32)
+    lo_y, hi_y = j, np.nextafter(j + 1, np.float32(0)).astype(np.float32)
+    X, Y = [], []
+    for xs in (lo_x, hi_x):
+        for ys in (lo_y, hi_y):
+            gx, gy = np.meshgrid(xs, ys)
+            X.append(gx.ravel())
+            Y.append(gy.ravel())
+    rng = np.random.default_rng(nx + tx)
+    order = rng.permutation(4 * nx * ny)
+    st = {"x": np.concatenate(X)[order].astype(np.float32),
+          "y": np.concatenate(Y)[order].astype(np.float32)}
+    for k in ("ux", "uy", "uz"):
+        st[k] = rng.uniform(-1, 1, st["x"].size).astype(np.float32)
+    og = oracle.grid(nx, ny, 0.5, 0.5)
+    with Ctx(_grid(nx, ny, 0.5, 0.5), tx, ty) as ctx:
+        assert (ctx.tx, ctx.ty) == (tx, ty)
+        p = DevParticles(st)
+        q = DevParticles.empty(st["x"].size)
+        soff = torch.empty(ctx.ntiles + 1, dtype=torch.int32, device="cuda")
+        L.call("aw_sort_by_tile", ctx.h, C.byref(p.struct()), C.byref(q.struct()),
+               C.c_void_p(soff.data_ptr()), stream())
+        torch.cuda.synchronize()
+        operm, ooff = oracle.counting_sort(oracle.tile_ids(og, tx, ty, st["x"], st["y"]),
+                                           ctx.ntiles)
+        assert np.array_equal(host(soff), ooff)
+        moved = q.numpy()
+        for k in ("x", "y", "ux", "uy", "uz"):
+            assert np.array_equal(moved[k], st[k][operm]), k
+
+
 def test_migrate_bit_exact(gpu, oracle):
     rng = np.random.default_rng(5)
     nx, ny, n = 40, 24, 50_000
