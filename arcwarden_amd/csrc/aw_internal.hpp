@@ -248,6 +248,10 @@ void permute_particles(const aw_particles& src, const aw_particles& dst, const i
                        hipStream_t s);
 bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* off,
                        int32_t* perm, hipStream_t s);
+// Stable bins by tile without moving the particles: perm[pos] = source index
+// in tile order, off[ntiles + 1] (one-pass counting sort when the per-tile
+// counters fit LDS, else the radix sort of tile keys).
+void bin_tiles(aw_ctx& c, const aw_particles& p, int32_t* perm, int32_t* off, hipStream_t s);
 
 // Deposit helpers (deposit.hip)
 // counter: ncounter counters cleared by the same launch (stray counts)

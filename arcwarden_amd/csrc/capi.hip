@@ -810,9 +810,7 @@ aw_status aw_bin_tiles(aw_ctx* ctx, const aw_particles* p, int32_t* off, int32_t
     return guard("aw_bin_tiles", [&] {
         AW_REQUIRE(ctx && off && idx, "ctx/off/idx is NULL");
         check_particles(p);
-        ctx->sort.keyA.reserve((size_t)std::max<int64_t>(1, p->n));
-        compute_tile_keys(*ctx, *p, ctx->sort.keyA.get(), false, S(s));
-        stable_sort_keys(*ctx, p->n, (uint32_t)ctx->ntiles, idx, off, S(s));
+        bin_tiles(*ctx, *p, idx, off, S(s));
     });
 }
 

@@ -783,11 +783,9 @@ int dcu_part_capacity() { return 2 * 256; }
 // the measured max |w| and max |u|.
 
 static void bin_particles(aw_ctx& c, const aw_particles& p, hipStream_t s) {
-    c.sort.keyA.reserve((size_t)p.n);
     c.sort.bin_off.reserve((size_t)c.ntiles + 1);
-    c.sort.bin_perm.reserve((size_t)p.n);
-    compute_tile_keys(c, p, c.sort.keyA.get(), false, s);
-    stable_sort_keys(c, p.n, (uint32_t)c.ntiles, c.sort.bin_perm.get(), c.sort.bin_off.get(), s);
+    c.sort.bin_perm.reserve((size_t)std::max<int64_t>(1, p.n));
+    bin_tiles(c, p, c.sort.bin_perm.get(), c.sort.bin_off.get(), s);
 }
 
 static void measure_scales(aw_ctx& c, const aw_particles& p, double qm, hipStream_t s) {

@@ -211,12 +211,9 @@ void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
             t.apron = c.apron;
         } else {
             // Particles::build_tile_bins (include/pic/particles.hpp:632-671)
-            c.sort.keyA.reserve((size_t)p.n);
             c.sort.bin_off.reserve((size_t)c.ntiles + 1);
-            c.sort.bin_perm.reserve((size_t)p.n);
-            compute_tile_keys(c, p, c.sort.keyA.get(), false, s);
-            stable_sort_keys(c, p.n, (uint32_t)c.ntiles, c.sort.bin_perm.get(),
-                             c.sort.bin_off.get(), s);
+            c.sort.bin_perm.reserve((size_t)std::max<int64_t>(1, p.n));
+            bin_tiles(c, p, c.sort.bin_perm.get(), c.sort.bin_off.get(), s);
             t.off = c.sort.bin_off.get();
             t.idx = c.sort.bin_perm.get();
             t.apron = 0;

@@ -279,12 +279,9 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
             aw_particles old = p;
             old.x = const_cast<float*>(x0);
             old.y = const_cast<float*>(y0);
-            c.sort.keyA.reserve((size_t)p.n);
             c.sort.bin_off.reserve((size_t)c.ntiles + 1);
-            c.sort.bin_perm.reserve((size_t)p.n);
-            compute_tile_keys(c, old, c.sort.keyA.get(), false, s);
-            stable_sort_keys(c, p.n, (uint32_t)c.ntiles, c.sort.bin_perm.get(),
-                             c.sort.bin_off.get(), s);
+            c.sort.bin_perm.reserve((size_t)std::max<int64_t>(1, p.n));
+            bin_tiles(c, old, c.sort.bin_perm.get(), c.sort.bin_off.get(), s);
             EskTiles t{};
             t.off = c.sort.bin_off.get();
             t.idx = c.sort.bin_perm.get();

@@ -330,6 +330,9 @@ struct TsMove {
     int32_t* perm;  // optional: perm[pos] = source index
 };
 
+// MOVE = false: only the permutation (m.perm) is written; the particles are
+// neither moved nor read beyond x, y.
+template <bool MOVE>
 __global__ void __launch_bounds__(kTsWave)
 ts_scatter_kernel(TsMove m, int n, TileMap tm, int ntiles, int rounds, int tbits,
                   const uint32_t* __restrict__ table) {
@@ -350,10 +353,14 @@ ts_scatter_kernel(TsMove m, int n, TileMap tm, int ntiles, int rounds, int tbits
         if (r < nr && i < n) {
             v[0] = m.x[i];
             v[1] = m.y[i];
-            v[2] = m.ux[i];
-            v[3] = m.uy[i];
-            v[4] = m.uz[i];
-            v[5] = m.w ? m.w[i] : 0.f;
+            if constexpr (MOVE) {
+                v[2] = m.ux[i];
+                v[3] = m.uy[i];
+                v[4] = m.uz[i];
+                v[5] = m.w ? m.w[i] : 0.f;
+            } else {
+                v[2] = v[3] = v[4] = v[5] = 0.f;
+            }
         } else {
             for (int q = 0; q < 6; ++q) v[q] = 0.f;
         }
@@ -397,12 +404,14 @@ ts_scatter_kernel(TsMove m, int n, TileMap tm, int ntiles, int rounds, int tbits
             // order, so every lane reads base[t] before its leader advances it
             if (valid) {
                 const uint32_t pos = base[t] + rank;
-                m.xo[pos] = x;
-                m.yo[pos] = y;
-                m.uxo[pos] = ux;
-                m.uyo[pos] = uy;
-                m.uzo[pos] = uz;
-                if (m.wo) m.wo[pos] = w;
+                if constexpr (MOVE) {
+                    m.xo[pos] = x;
+                    m.yo[pos] = y;
+                    m.uxo[pos] = ux;
+                    m.uyo[pos] = uy;
+                    m.uzo[pos] = uz;
+                    if (m.wo) m.wo[pos] = w;
+                }
                 if (m.perm) m.perm[pos] = i;
                 if (rank == 0) base[t] += (uint32_t)__popcll(peers);
             }
@@ -688,10 +697,23 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
     AW_LAUNCH_CHECK();
     TsMove m{src.x, src.y, src.ux, src.uy, src.uz, src.w,
              dst.x, dst.y, dst.ux, dst.uy, dst.uz, dst.w, perm};
-    ts_scatter_kernel<<<nchunks, kTsWave, lds, s>>>(m, n, tm, c.ntiles, rounds, tbits,
-                                                    c.sort.table.get());
+    if (dst.x)
+        ts_scatter_kernel<true><<<nchunks, kTsWave, lds, s>>>(m, n, tm, c.ntiles, rounds, tbits,
+                                                              c.sort.table.get());
+    else
+        ts_scatter_kernel<false><<<nchunks, kTsWave, lds, s>>>(m, n, tm, c.ntiles, rounds,
+                                                               tbits, c.sort.table.get());
     AW_LAUNCH_CHECK();
     return true;
+}
+
+void bin_tiles(aw_ctx& c, const aw_particles& p, int32_t* perm, int32_t* off, hipStream_t s) {
+    AW_REQUIRE(perm && off, "bin_tiles needs perm and off");
+    aw_particles none{};
+    if (tile_sort_onepass(c, p, none, off, perm, s)) return;
+    c.sort.keyA.reserve((size_t)std::max<int64_t>(1, p.n));
+    compute_tile_keys(c, p, c.sort.keyA.get(), false, s);
+    stable_sort_keys(c, p.n, (uint32_t)c.ntiles, perm, off, s);
 }
 
 void permute_particles(const aw_particles& src, const aw_particles& dst, const int32_t* perm,

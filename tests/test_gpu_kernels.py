@@ -324,3 +324,62 @@ def test_reduce_energies_closed_form(gpu):
         assert d.max_e == pytest.approx(0.5, abs=1e-7)
         assert d.charge == pytest.approx(16.0, abs=1e-5)
         assert d.ke == pytest.approx(0.25 * 0.5 * 10 * 0.75, rel=1e-12)
+
+
+def test_every_particle_operator_accepts_empty_input(gpu):
+    """n = 0 through every particle entry point: each returns AW_OK, leaves
+    the grids it accumulates into untouched, and reports zero energy."""
+    nx, ny = 16, 8
+    nc = nx * ny
+    g = _grid(nx, ny, 0.5, 0.5)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, c=5.0, B0=(0.1, 0.0, 0.2))
+    with Ctx(g) as ctx:
+        p = DevParticles.empty(0, w_uniform=1e-3)
+        ps = p.struct()
+        q = DevParticles.empty(0, w_uniform=1e-3)
+        qs = q.struct()
+        grids = {k: torch.full((nc,), 7.0, dtype=torch.float32, device="cuda")
+                 for k in ("rho", "Jx", "Jy", "Jz", "dcux", "dcuy", "dcuz", "amu0", "amu1",
+                           "amu2", "amu3", "jx", "jy", "jz")}
+        fld = {k: torch.full((nc,), 0.01, dtype=torch.float32, device="cuda")
+               for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz")}
+        keep = {k: v.clone() for k, v in grids.items()}
+        x0 = torch.zeros(1, dtype=torch.float32, device="cuda")
+        cell = torch.zeros(1, dtype=torch.int32, device="cuda")
+        off = torch.full((ctx.ntiles + 1,), -1, dtype=torch.int32, device="cuda")
+        s = stream()
+        vp = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        for pol in (L.AW_DEPOSIT_GLOBAL, L.AW_DEPOSIT_LDS_GRID, L.AW_DEPOSIT_BINNED,
+                    L.AW_DEPOSIT_AUTO):
+            L.call("aw_deposit_rho", ctx.h, C.byref(ps), -1.0, pol, vp(grids["rho"]), s)
+            L.call("aw_deposit_esirkepov", ctx.h, vp(x0), vp(x0), C.byref(ps), -1.0, 0.1, pol,
+                   vp(grids["jx"]), vp(grids["jy"]), vp(grids["jz"]), s)
+        L.call("aw_cell_index", ctx.h, C.byref(ps), vp(cell), s)
+        L.call("aw_migrate", ctx.h, C.byref(ps), vp(cell), s)
+        L.call("aw_sort_by_tile", ctx.h, C.byref(ps), C.byref(qs), vp(off), s)
+        L.call("aw_push_es", ctx.h, C.byref(ps), vp(fld["Ex"]), vp(fld["Ey"]), C.byref(rp), 1, s)
+        L.call("aw_half_step_back", ctx.h, C.byref(ps), vp(fld["Ex"]), vp(fld["Ey"]),
+               C.byref(rp), 1, s)
+        strays = C.c_int64(-1)
+        L.call("aw_push_deposit_es", ctx.h, C.byref(ps), vp(off), vp(fld["Ex"]), vp(fld["Ey"]),
+               C.byref(rp), 1, vp(grids["rho"]), C.byref(strays), s)
+        src = L.AwSources()
+        for k in ("rho", "Jx", "Jy", "Jz", "dcux", "dcuy", "dcuz", "amu0", "amu1", "amu2",
+                  "amu3"):
+            setattr(src, k, grids[k].data_ptr())
+        f = L.AwFields()
+        for k, t in fld.items():
+            setattr(f, k, t.data_ptr())
+        L.call("aw_deposit_rho_j_amu", ctx.h, C.byref(ps), -1.0, C.byref(src), s)
+        L.call("aw_deposit_dcu", ctx.h, C.byref(ps), -1.0, C.byref(f), C.byref(src), s)
+        L.call("aw_push_boris_em", ctx.h, C.byref(ps), C.byref(f), C.byref(rp), s)
+        d = L.AwDiagSample()
+        L.call("aw_reduce_energies", ctx.h, C.byref(ps), vp(fld["Ex"]), vp(fld["Ey"]), None,
+               1.0, C.byref(d), s)
+        torch.cuda.synchronize()
+        for k in grids:
+            assert torch.equal(grids[k], keep[k]), k
+        assert np.array_equal(host(off), np.zeros(ctx.ntiles + 1, np.int32))
+        assert strays.value == 0
+        assert d.ke == 0.0 and d.charge == 0.0
+        assert d.ee == pytest.approx(0.5 * 2 * 0.01 ** 2 * nc * 0.25, rel=1e-6)
