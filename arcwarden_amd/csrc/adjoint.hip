@@ -24,7 +24,9 @@
 
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
+#include <utility>
 #include <vector>
 
 namespace aw {
@@ -34,6 +36,10 @@ void solve_es_contig(aw_ctx& c, const float* rho, float* E2, double eps0, hipStr
 int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
                     const float* Ey, const aw_run_params& rp, bool has_b0,
                     unsigned long long* strays, double* ke_part, int bpt, hipStream_t s);
+int push_deposit_es_save(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
+                         const float* Ey, const aw_run_params& rp, bool has_b0,
+                         unsigned long long* strays, double* ke_part, int bpt, float* save_xy,
+                         hipStream_t s);
 
 namespace {
 
@@ -278,6 +284,30 @@ __global__ void scatter4_kernel(const int32_t* __restrict__ perm, int n, const f
     b3[j] = a3[i];
 }
 
+// p[0..nw) = 0 (32-bit words)
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t nw) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw) p[i] = 0u;
+}
+
+__global__ void copy_words_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                  int64_t nw) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nw) dst[i] = src[i];
+}
+
+// dst[q][i] = src[q][i] for nq <= 4 arrays of n floats, one launch
+struct Copy4 {
+    const float* src[4];
+    float* dst[4];
+};
+
+__global__ void copy4_kernel(Copy4 c, int nq, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < nq; ++q) c.dst[q][i] = c.src[q][i];
+}
+
 // per-block partial sums of the parameter gradients: modes m (lx * sin((m+1)
 // k1 xbase)/dx) and per species (lux, luy); the adjoint state in load order
 struct GradArgs {
@@ -355,6 +385,21 @@ struct aw_adjoint {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    void zero_words(void* p, size_t bytes) {
+        const int64_t nw = (int64_t)(bytes / 4);
+        if (!nw) return;
+        zero_words_kernel<<<blocks_for(nw), kThreads, 0, stream>>>(static_cast<uint32_t*>(p), nw);
+        AW_LAUNCH_CHECK();
+    }
+
+    void copy_words(const void* src, void* dst, size_t bytes) {
+        const int64_t nw = (int64_t)(bytes / 4);
+        if (!nw) return;
+        copy_words_kernel<<<blocks_for(nw), kThreads, 0, stream>>>(
+            static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), nw);
+        AW_LAUNCH_CHECK();
+    }
+
     aw_particles parts(int which) const {
         aw_particles p{};
         p.x = P[which][0].get();
@@ -384,15 +429,28 @@ struct aw_adjoint {
 
     void deposit_rho() {  // rho of the current state into the (zeroed) accumulator
         const hipStream_t s = stream;
-        ctx->acc.zero(s);
+        zero_words(ctx->acc.get(), ctx->acc.size() * sizeof(unsigned long long));
         deposit_rho_fixed(*ctx, parts(cur), rp.qm, AW_DEPOSIT_BINNED, off.get(), s);
     }
 
-    void solve_current() {  // rho^n (in ctx->acc) -> rho, E; acc cleared
+    // rho^n (in ctx->acc) -> rho, E2 = [Ex | Ey] (default: E); acc cleared
+    void solve_current(float* E2 = nullptr) {
         ctx->rin.reserve((size_t)ncell(*ctx));
         fix_to_float_add(ctx->acc.get(), rho.get(), ncell(*ctx), ctx->fix_log2, true, stream, true,
                          strays.get(), ctx->rin.get());
-        solve_es_contig(*ctx, ctx->rin.get(), E.get(), rp.eps0, stream);
+        solve_es_contig(*ctx, ctx->rin.get(), E2 ? E2 : E.get(), rp.eps0, stream);
+    }
+
+    void copy_arrays(std::initializer_list<std::pair<const float*, float*>> pairs) {
+        Copy4 c{};
+        int q = 0;
+        for (const auto& [src, dst] : pairs) {
+            c.src[q] = src;
+            c.dst[q] = dst;
+            ++q;
+        }
+        copy4_kernel<<<blocks_for(n), kThreads, 0, stream>>>(c, q, (int)n);
+        AW_LAUNCH_CHECK();
     }
 
     void save_checkpoint(int k) {
@@ -408,17 +466,12 @@ struct aw_adjoint {
                               (size_t)(ctx->ntiles + 1) * 4, hipMemcpyDeviceToDevice, s));
     }
 
-    void load_checkpoint(int k) {
-        const hipStream_t s = stream;
-        const size_t b = (size_t)n * sizeof(float);
+    void load_checkpoint(int k) {  // kernels only: part of the captured sweep
         const float* c = ck.get() + (size_t)k * 4 * n;
         const aw_particles p = parts(cur);
-        AW_HIP(hipMemcpyAsync(p.x, c, b, hipMemcpyDeviceToDevice, s));
-        AW_HIP(hipMemcpyAsync(p.y, c + n, b, hipMemcpyDeviceToDevice, s));
-        AW_HIP(hipMemcpyAsync(p.ux, c + 2 * n, b, hipMemcpyDeviceToDevice, s));
-        AW_HIP(hipMemcpyAsync(p.uy, c + 3 * n, b, hipMemcpyDeviceToDevice, s));
-        AW_HIP(hipMemcpyAsync(off.get(), ck_off.get() + (size_t)k * (ctx->ntiles + 1),
-                              (size_t)(ctx->ntiles + 1) * 4, hipMemcpyDeviceToDevice, s));
+        copy_arrays({{c, p.x}, {c + n, p.y}, {c + 2 * n, p.ux}, {c + 3 * n, p.uy}});
+        copy_words(ck_off.get() + (size_t)k * (ctx->ntiles + 1), off.get(),
+                   (size_t)(ctx->ntiles + 1) * 4);
     }
 
     // one forward step n: E^n from rho^n, fused push + deposit of rho^{n+1}
@@ -434,6 +487,7 @@ struct aw_adjoint {
         const hipStream_t s = stream;
         AW_REQUIRE(steps > 0, "nsteps must be positive");
         nsteps = steps;
+        cur = 0;  // the load below rewrites both buffers: every run starts alike
         nck = (int)((steps + K - 1) / K);
         // load: the displaced state, then the undisplaced x (same draws)
         std::vector<aw_species> sp = species;
@@ -501,8 +555,7 @@ struct aw_adjoint {
         ctx->ktmp.reserve((size_t)6 * nk);
         float2* ak = reinterpret_cast<float2*>(ctx->ktmp.get());
         float2* rk = ak + 2 * nk;
-        fft_exec(*ctx, ctx->r2c, lEf.get(), ak, s);
-        fft_exec(*ctx, ctx->r2c, lEf.get() + nc, ak + nk, s);
+        fft_exec(*ctx, ctx->r2c2, lEf.get(), ak, s);  // [lEx | lEy] -> [ak_x | ak_y]
         KG k{g.nx, g.ny, g.nx / 2 + 1, 6.28318530717958647692 / g.Lx,
              6.28318530717958647692 / g.Ly, rp.eps0};
         es_adj_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(ak, rk, k);
@@ -512,6 +565,13 @@ struct aw_adjoint {
 
     void backward(double* grad_modes, double* grad_ufl) {
         AW_REQUIRE(forward_done, "aw_adj_backward before aw_adj_forward");
+        sweep();
+        gradients(grad_modes, grad_ufl);
+    }
+
+    // seed, then every segment recomputed and swept in reverse; the adjoint
+    // state ends in load order
+    void sweep() {
         const hipStream_t s = stream;
         const int64_t nc = ncell(*ctx);
         const float coef = (float)(rp.qm * w_uniform / (g.dx * g.dy));
@@ -519,10 +579,9 @@ struct aw_adjoint {
         const float a = (float)(rp.qm * rp.dt);
         // seed: lrho^N = S^T (eps0 dx dy E^N), lx = lu = 0 (the seed is kept, so
         // the backward sweep can be repeated)
-        AW_HIP(hipMemcpyAsync(lE.get(), lE0.get(), (size_t)2 * nc * sizeof(double),
-                              hipMemcpyDeviceToDevice, s));
+        copy_words(lE0.get(), lE.get(), (size_t)2 * nc * sizeof(double));
         adj_solve();
-        for (auto* b : {&lx, &ly, &lux, &luy}) AW_HIP(hipMemsetAsync(b->get(), 0, (size_t)n * 4, s));
+        for (auto* b : {&lx, &ly, &lux, &luy}) zero_words(b->get(), (size_t)n * 4);
         segx.reserve((size_t)(K + 1) * 2 * n);
         segE.reserve((size_t)(K + 1) * 2 * nc);
         const size_t b = (size_t)n * sizeof(float);
@@ -535,16 +594,17 @@ struct aw_adjoint {
             for (int64_t st = s0; st <= s1; ++st) {
                 const int64_t j = st - s0;
                 const aw_particles p = parts(cur);
-                AW_HIP(hipMemcpyAsync(segx.get() + (size_t)j * 2 * n, p.x, b,
-                                      hipMemcpyDeviceToDevice, s));
-                AW_HIP(hipMemcpyAsync(segx.get() + (size_t)j * 2 * n + n, p.y, b,
-                                      hipMemcpyDeviceToDevice, s));
-                if (st == s1) break;
-                solve_current();
-                AW_HIP(hipMemcpyAsync(segE.get() + (size_t)j * 2 * nc, E.get(),
-                                      (size_t)2 * nc * 4, hipMemcpyDeviceToDevice, s));
-                push_deposit_es(*ctx, p, off.get(), E.get(), E.get() + nc, rp, false,
-                                strays.get(), nullptr, bpt, s);
+                float* xj = segx.get() + (size_t)j * 2 * n;
+                if (st == s1) {  // the segment's last positions (no push follows)
+                    copy_arrays({{p.x, xj}, {p.y, xj + n}});
+                    break;
+                }
+                // E^n straight into the segment's store, and pushed from there;
+                // the push saves x^n, y^n into the store as it moves them
+                float* Ej = segE.get() + (size_t)j * 2 * nc;
+                solve_current(Ej);
+                push_deposit_es_save(*ctx, p, off.get(), Ej, Ej + nc, rp, false, strays.get(),
+                                     nullptr, bpt, xj, s);
             }
             // the last position of this segment is the first of the next one, in
             // this segment's order: apply the pending D_x^T at x^{s1}
@@ -606,12 +666,14 @@ struct aw_adjoint {
                 ck_perm.get() + (size_t)k * n, (int)n, lx.get(), ly.get(), lux.get(), luy.get(),
                 t, t + n, t + 2 * n, t + 3 * n);
             AW_LAUNCH_CHECK();
-            for (int q = 0; q < 4; ++q) {
-                float* dst = q == 0 ? lx.get() : q == 1 ? ly.get() : q == 2 ? lux.get() : luy.get();
-                AW_HIP(hipMemcpyAsync(dst, t + (size_t)q * n, b, hipMemcpyDeviceToDevice, s));
-            }
+            copy_arrays({{t, lx.get()}, {t + n, ly.get()}, {t + 2 * n, lux.get()},
+                         {t + 3 * n, luy.get()}});
         }
-        // the adjoint state is now in load order: parameter gradients
+    }
+
+    // parameter gradients from the adjoint state (load order)
+    void gradients(double* grad_modes, double* grad_ufl) {
+        const hipStream_t s = stream;
         GradArgs ga{};
         ga.lx = lx.get();
         ga.lux = lux.get();

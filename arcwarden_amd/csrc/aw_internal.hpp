@@ -209,7 +209,7 @@ struct aw_ctx {
     size_t lds_per_block = 0;
     // rocFFT
     rocfft_plan r2c = nullptr, c2r = nullptr, c2r2 = nullptr;
-    rocfft_plan r2c3 = nullptr, c2r3 = nullptr;
+    rocfft_plan r2c2 = nullptr, r2c3 = nullptr, c2r3 = nullptr;
     rocfft_plan r2c4 = nullptr, r2c7 = nullptr, c2r5 = nullptr;  // Darwin batches
     rocfft_plan r2cB = nullptr, c2rB = nullptr;                  // ensemble batches
     int fft_batch = 0;

@@ -77,6 +77,7 @@ struct FusedArgs {
     double qm, area, coef_u, scale;
     unsigned long long* strays;
     double* ke_part;  // per-workgroup sum of w*u^2 (nullptr: skip)
+    float *sx, *sy;   // non-null: receive the positions before the move
 };
 
 __device__ __forceinline__ int local_index(int i, int org, int n) {
@@ -96,7 +97,7 @@ struct TileFrame {
 // wrap, store, deposit rho(x^{n+1}).  UZ = false: B0 is along z (or absent),
 // so uz is a constant of the motion; it is then neither rotated nor stored,
 // and read only for the kinetic energy (KE).
-template <bool PER_W, bool KE, bool UZ>
+template <bool PER_W, bool KE, bool UZ, bool SAVE = false>
 __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f, int k, float x,
                                          float y, float ux, float uy, float uz, float wk,
                                          unsigned int& nstray, double& ke) {
@@ -118,6 +119,10 @@ __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f,
     else dev::boris_es_bz(a.bc, a.has_b0 != 0, ex, ey, ux, uy);
     const float xn = dev::wrap_coord(fmaf(ux, a.cx, x), (float)a.nx);
     const float yn = dev::wrap_coord(fmaf(uy, a.cy, y), (float)a.ny);
+    if constexpr (SAVE) {
+        a.sx[k] = x;
+        a.sy[k] = y;
+    }
     a.x[k] = xn;
     a.y[k] = yn;
     a.ux[k] = ux;
@@ -159,7 +164,7 @@ __device__ __forceinline__ void push_one(const FusedArgs& a, const TileFrame& f,
 
 // One workgroup's share of the fused step: blk is the workgroup index within
 // the run (tile blk / bpt, part blk % bpt).
-template <bool PER_W, bool KE, bool UZ>
+template <bool PER_W, bool KE, bool UZ, bool SAVE = false>
 __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) {
     constexpr bool kLoadUz = UZ || KE;
     extern __shared__ unsigned long long smem[];
@@ -210,13 +215,16 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
     unsigned int nstray = 0;
     double ke = 0.0;
     for (; k + kThreads < r1; k += 2 * kThreads) {
-        push_one<PER_W, KE, UZ>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray, ke);
-        push_one<PER_W, KE, UZ>(a, f, k + kThreads, c1.x, c1.y, c1.ux, c1.uy, c1.uz, c1.w,
-                                nstray, ke);
+        push_one<PER_W, KE, UZ, SAVE>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray,
+                                      ke);
+        push_one<PER_W, KE, UZ, SAVE>(a, f, k + kThreads, c1.x, c1.y, c1.ux, c1.uy, c1.uz, c1.w,
+                                      nstray, ke);
         c0 = fetch(k + 2 * kThreads);
         c1 = fetch(k + 3 * kThreads);
     }
-    if (k < r1) push_one<PER_W, KE, UZ>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray, ke);
+    if (k < r1)
+        push_one<PER_W, KE, UZ, SAVE>(a, f, k, c0.x, c0.y, c0.ux, c0.uy, c0.uz, c0.w, nstray,
+                                      ke);
 
     // per-workgroup stray count and kinetic-energy partial
     __shared__ unsigned int s_stray;
@@ -249,9 +257,9 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
     }
 }
 
-template <bool PER_W, bool KE, bool UZ>
+template <bool PER_W, bool KE, bool UZ, bool SAVE = false>
 __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
-    push_deposit_block<PER_W, KE, UZ>(a, blockIdx.x);
+    push_deposit_block<PER_W, KE, UZ, SAVE>(a, blockIdx.x);
 }
 
 // Ensembles: one launch over every run.  runs[r] holds run r's arguments
@@ -341,14 +349,15 @@ static FusedArgs make_fused(aw_ctx& c, const aw_particles& p, const int32_t* off
     return a;
 }
 
-// Returns the number of workgroups launched (size of ke_part).
-int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
-                    const float* Ey, const aw_run_params& rp, bool has_b0,
-                    unsigned long long* strays, double* ke_part, int bpt, hipStream_t s) {
+// Returns the number of workgroups launched (size of ke_part).  save_xy
+// (2n floats, may be null) receives [x | y] before the move.
+int push_deposit_es_save(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
+                         const float* Ey, const aw_run_params& rp, bool has_b0,
+                         unsigned long long* strays, double* ke_part, int bpt, float* save_xy,
+                         hipStream_t s) {
     if (p.n == 0) return 0;
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
-    const FusedArgs a =
-        make_fused(c, p, off, Ex, Ey, rp, has_b0, c.acc.get(), strays, ke_part, bpt);
+    FusedArgs a = make_fused(c, p, off, Ex, Ey, rp, has_b0, c.acc.get(), strays, ke_part, bpt);
     const size_t sh = (size_t)a.W * a.H * (8 + 4 + 4);
     const int nblk = c.ntiles * a.bpt;
     const bool pw = p.w != nullptr, ke = ke_part != nullptr;
@@ -357,6 +366,13 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
         kernel<<<nblk, kThreads, sh, s>>>(a);
         AW_LAUNCH_CHECK();
     };
+    if (save_xy) {  // the adjoint's recompute: uniform weights, B0-free, no diagnostics
+        AW_REQUIRE(!uz && !pw && !ke, "saving the positions needs the uniform-weight ES push");
+        a.sx = save_xy;
+        a.sy = save_xy + p.n;
+        launch(push_deposit_kernel<false, false, false, true>);
+        return nblk;
+    }
     if (uz) {
         if (pw && ke) launch(push_deposit_kernel<true, true, true>);
         else if (pw) launch(push_deposit_kernel<true, false, true>);
@@ -369,6 +385,12 @@ int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const 
         else launch(push_deposit_kernel<false, false, false>);
     }
     return nblk;
+}
+
+int push_deposit_es(aw_ctx& c, const aw_particles& p, const int32_t* off, const float* Ex,
+                    const float* Ey, const aw_run_params& rp, bool has_b0,
+                    unsigned long long* strays, double* ke_part, int bpt, hipStream_t s) {
+    return push_deposit_es_save(c, p, off, Ex, Ey, rp, has_b0, strays, ke_part, bpt, nullptr, s);
 }
 
 // uz changes in the ES push only through a B0 component in the plane

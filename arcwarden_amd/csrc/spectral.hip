@@ -95,10 +95,11 @@ void init_fft(aw_ctx& c) {
     c.r2c = make_plan(c, true, 1, 1.0);
     c.c2r = make_plan(c, false, 1, inv);
     c.c2r2 = make_plan(c, false, 2, inv);
+    c.r2c2 = make_plan(c, true, 2, 1.0);
     c.r2c3 = make_plan(c, true, 3, 1.0);
     c.c2r3 = make_plan(c, false, 3, inv);
     size_t work = 0;
-    for (rocfft_plan p : {c.r2c, c.c2r, c.c2r2, c.r2c3, c.c2r3}) {
+    for (rocfft_plan p : {c.r2c, c.c2r, c.c2r2, c.r2c2, c.r2c3, c.c2r3}) {
         size_t w = 0;
         AW_FFT(rocfft_plan_get_work_buffer_size(p, &w));
         work = std::max(work, w);
@@ -145,8 +146,8 @@ void enable_em_batch_fft(aw_ctx& c, int R) {
 }
 
 void destroy_fft(aw_ctx& c) noexcept {
-    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7, &c.c2r5,
-                           &c.r2cB, &c.c2rB, &c.r2c4B, &c.c2r5B, &c.r2c7B, &c.c2r3B})
+    for (rocfft_plan* p : {&c.r2c, &c.c2r, &c.c2r2, &c.r2c2, &c.r2c3, &c.c2r3, &c.r2c4, &c.r2c7,
+                           &c.c2r5, &c.r2cB, &c.c2rB, &c.r2c4B, &c.c2r5B, &c.r2c7B, &c.c2r3B})
         if (*p) {
             rocfft_plan_destroy(*p);
             *p = nullptr;

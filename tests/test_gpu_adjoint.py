@@ -90,6 +90,28 @@ def test_checkpoint_interval_changes_nothing(gpu):
         assert np.allclose(gU, out[0][2], rtol=1e-5, atol=1e-9 * np.max(np.abs(out[0][2])))
 
 
+def test_repeated_runs_are_identical(gpu):
+    """forward + backward again on the same object (after runs with other
+    amplitudes and another step count) gives the same loss and gradients bit
+    for bit."""
+    g, rp, sp, A = _setup(seed=9)
+    run = AdjointRun(g, rp, sp, len(A), checkpoint_every=7)
+    l0 = run.forward(A, None, 20)
+    ref = run.backward()
+    ref_sg = run.state_grad()
+    run.forward(-0.7 * A, None, 20)
+    run.backward()
+    run.forward(A, None, 13)
+    run.backward()
+    assert run.forward(A, None, 20) == l0
+    gA, gU = run.backward()
+    assert np.array_equal(gA, ref[0]) and np.array_equal(gU, ref[1])
+    sg = run.state_grad()
+    for k in ("x", "y", "ux", "uy"):
+        assert np.array_equal(sg[k], ref_sg[k]), k
+    run.close()
+
+
 def test_torch_autograd_function(gpu):
     import torch
     g, rp, sp, A = _setup(seed=7)
