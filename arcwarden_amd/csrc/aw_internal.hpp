@@ -284,8 +284,9 @@ int fill_run_slot(void* slot, aw_ctx& c, const aw_particles& p, const int32_t* o
                   unsigned long long* acc, unsigned long long* strays, double* ke_part, int bpt,
                   int wg_begin);
 size_t push_lds_bytes(const aw_ctx& c);
-void push_deposit_multi(const void* slots, int nruns, int total_wg, size_t lds, bool ke, bool uz,
-                        hipStream_t s);
+// wg_run[w] = the run (slot index) of workgroup w of the launch
+void push_deposit_multi(const void* slots, const int32_t* wg_run, int total_wg, size_t lds,
+                        bool ke, bool uz, hipStream_t s);
 
 // Contexts (capi.hip)
 aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y);

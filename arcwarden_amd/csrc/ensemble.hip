@@ -147,6 +147,7 @@ struct aw_ensemble {
     std::vector<hipStream_t> side;         // concurrent per-size solves
     std::vector<hipEvent_t> fork_join;     // [0] fork, [1 + i] join of side[i]
     DevBuf<int32_t> tile_off, perm;
+    DevBuf<int32_t> wg_run;  // workgroup -> run (slot) of the multi-run launch
     DevBuf<unsigned long long> acc, strays;
     DevBuf<float> rho, rin, E;
     DevBuf<double> ke_part, diag_stage, diag_ring;
@@ -285,7 +286,7 @@ struct aw_ensemble {
 
     void enqueue_step(bool dump) {
         convert_and_solve();
-        push_deposit_multi(active.get(), (int)runs.size(), total_wg, lds, dump, uz, stream);
+        push_deposit_multi(active.get(), wg_run.get(), total_wg, lds, dump, uz, stream);
         if (dump) {
             ens_diag_kernel<<<(int)runs.size(), kThreads, 0, stream>>>(
                 diag_runs.get(), ke_part.get(), E.get(), rho.get(), diag_stage.get());
@@ -350,7 +351,7 @@ struct aw_ensemble {
         const bool dump = cfg_dump(nstep);
         if (fields_ready) {
             // the init solve already holds E^0: push only (first step)
-            push_deposit_multi(active.get(), (int)runs.size(), total_wg, lds, dump, uz, s);
+            push_deposit_multi(active.get(), wg_run.get(), total_wg, lds, dump, uz, s);
             if (dump) {
                 ens_diag_kernel<<<(int)runs.size(), kThreads, 0, s>>>(
                     diag_runs.get(), ke_part.get(), E.get(), rho.get(), diag_stage.get());
@@ -528,6 +529,15 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         e->rin.resize((size_t)cb);
         e->E.resize((size_t)2 * cb);
         e->ke_part.resize((size_t)std::max(1, wg));
+        {
+            std::vector<int32_t> h((size_t)std::max(1, wg), 0);
+            for (size_t k = 0; k < e->runs.size(); ++k)
+                std::fill(h.begin() + e->runs[k].wg_begin,
+                          h.begin() + e->runs[k].wg_begin + e->runs[k].nwg, (int32_t)k);
+            e->wg_run.resize(h.size());
+            AW_HIP(hipMemcpy(e->wg_run.get(), h.data(), h.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice));
+        }
         const size_t R = e->runs.size();
         e->diag_stage.resize(4 * R);
         e->diag_ring.resize(4 * R * (size_t)e->diag_cap);

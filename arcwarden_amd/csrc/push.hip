@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
 
 // Ensembles: one launch over every run.  runs[r] holds run r's arguments
 // (pointers into its slices of the concatenated particle / grid arrays) and
-// wg_begin its first workgroup; a workgroup finds its run by binary search.
+// wg_begin its first workgroup; wg_run[blockIdx.x] names a workgroup's run
+// (one load where a search over the runs took several dependent ones).
 struct RunSlot {
     FusedArgs a;
     int wg_begin;
@@ -272,15 +273,10 @@ struct RunSlot {
 
 template <bool KE, bool UZ>
 __global__ void __launch_bounds__(kThreads)
-push_deposit_multi_kernel(const RunSlot* __restrict__ runs, int nruns) {
-    int lo = 0, hi = nruns - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (runs[mid].wg_begin <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const FusedArgs a = runs[lo].a;
-    push_deposit_block<false, KE, UZ>(a, (int)blockIdx.x - runs[lo].wg_begin);
+push_deposit_multi_kernel(const RunSlot* __restrict__ runs, const int32_t* __restrict__ wg_run) {
+    const RunSlot& r = runs[wg_run[blockIdx.x]];
+    const FusedArgs a = r.a;
+    push_deposit_block<false, KE, UZ>(a, (int)blockIdx.x - r.wg_begin);
 }
 
 }  // namespace
@@ -420,16 +416,16 @@ size_t push_lds_bytes(const aw_ctx& c) {
     return (size_t)(c.tx + 2 * c.apron + 1) * (c.ty + 2 * c.apron + 1) * (8 + 4 + 4);
 }
 
-void push_deposit_multi(const void* slots, int nruns, int total_wg, size_t lds, bool ke, bool uz,
-                        hipStream_t s) {
+void push_deposit_multi(const void* slots, const int32_t* wg_run, int total_wg, size_t lds,
+                        bool ke, bool uz, hipStream_t s) {
     if (total_wg == 0) return;
     const RunSlot* r = static_cast<const RunSlot*>(slots);
     if (uz) {
-        if (ke) push_deposit_multi_kernel<true, true><<<total_wg, kThreads, lds, s>>>(r, nruns);
-        else push_deposit_multi_kernel<false, true><<<total_wg, kThreads, lds, s>>>(r, nruns);
+        if (ke) push_deposit_multi_kernel<true, true><<<total_wg, kThreads, lds, s>>>(r, wg_run);
+        else push_deposit_multi_kernel<false, true><<<total_wg, kThreads, lds, s>>>(r, wg_run);
     } else {
-        if (ke) push_deposit_multi_kernel<true, false><<<total_wg, kThreads, lds, s>>>(r, nruns);
-        else push_deposit_multi_kernel<false, false><<<total_wg, kThreads, lds, s>>>(r, nruns);
+        if (ke) push_deposit_multi_kernel<true, false><<<total_wg, kThreads, lds, s>>>(r, wg_run);
+        else push_deposit_multi_kernel<false, false><<<total_wg, kThreads, lds, s>>>(r, wg_run);
     }
     AW_LAUNCH_CHECK();
 }
