@@ -127,19 +127,11 @@ def two_stream_bench(steps: int = 200, cpu: bool = True):
     return out
 
 
-def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps: int = 5):
-    """The 24-run batched ensemble (two-beam plasmas, seed 20260629: side in
-    {64..256}, ppc in {16, 64, 256}, vth log-uniform [0.005, 0.05], v0 in
-    [0.1, 0.3], dx 0.5, dt 0.1) stepped as one aw_ensemble versus the same
-    runs stepped one after another by aw_sim.  Loading is outside both timed
-    regions; each timed region ends with a device synchronisation.  cpu: the
-    oracle's FP32 ES step (OpenMP) over cpu_steps steps of each of four runs
-    (the first four distinct (side, ppc) shapes in draw order), from their GPU
-    initial states."""
-    from arcwarden_amd.ensemble import Ensemble, random_ensemble
+def _ensemble_speedup(runs, steps: int, graph: bool):
+    """Sequential (one aw_sim per run, run after run) and batched (one
+    aw_ensemble) wall time of the same runs over `steps` steps."""
+    from arcwarden_amd.ensemble import Ensemble
     from arcwarden_amd.sim import SimConfig, Simulation
-    runs = random_ensemble(dump_every=100)
-    ntot = sum(r.grid.nx * r.grid.ny * sum(s.ppc for s in r.species) for r in runs)
     seq = 0.0
     for r in runs:
         sim = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=0))
@@ -149,7 +141,6 @@ def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps:
         sim.run(steps)
         seq += time.perf_counter() - t0
         sim.close()
-    print(f"# ensemble: sequential {seq:.3f}s for {steps} steps", file=sys.stderr, flush=True)
     ens = Ensemble(runs, SimConfig(sort_every=0), graph=graph)
     ens.init()
     ens.synchronize()
@@ -158,7 +149,32 @@ def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps:
     bat = time.perf_counter() - t0
     info = ens.info()
     ens.close()
-    print(f"# ensemble: batched {bat:.3f}s (graph={graph})", file=sys.stderr, flush=True)
+    return seq, bat, info
+
+
+def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps: int = 5):
+    """The 24-run batched ensemble (two-beam plasmas, seed 20260629: side in
+    {64..256}, ppc in {16, 64, 256}, vth log-uniform [0.005, 0.05], v0 in
+    [0.1, 0.3], dx 0.5, dt 0.1) stepped as one aw_ensemble versus the same
+    runs stepped one after another by aw_sim.  Loading is outside both timed
+    regions; each timed region ends with a device synchronisation.  cpu: the
+    oracle's FP32 ES step (OpenMP) over cpu_steps steps of each of four runs
+    (the first four distinct (side, ppc) shapes in draw order), from their GPU
+    initial states."""
+    from arcwarden_amd.ensemble import random_ensemble
+    from arcwarden_amd.sim import SimConfig, Simulation
+    runs = random_ensemble(dump_every=100)
+    ntot = sum(r.grid.nx * r.grid.ny * sum(s.ppc for s in r.species) for r in runs)
+    seq, bat, info = _ensemble_speedup(runs, steps, graph)
+    print(f"# ensemble: sequential {seq:.3f}s, batched {bat:.3f}s for {steps} steps "
+          f"(graph={graph})", file=sys.stderr, flush=True)
+    # the same generator restricted to small runs (sides 64-128, ppc 16/64),
+    # where each run alone is launch-bound: what batching gains there
+    small = random_ensemble(dump_every=100, sides=(64, 96, 128), ppcs=(16, 64))
+    nsmall = sum(r.grid.nx * r.grid.ny * sum(s.ppc for s in r.species) for r in small)
+    sseq, sbat, _ = _ensemble_speedup(small, steps, graph)
+    print(f"# small-run ensemble: sequential {sseq:.3f}s, batched {sbat:.3f}s",
+          file=sys.stderr, flush=True)
     cpu_out = None
     if cpu:
         import numpy as np
@@ -200,6 +216,11 @@ def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps:
             "graph": bool(graph), "size_groups": info["ngroups"],
             "sort_events": info["sorts"], "run_sorts": info["run_sorts"],
             "cpu_baseline": cpu_out,
+            "small_runs": {"workload": "the same generator and seed with sides 64-128 and "
+                                       "ppc 16/64 (24 runs)",
+                           "particles": nsmall, "value": nsmall * steps / sbat,
+                           "sequential_value": nsmall * steps / sseq,
+                           "speedup_vs_sequential": sseq / sbat},
             "workload": "24 two-beam ES runs, seed 20260629, sides 64-256, ppc 16/64/256, "
                         "vth log-U[0.005,0.05], v0 U[0.1,0.3], dx 0.5, dt 0.1"}
 
