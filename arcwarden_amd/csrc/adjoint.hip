@@ -441,7 +441,9 @@ struct aw_adjoint {
         solve_es_contig(*ctx, ctx->rin.get(), E2 ? E2 : E.get(), rp.eps0, stream);
     }
 
+    // up to four arrays of n floats, one launch
     void copy_arrays(std::initializer_list<std::pair<const float*, float*>> pairs) {
+        AW_REQUIRE(pairs.size() <= 4, "copy_arrays takes at most four arrays");
         Copy4 c{};
         int q = 0;
         for (const auto& [src, dst] : pairs) {
