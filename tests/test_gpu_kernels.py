@@ -293,7 +293,7 @@ def test_poisson_sine(gpu):
         assert np.max(np.abs(e)) == pytest.approx(1.0, abs=1e-3)
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 64), (96, 48), (512, 512)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (96, 48), (512, 512), (16, 32), (128, 32), (2, 8)])
 def test_solve_es_vs_fp64_oracle(gpu, oracle, nx, ny):
     rng = np.random.default_rng(nx + ny)
     rho = rng.normal(size=(ny, nx)).astype(np.float32)
