@@ -66,6 +66,31 @@ def test_ensemble_matches_single_runs(gpu, graph):
 
 
 @pytest.mark.parametrize("graph", [False, True])
+def test_five_grid_sizes_share_the_solve_streams(gpu, graph):
+    """Five grid sizes: the per-size solves run on the main stream and two side
+    streams, so two streams carry two sizes each; every run still equals the
+    same run stepped alone (rho, E and the particle multiset bit for bit)."""
+    spec = [(16, 16, 0.02, 0.2), (24, 16, 0.01, 0.15), (32, 16, 0.03, 0.25), (40, 16, 0.005, 0.3),
+            (48, 16, 0.04, 0.1), (16, 32, 0.02, 0.12)]
+    runs = [two_beam_run(sd, p, t, v, seed=400 + i) for i, (sd, p, t, v) in enumerate(spec)]
+    nsteps = 9
+    ens = Ensemble(runs, SimConfig(sort_every=4), graph=graph)
+    ens.init()
+    assert ens.info()["ngroups"] == 5
+    ens.run(0, nsteps)
+    for i, r in enumerate(runs):
+        s = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=4))
+        s.init()
+        s.run(nsteps)
+        assert np.array_equal(ens.rho(i), s.rho()), i
+        f, fs = ens.fields(i), s.fields()
+        assert np.array_equal(f["Ex"], fs["Ex"]) and np.array_equal(f["Ey"], fs["Ey"]), i
+        assert np.array_equal(_multiset(ens.particles(i)), _multiset(s.particles())), i
+        s.close()
+    ens.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
 def test_stray_sort_is_per_run(gpu, graph):
     """Only the run whose particles strayed beyond the threshold is re-sorted
     (fast beams: 0.2 cells per step); the slow runs keep their order.  Every
