@@ -69,12 +69,14 @@ def test_cells_bins_and_sorts_bit_exact(gpu, oracle, nx, ny, n):
 
 
 @pytest.mark.parametrize("nx,ny,tx,ty", [(1000, 24, 7, 5), (250, 250, 3, 3), (96, 40, 12, 40),
-                                         (64, 64, 1, 64)])
+                                         (64, 64, 1, 64), (512, 512, 2, 2)])
 def test_tile_sort_odd_tiles_bit_exact(gpu, oracle, nx, ny, tx, ty):
-    """The one-pass tile sort (tile ids from an FP32 reciprocal corrected to
-    the exact quotient) on tile sides that are not powers of two, with
-    particles on both edges of every cell: the same stable permutation and
-    offsets as the oracle's counting sort."""
+    """The one-pass tile sort and tile binning (tile ids from an FP32
+    reciprocal corrected to the exact quotient) on tile sides that are not
+    powers of two, with particles on both edges of every cell: the same
+    stable permutation and offsets as the oracle's counting sort.  512x512
+    with 2x2 tiles has 65,536 tiles, whose counters exceed LDS: the radix
+    sort path."""
     i = np.arange(nx, dtype=np.float32)
     j = np.arange(ny, dtype=np.float32)
     lo_x, hi_x = i, np.nextafter(i + 1, np.float32(0)).astype(np.float32)
@@ -99,10 +101,16 @@ def test_tile_sort_odd_tiles_bit_exact(gpu, oracle, nx, ny, tx, ty):
         soff = torch.empty(ctx.ntiles + 1, dtype=torch.int32, device="cuda")
         L.call("aw_sort_by_tile", ctx.h, C.byref(p.struct()), C.byref(q.struct()),
                C.c_void_p(soff.data_ptr()), stream())
+        boff = torch.empty(ctx.ntiles + 1, dtype=torch.int32, device="cuda")
+        bidx = torch.empty(st["x"].size, dtype=torch.int32, device="cuda")
+        L.call("aw_bin_tiles", ctx.h, C.byref(p.struct()), C.c_void_p(boff.data_ptr()),
+               C.c_void_p(bidx.data_ptr()), stream())
         torch.cuda.synchronize()
         operm, ooff = oracle.counting_sort(oracle.tile_ids(og, tx, ty, st["x"], st["y"]),
                                            ctx.ntiles)
         assert np.array_equal(host(soff), ooff)
+        assert np.array_equal(host(boff), ooff)
+        assert np.array_equal(host(bidx), operm)
         moved = q.numpy()
         for k in ("x", "y", "ux", "uy", "uz"):
             assert np.array_equal(moved[k], st[k][operm]), k
