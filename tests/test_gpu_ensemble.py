@@ -123,6 +123,25 @@ def test_stray_sort_is_per_run(gpu, graph):
     ens.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_single_run_ensemble(gpu, graph):
+    """One run: one grid size, so no side streams and no fork or join; it
+    equals the run stepped alone."""
+    r = two_beam_run(32, 16, 0.02, 0.2, seed=500, dump_every=4)
+    ens = Ensemble([r], SimConfig(sort_every=3), graph=graph)
+    ens.init()
+    ens.run(0, 7)
+    s = Simulation(r.grid, r.rp, r.species, SimConfig(has_b0=False, sort_every=3))
+    s.init()
+    s.run(7)
+    assert ens.info()["nruns"] == 1 and ens.info()["ngroups"] == 1
+    assert np.array_equal(ens.rho(0), s.rho())
+    assert np.array_equal(_multiset(ens.particles(0)), _multiset(s.particles()))
+    assert [h["step"] for h in ens.history(0)] == [h["step"] for h in s.history()]
+    s.close()
+    ens.close()
+
+
 def test_random_ensemble_generator_and_invariants(gpu):
     """The 24-run workload generator (seeded) and a short run of a scaled-down
     draw: charge conserved in every run."""
