@@ -580,6 +580,15 @@ __device__ __forceinline__ void esk_emit(const EskShape& m, double cJx, double c
         }
 }
 
+// The dispatcher hands consecutive workgroup ids to the 8 XCDs in turn.  This
+// bijection of [0, n) gives XCD x a contiguous range of logical workgroups
+// (x * (n/8) + min(x, n%8) onward), so the workgroups of one tile, which
+// read the same field tile and flush into the same cells, share one L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + k;
+}
+
 // Particle range [r0, r1) of workgroup `part` of the bpt workgroups sharing
 // the tile range [beg, end): the split points are rounded up to multiples of
 // 32 particles (128 B of each FP32 array), so neighbouring workgroups never

@@ -259,7 +259,7 @@ __device__ __forceinline__ void push_deposit_block(const FusedArgs& a, int blk) 
 
 template <bool PER_W, bool KE, bool UZ, bool SAVE = false>
 __global__ void __launch_bounds__(kThreads) push_deposit_kernel(FusedArgs a) {
-    push_deposit_block<PER_W, KE, UZ, SAVE>(a, blockIdx.x);
+    push_deposit_block<PER_W, KE, UZ, SAVE>(a, dev::xcd_block(blockIdx.x, gridDim.x));
 }
 
 // Ensembles: one launch over every run.  runs[r] holds run r's arguments
@@ -274,9 +274,12 @@ struct RunSlot {
 template <bool KE, bool UZ>
 __global__ void __launch_bounds__(kThreads)
 push_deposit_multi_kernel(const RunSlot* __restrict__ runs, const int32_t* __restrict__ wg_run) {
-    const RunSlot& r = runs[wg_run[blockIdx.x]];
+    // dispatch order kept: workgroups of different runs differ widely in
+    // cost, and the round-robin over the XCDs balances them
+    const int w = blockIdx.x;
+    const RunSlot& r = runs[wg_run[w]];
     const FusedArgs a = r.a;
-    push_deposit_block<false, KE, UZ>(a, (int)blockIdx.x - r.wg_begin);
+    push_deposit_block<false, KE, UZ>(a, w - r.wg_begin);
 }
 
 }  // namespace
