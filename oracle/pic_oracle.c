@@ -867,6 +867,22 @@ float orc_quiet_u(int64_t t, uint32_t base, double uth, double ufl) {
     return (float)(ufl + 1.41421356237309515 * uth * orc_erfinv(2.0 * q - 1.0));
 }
 
+/* x-displacement modes applied at load: x += sum_m A_m sin(m k1 x), k1 =
+ * 2 pi/nx in cell units, A_m in cells, then the periodic wrap.  This is the
+ * multi-mode form of the reference's single perturbation
+ * (include/pic/particles.hpp:239-245, perturb_amp sin(perturb_kx k1 x)),
+ * evaluated in double and rounded once. */
+void orc_apply_modes_f32(const orc_grid* g, float* x, int64_t n, const double* amp_cells,
+                         int32_t nmodes) {
+    if (nmodes <= 0) return;
+    const double k1 = TWO_PI / (double)g->nx;
+    for (int64_t i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (int32_t m = 0; m < nmodes; ++m) d += amp_cells[m] * sin((m + 1) * k1 * (double)x[i]);
+        x[i] = wrap_coord((float)((double)x[i] + d), (float)g->nx);
+    }
+}
+
 /* species_init_kernel (include/pic/particles.hpp:333-379).  The draw for
  * l = c*ppc + s (t = base + l) is the reference's; the particle is STORED at
  * slot base + s*ncell + c so that a stable tile sort leaves consecutive

@@ -129,6 +129,8 @@ typedef struct {
 } orc_species_init;
 void orc_load_species(const orc_grid* g, const orc_species_init* sp, float* x, float* y,
                       float* ux, float* uy, float* uz);
+void orc_apply_modes_f32(const orc_grid* g, float* x, int64_t n, const double* amp_cells,
+                         int32_t nmodes);
 
 /* ---- diagnostics (double) -------------------------------------------------- */
 void orc_energies(const orc_grid* g, const float* ux, const float* uy, const float* uz,

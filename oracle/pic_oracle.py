@@ -292,9 +292,18 @@ def quiet_u(t, base, uth=1.0, ufl=0.0):
     return lib().orc_quiet_u(t, base, uth, ufl)
 
 
-def load_species(g, specs, noisy, seed):
+def apply_modes(g, x, mode_amp):
+    """x += sum_m A_m sin(m k1 x) with A_m physical (divided by dx into cells),
+    then the periodic wrap; float32 in, float32 out."""
+    x = _f(x).copy()
+    a = _d(np.asarray(mode_amp, np.float64) / g.dx)
+    lib().orc_apply_modes_f32(C.byref(g), _p(x), C.c_int64(x.size), _p(a), C.c_int32(a.size))
+    return x
+
+
+def load_species(g, specs, noisy, seed, mode_amp=None):
     """specs: list of (density, ppc, uth3, ufl3).  Same draw + rank-major storage as
-    the library's loader."""
+    the library's loader; mode_amp (physical amplitudes of modes 1..M) displaces x."""
     nc = g.nx * g.ny
     n = sum(p * nc for _, p, _, _ in specs)
     arrs = {k: np.empty(n, np.float32) for k in ("x", "y", "ux", "uy", "uz")}
@@ -308,6 +317,8 @@ def load_species(g, specs, noisy, seed):
                                                            ("x", "y", "ux", "uy", "uz")))
         weights.append(si.weight)
         base += ppc * nc
+    if mode_amp is not None and len(mode_amp):
+        arrs["x"] = apply_modes(g, arrs["x"], mode_amp)
     return arrs, weights
 
 

@@ -114,6 +114,21 @@ def test_quiet_load_statistics(oracle):
     assert abs(ke - 1.5) < 0.03
 
 
+def test_mode_displacement_closed_form(oracle):
+    """x += sum_m A_m sin(m 2 pi x / Lx): at x = Lx/4 mode 1 moves by A_1 and
+    mode 2 not at all; at x = Lx/8 mode 2 moves by A_2; the result wraps."""
+    g = oracle.grid(64, 8, 0.5, 0.5)
+    x = np.array([16.0, 8.0, 0.0, 48.0], np.float32)
+    out = oracle.apply_modes(g, x, [0.05, 0.0])              # 0.1 cells
+    assert out[0] == np.float32(16.1) and out[2] == 0.0
+    assert out[3] == np.float32(47.9)
+    out = oracle.apply_modes(g, x, [0.0, -0.02])             # -0.04 cells
+    assert out[0] == pytest.approx(16.0, abs=4e-6) and out[1] == np.float32(7.96)
+    amp16 = [0.0] * 15 + [-0.5]                              # mode 16, -1 cell
+    out = oracle.apply_modes(g, np.array([0.5], np.float32), amp16)
+    assert out[0] == pytest.approx(64.0 + 0.5 - math.sin(math.pi / 4), abs=1e-5)
+
+
 def test_diag_golden_csv(oracle):
     """diag_test.csv (reference root): step 7, KE 11.9855, EE 1, total 12.9855,
     charge 16, max_e 0.5 -- quiet load 16x8, ppc 64, vth 1 (tests/test_diagnostics.cu).
@@ -411,6 +426,156 @@ def test_em_boris_f32_matches_f64_and_rotation_invariants(oracle, B):
     assert np.max(np.abs(u2_a - u2_0) / u2_0) < 1e-6      # pure rotation conserves |u|
     for k in ("ux", "uy", "uz"):
         assert np.max(np.abs(a[k] - b[k])) < 1e-6
+
+
+# ---- EM push: the E kick (include/pic/pusher.hpp:85-109,211-258) ----------------------
+
+def _uniform3(shape, v):
+    return [np.full(shape, c, np.float32) for c in v]
+
+
+@pytest.mark.parametrize("flavour", ["f32", "f64"])
+def test_em_push_field_free_acceleration(oracle, flavour):
+    """B = 0: u = qm E t in all three components, each from its own grid."""
+    g = oracle.grid(64, 64, 1.0, 1.0)
+    E = (0.3, -0.2, 0.45)
+    E3 = _uniform3((64, 64), E)
+    B3 = _uniform3((64, 64), (0, 0, 0))
+    s = _one(30.25, 30.5, 0.0, 0.0, 0.0)
+    push = oracle.push_em_f32 if flavour == "f32" else oracle.push_em_f64
+    dt, N = 0.01, 100
+    for _ in range(N):
+        s = push(g, -1.0, dt, s, E3, B3)
+    for k, e in zip(("ux", "uy", "uz"), E):
+        assert float(s[k][0]) == pytest.approx(-e * N * dt, abs=2e-6)
+
+
+@pytest.mark.parametrize("flavour", ["f32", "f64"])
+def test_em_push_parallel_kick_along_oblique_b(oracle, flavour):
+    """E parallel to an oblique B: u_par = qm |E| t, u_perp stays zero."""
+    g = oracle.grid(64, 64, 1.0, 1.0)
+    bhat = np.array([0.48, -0.6, 0.64])                      # unit vector
+    B3 = _uniform3((64, 64), 1.7 * bhat)
+    E3 = _uniform3((64, 64), 0.05 * bhat)
+    s = _one(20.5, 40.5, 0.0, 0.0, 0.0)
+    push = oracle.push_em_f32 if flavour == "f32" else oracle.push_em_f64
+    dt, N = 0.05, 200
+    for _ in range(N):
+        s = push(g, -1.0, dt, s, E3, B3)
+    u = np.array([float(s[k][0]) for k in ("ux", "uy", "uz")])
+    upar = u @ bhat
+    assert upar == pytest.approx(-0.05 * N * dt, rel=1e-5)
+    assert np.linalg.norm(u - upar * bhat) < 1e-5
+
+
+@pytest.mark.parametrize("flavour", ["f32", "f64"])
+def test_em_push_exb_drift_is_a_fixed_point(oracle, flavour):
+    """E perpendicular to B, u = E x B / B^2: Boris leaves the drift velocity
+    unchanged (E + u x B = 0 is its fixed-point condition) and the particle
+    moves at that velocity.  A sign slip in any kick or in the gathered
+    components breaks the fixed point."""
+    g = oracle.grid(128, 128, 0.5, 0.5)
+    B = np.array([0.3, -0.8, 1.1])
+    E = np.cross(np.array([0.2, 0.5, -0.1]), B) * 0.05       # E . B = 0
+    Ef, Bf = (a.astype(np.float32).astype(np.float64) for a in (E, B))
+    vd = np.cross(Ef, Bf) / (Bf @ Bf)
+    s = _one(60.0, 61.0, *vd)
+    push = oracle.push_em_f32 if flavour == "f32" else oracle.push_em_f64
+    dt, N = 0.1, 200
+    for _ in range(N):
+        s = push(g, -1.0, dt, s, _uniform3((128, 128), E), _uniform3((128, 128), B))
+    u = np.array([float(s[k][0]) for k in ("ux", "uy", "uz")])
+    # f64: the FP32 rounding of the fields and of u0 (a gyration of ~1e-9)
+    tol = 1e-5 if flavour == "f32" else 5e-9
+    assert np.max(np.abs(u - vd)) < tol
+    assert float(s["x"][0]) == pytest.approx(60.0 + vd[0] * N * dt / 0.5, abs=2e-3)
+    assert float(s["y"][0]) == pytest.approx(61.0 + vd[1] * N * dt / 0.5, abs=2e-3)
+
+
+# ---- amu and dcu deposits (include/pic/depositor.hpp:225-324) -------------------------
+
+# CIC split of a particle at (2.3, 3.7): cell (i, j) -> weight
+_CORNERS = {(2, 3): 0.7 * 0.3, (3, 3): 0.3 * 0.3, (2, 4): 0.7 * 0.7, (3, 4): 0.3 * 0.7}
+
+
+def _corner_grid(value_of_weight):
+    out = np.zeros((8, 8))
+    for (i, j), w in _CORNERS.items():
+        out[j, i] = value_of_weight(w)
+    return out
+
+
+def test_amu_single_particle_corner_split(oracle):
+    """amu0 = (vx^2 - vy^2)/2, amu1 = vx vy, amu2 = vz vx, amu3 = vz vy, times
+    q w / (dx dy) and the CIC weight, for u = (0.3, -0.5, 0.7)."""
+    g = oracle.grid(8, 8, 0.5, 0.5)
+    s = _one(2.3, 3.7, 0.3, -0.5, 0.7)
+    amu = oracle.deposit_amu_f64(g, s, -1.0, w=np.array([0.2], np.float32))
+    coef = -1.0 * 0.2 / 0.25
+    for a, val in zip(amu, (-0.08, -0.15, 0.21, -0.35)):
+        np.testing.assert_allclose(a, _corner_grid(lambda w: coef * val * w), rtol=2e-6,
+                                   atol=1e-9)
+
+
+def test_amu_of_cold_beam_is_rho_times_flux(oracle):
+    """All particles at one velocity V: amu_k = rho * a_k(V) in every cell, and
+    its mean is q n a_k(V)."""
+    g = oracle.grid(32, 16, 0.5, 0.5)
+    V = (0.25, -0.15, 0.4)
+    st, (w,) = oracle.load_species(g, [(0.8, 16, (0, 0, 0), V)], False, 3)
+    rho, *_ = oracle.deposit_rho_j_f64(g, st, -1.0, w_uniform=w)
+    amu = oracle.deposit_amu_f64(g, st, -1.0, w_uniform=w)
+    vx, vy, vz = (float(np.float32(v)) for v in V)
+    for a, ak in zip(amu, (0.5 * (vx * vx - vy * vy), vx * vy, vz * vx, vz * vy)):
+        np.testing.assert_allclose(a, rho * ak, rtol=1e-12, atol=1e-15)
+        assert a.mean() == pytest.approx(-0.8 * ak, rel=1e-12)
+
+
+def test_amu_of_anisotropic_maxwellian(oracle):
+    """Quiet Maxwellian at rest with (vth_x, vth_y, vth_z) = (0.2, 0.1, 0.05):
+    mean amu0 = q n (vth_x^2 - vth_y^2)/2 (the pressure anisotropy); the
+    off-diagonal components average to zero."""
+    g = oracle.grid(32, 32, 0.5, 0.5)
+    st, (w,) = oracle.load_species(g, [(1.0, 64, (0.2, 0.1, 0.05), (0, 0, 0))], False, 0)
+    amu = oracle.deposit_amu_f64(g, st, -1.0, w_uniform=w)
+    assert amu[0].mean() == pytest.approx(-0.5 * (0.04 - 0.01), rel=0.01)
+    for a in amu[1:]:
+        assert abs(a.mean()) < 1e-4
+
+
+def test_dcu_single_particle_gathers_linear_field(oracle):
+    """A particle at rest at (2.3, 3.7) in E_x = 0.1 i + 0.01 j (nodes)
+    gathers E_x = 0.267 and deposits q^2 w/(dx dy) E W at its corners."""
+    g = oracle.grid(8, 8, 0.5, 0.5)
+    jj, ii = np.mgrid[0:8, 0:8]
+    ex = (0.1 * ii + 0.01 * jj).astype(np.float32)
+    z = np.zeros((8, 8), np.float32)
+    s = _one(2.3, 3.7, 0.0, 0.0, 0.0)
+    dx, dy, dz = oracle.deposit_dcu_f64(g, s, -1.0, (ex, z, z), (z, z, z),
+                                        w=np.array([0.2], np.float32))
+    coef = 0.2 / 0.25
+    np.testing.assert_allclose(dx, _corner_grid(lambda w: coef * 0.267 * w), rtol=1e-6,
+                               atol=1e-9)
+    assert not dy.any() and not dz.any()
+
+
+def test_dcu_uniform_fields_closed_form(oracle):
+    """Uniform E, B and a cold beam at velocity v: dcu = (q/m) rho (E + v x B)
+    in every cell; particles at rest give (q/m) rho E."""
+    g = oracle.grid(32, 16, 0.5, 0.5)
+    E = np.array([0.3, -0.2, 0.1])
+    B = np.array([0.4, 0.5, -0.6])
+    E3 = _uniform3((16, 32), E)
+    B3 = _uniform3((16, 32), B)
+    for V in ((0.0, 0.0, 0.0), (0.2, -0.1, 0.3)):
+        st, (w,) = oracle.load_species(g, [(0.7, 16, (0, 0, 0), V)], False, 1)
+        rho, *_ = oracle.deposit_rho_j_f64(g, st, -1.0, w_uniform=w)
+        dcu = oracle.deposit_dcu_f64(g, st, -1.0, E3, B3, w_uniform=w)
+        v = np.array([float(np.float32(c)) for c in V])
+        force = E.astype(np.float32).astype(np.float64) + np.cross(
+            v, B.astype(np.float32).astype(np.float64))
+        for k in range(3):
+            np.testing.assert_allclose(dcu[k], -1.0 * rho * force[k], rtol=1e-6, atol=1e-12)
 
 
 # ---- Esirkepov (charge-conserving current) -------------------------------------------
