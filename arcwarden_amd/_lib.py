@@ -16,6 +16,8 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libarcwarden_hip.so"
 
 AW_OK = 0
+AW_ERR_INVALID, AW_ERR_HIP, AW_ERR_FFT, AW_ERR_NOMEM, AW_ERR_UNSUPPORTED, AW_ERR_STATE = \
+    1, 2, 3, 4, 5, 6
 AW_DEPOSIT_GLOBAL, AW_DEPOSIT_LDS_GRID, AW_DEPOSIT_BINNED, AW_DEPOSIT_AUTO = 0, 1, 2, 3
 AW_CURRENT_CIC, AW_CURRENT_ESIRKEPOV = 0, 1
 AW_J_STAGGERED = 1
@@ -67,6 +69,25 @@ class AwSpecies(C.Structure):
                 ("uth", C.c_double * 3), ("ufl", C.c_double * 3), ("deltaf", C.c_int32),
                 ("dist", C.c_int32), ("lc_rho", C.c_double), ("lc_kappa", C.c_double),
                 ("taud", C.c_double), ("wdnoise", C.c_double)]
+
+
+class AwYeeMesh(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz",
+                                          "vcy", "vcz")]
+
+
+class AwDeltafState(C.Structure):
+    _fields_ = [("wd", C.c_void_p), ("wc", C.c_void_p), ("wdd", C.c_void_p)]
+
+
+class AwCheckpointHeader(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("rng_seed", C.c_uint64), ("step", C.c_int64),
+                ("time", C.c_double), ("sd", C.c_double), ("eps_l", C.c_double)]
+
+
+class AwCheckpointArray(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("dtype", C.c_uint32), ("count", C.c_uint64),
+                ("data", C.c_void_p), ("on_device", C.c_int32)]
 
 
 class AwParticles(C.Structure):
@@ -186,7 +207,44 @@ _SIGS = {
     "aw_sim_sources": (C.c_int32, [P, PP(AwSources)]),
     "aw_sim_history": (C.c_int32, [P, PP(AwDiagSample), C.c_int64, PP(C.c_int64)]),
     "aw_sim_diag_now": (C.c_int32, [P, C.c_int64, PP(AwDiagSample)]),
+    "aw_phase_hist": (C.c_int32, [P, PP(AwParticles), PP(AwGrid), C.c_int32, C.c_int32,
+                                  C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_int32, P, P]),
+    "aw_sum_sq": (C.c_int32, [P, C.c_int64, C.c_double, PP(C.c_double), P]),
+    "aw_gather_f32": (C.c_int32, [P, P, C.c_int64, P, P]),
+    "aw_device_synchronize": (C.c_int32, []),
+    "aw_checkpoint_write": (C.c_int32, [C.c_char_p, PP(AwCheckpointHeader), C.c_char_p,
+                                        C.c_char_p, PP(AwCheckpointArray), C.c_int32]),
+    "aw_checkpoint_open": (C.c_int32, [C.c_char_p, PP(P)]),
+    "aw_checkpoint_close": (C.c_int32, [P]),
+    "aw_checkpoint_info": (C.c_int32, [P, PP(AwCheckpointHeader), PP(C.c_int32),
+                                       PP(C.c_int64), PP(C.c_int64)]),
+    "aw_checkpoint_text": (C.c_int32, [P, C.c_int32, C.c_char_p, C.c_int64]),
+    "aw_checkpoint_array_info": (C.c_int32, [P, C.c_int32, C.c_char_p, C.c_int64,
+                                             PP(C.c_uint32), PP(C.c_uint64), PP(C.c_int32)]),
+    "aw_checkpoint_read_array": (C.c_int32, [P, C.c_int32, P, C.c_int32]),
+    "aw_sim_checkpoint_save": (C.c_int32, [P, C.c_char_p, C.c_int64, C.c_double, C.c_char_p,
+                                           C.c_char_p]),
+    "aw_sim_checkpoint_restore": (C.c_int32, [P, C.c_char_p]),
+    "aw_maxwell_create": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(P)]),
+    "aw_maxwell_destroy": (C.c_int32, [P]),
+    "aw_maxwell_stream": (C.c_int32, [P, PP(P)]),
+    "aw_maxwell_synchronize": (C.c_int32, [P]),
+    "aw_maxwell_mesh": (C.c_int32, [P, PP(AwYeeMesh)]),
+    "aw_maxwell_step": (C.c_int32, [P, PP(AwParticles), PP(AwDeltafState)]),
+    "aw_maxwell_step_at": (C.c_int32, [P, PP(AwParticles), PP(AwDeltafState), C.c_double]),
+    "aw_maxwell_field_energy": (C.c_int32, [P, PP(C.c_double), PP(C.c_double)]),
+    "aw_maxwell_residuals": (C.c_int32, [P, PP(AwParticles), P, PP(C.c_double),
+                                         PP(C.c_double)]),
+    "aw_maxwell_diag_fields": (C.c_int32, [P, C.c_double, PP(AwFields)]),
+    "aw_wd_stats": (C.c_int32, [P, C.c_int64, PP(C.c_double), PP(C.c_double), PP(C.c_double),
+                                P]),
+    "aw_mirror_count": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(AwSpecies),
+                                    PP(C.c_int64)]),
+    "aw_load_mirror": (C.c_int32, [PP(AwGrid), PP(AwRunParams), PP(AwSpecies),
+                                   PP(AwParticles), P]),
     "aw_sim_request_sort": (C.c_int32, [P]),
+    "aw_sim_particles_changed": (C.c_int32, [P]),
     "aw_sim_set_profiling": (C.c_int32, [P, C.c_int32]),
     "aw_sim_kernel_time": (C.c_int32, [P, PP(C.c_double), PP(C.c_int64), PP(C.c_int64)]),
     "aw_sim_exec_config": (C.c_int32, [P, PP(C.c_int32), PP(C.c_int32), PP(C.c_int32)]),

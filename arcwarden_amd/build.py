@@ -80,8 +80,8 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB.relative_to(ROOT)} from {len(sources)} sources", flush=True)
-    for src, exe in PROGRAMS:
-        _link_program(src, exe)
+    for src, exe, extra in PROGRAMS:
+        _link_program(src, exe, extra)
         if verbose:
             print(f"built {exe.relative_to(ROOT)}", flush=True)
     return LIB
@@ -89,15 +89,18 @@ def build(verbose: bool = True) -> Path:
 
 # C++ programs on the facade (include/pic) linked against the library; the
 # run-path is relative to the executable so the tree can move.
+# (source, executable, extra flags): arcsim and run_deck are one runner with
+# two command lines.
 PROGRAMS = [
-    (PKG / "tools" / "run_deck.cpp", PKG / "tools" / "run_deck"),
-    (ROOT / "tests" / "cpp" / "facade_check.cpp", ROOT / "tests" / "cpp" / "facade_check"),
+    (PKG / "tools" / "arcsim.cpp", PKG / "tools" / "arcsim", []),
+    (PKG / "tools" / "arcsim.cpp", PKG / "tools" / "run_deck", ["-DARC_RUNNER_RUN_DECK=1"]),
+    (ROOT / "tests" / "cpp" / "facade_check.cpp", ROOT / "tests" / "cpp" / "facade_check", []),
 ]
 
 
-def _link_program(src: Path, exe: Path) -> None:
+def _link_program(src: Path, exe: Path, extra: list) -> None:
     rel = os.path.relpath(PKG, exe.parent)
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-Wall",
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++20", "-Wall", *extra,
            f"-I{ROOT / 'include'}", f"-I{PKG / 'include'}", str(src), "-o", str(exe),
            f"-L{PKG}", "-larcwarden_hip", f"-Wl,-rpath,$ORIGIN/{rel}",
            "-Wl,-rpath,/opt/rocm/lib"]

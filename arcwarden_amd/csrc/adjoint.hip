@@ -20,6 +20,14 @@
 // and the adjoint state follows the sort permutations back.  Gradients with
 // respect to the x-displacement mode amplitudes and the species drifts are
 // reductions of the adjoint of the initial state.
+//
+// The adjoint field lE^n = gather^T le is summed in int64 fixed point, like
+// rho, so the sweep returns the same bits on every run.  Its scale 2^e is a
+// pure function of order-free maxima: the largest |component| of the adjoint
+// particle state entering the step (each kernel that writes the state
+// atomicMax-es the FP32 bit pattern of its outputs' maximum) and the largest
+// |Re|+|Im| of the lrho spectrum.  From them the step bounds |le| by B, and e
+// is chosen so that n * B * 2^e < 2^62: no cell sum can wrap.
 #include "aw_internal.hpp"
 
 #include <cmath>
@@ -55,10 +63,11 @@ struct KG {
 // S^T in k-space: lrho_k = i G (kx lEx_k + ky lEy_k), G = 1/(eps0 k^2);
 // DC and Nyquist zero (the transpose of es_kspace_kernel).
 __global__ void es_adj_kspace_kernel(const float2* __restrict__ ak, float2* __restrict__ rk,
-                                     KG k) {
+                                     KG k, unsigned int* __restrict__ rmax) {
     const int64_t nk = (int64_t)k.nkx * k.ny;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nk) return;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t0 < nk;
+    const int64_t t = live ? t0 : 0;  // lanes past the end recompute mode 0 and store nothing
     const int i = (int)(t % k.nkx), j = (int)(t / k.nkx);
     const int m = (j <= k.ny / 2) ? j : j - k.ny;
     const double kx = i * k.dkx, ky = m * k.dky;
@@ -71,28 +80,49 @@ __global__ void es_adj_kspace_kernel(const float2* __restrict__ ak, float2* __re
         r.x = (float)(-g * (kx * (double)ax.y + ky * (double)ay.y));
         r.y = (float)(g * (kx * (double)ax.x + ky * (double)ay.x));
     }
-    rk[t] = r;
+    if (live) rk[t] = r;
+    float mx = live ? fabsf(r.x) + fabsf(r.y) : 0.f;
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(rmax, __float_as_uint(mx));
 }
 
-// double adjoint-field accumulators -> float FFT input; clears the accumulators
-__global__ void lE_convert_kernel(double* __restrict__ acc, float* __restrict__ out, int64_t n,
-                                  float scale) {
+// Order-free bounds of one reverse step (FP32 bit patterns of non-negative
+// maxima, unsigned order = value order): smax[2] alternate between steps (the
+// state entering step st is bounded by smax[(st+1)&1], the step's outputs go
+// to smax[st&1]); rmax bounds the lrho spectrum; e is the step's exponent.
+struct AdjBound {
+    unsigned int smax[2];
+    unsigned int rmax;
+    int e;
+};
+
+// int64 adjoint-field accumulators (scale 2^e) -> float FFT input; clears the
+// accumulators, the state slot the finished step read, and the spectrum bound
+// that the following k-space pass rewrites
+__global__ void lE_convert_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
+                                  int64_t n, AdjBound* __restrict__ bd, int free_slot) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) {
+        bd->smax[free_slot] = 0u;
+        bd->rmax = 0u;
+    }
     if (c >= n) return;
-    out[c] = (float)(acc[c] * (double)scale);
-    acc[c] = 0.0;
+    const double inv = ldexp(1.0, -bd->e);
+    out[c] = (float)((double)(long long)acc[c] * inv);
+    acc[c] = 0ull;
 }
 
-// lE = eps0 dx dy E (the loss seed) and the per-block partial sums of |E|^2
+// lE = eps0 dx dy E (the loss seed, as the FFT's FP32 input) and the
+// per-block partial sums of |E|^2
 __global__ void loss_kernel(const float* __restrict__ E2, int64_t nc, double w,
-                            double* __restrict__ lE, double* __restrict__ part) {
+                            float* __restrict__ lE, double* __restrict__ part) {
     __shared__ double sh[kThreads];
     double s = 0.0;
     for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < 2 * nc;
          c += (int64_t)gridDim.x * kThreads) {
         const double v = E2[c];
         s += v * v;
-        lE[c] = w * v;
+        lE[c] = (float)(w * v);
     }
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -112,7 +142,11 @@ struct AdjArgs {
     const float* Ex;   // E^n
     const float* Ey;
     const float* lrho;  // lrho^{n+1} (with xb)
-    double* lE;         // [lEx | lEy] accumulators of step n
+    unsigned long long* lE;  // [lEx | lEy] int64 accumulators of step n
+    AdjBound* bd;       // bounds: state entering the step, lrho spectrum; e out
+    int slot_in;        // smax slot bounding the entry state; outputs go to slot_in ^ 1
+    float rfac;         // |lrho(x)| <= rfac * max(|Re|+|Im|) of its spectrum
+    int log2n;          // ceil(log2(n))
     const int32_t* off;
     int ntx, tx, ty, W, H, bpt;
     int nx, ny, n;
@@ -172,21 +206,54 @@ __device__ __forceinline__ void grad4(const Cic& c, const float v[4], float& dx,
 
 // One reverse step n (with the D_x^T part of step n+1 when xb is set), over
 // a tile's particles: LDS holds E^n, lrho^{n+1} and the lE^n accumulators.
+__device__ __forceinline__ float max4abs(float a, float b, float c, float d) {
+    return fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d)));
+}
+
+// block-wide max of non-negative v, one atomicMax per wavefront
+__device__ __forceinline__ void wave_max_to(float v, unsigned int* dst) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
+}
+
+// Exponent e of the step's fixed-point scale: every contribution le * W is
+// bounded by B = |a| (U + c (U + |coef| 2 R)), U the entry-state bound and R
+// the lrho bound (2 R bounds a CIC gradient of lrho), and e makes
+// 2^log2n * B * 2^e < 2^62.  The FP32 rounding of the products stays inside
+// the 1e-3 margin.
+__device__ __forceinline__ int adj_exponent(const AdjArgs& a) {
+    const double U = (double)__uint_as_float(a.bd->smax[a.slot_in]);
+    const double R = a.xb ? (double)__uint_as_float(a.bd->rmax) * (double)a.rfac : 0.0;
+    const double c = fmax(fabs((double)a.cx), fabs((double)a.cy));
+    const double B =
+        fabs((double)a.a_kick) * (U + c * (U + fabs((double)a.coef) * 2.0 * R)) * (1.0 + 1e-3);
+    if (!(B > 0.0) || !(B < 1e300)) return 0;
+    int k;
+    (void)frexp(B, &k);  // B < 2^k
+    const int e = 62 - a.log2n - k;
+    return e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
+}
+
+// One reverse step n (with the D_x^T part of step n+1 when xb is set), over
+// a tile's particles: LDS holds E^n, lrho^{n+1} and the int64 lE^n tiles.
 __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
-    extern __shared__ double smem[];
+    extern __shared__ unsigned long long smem[];
     const int W = a.W, H = a.H, R = W * H;
-    double* s_lex = smem;
-    double* s_ley = smem + R;
+    unsigned long long* s_lex = smem;
+    unsigned long long* s_ley = smem + R;
     float* s_ex = reinterpret_cast<float*>(smem + 2 * R);
     float* s_ey = s_ex + R;
     float* s_lr = s_ey + R;
     const int tile = blockIdx.x / a.bpt, part = blockIdx.x % a.bpt;
     const int orgx = (tile % a.ntx) * a.tx - 2, orgy = (tile / a.ntx) * a.ty - 2;
     const int64_t nc = (int64_t)a.nx * a.ny;
+    const int e = adj_exponent(a);
+    const double scale = ldexp(1.0, e);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.bd->e = e;
     for (int c = threadIdx.x; c < R; c += kThreads) {
         const int gc = dev::wrapfar(orgy + c / W, a.ny) * a.nx + dev::wrapfar(orgx + c % W, a.nx);
-        s_lex[c] = 0.0;
-        s_ley[c] = 0.0;
+        s_lex[c] = 0ull;
+        s_ley[c] = 0ull;
         s_ex[c] = a.Ex[gc];
         s_ey[c] = a.Ey[gc];
         s_lr[c] = a.xb ? a.lrho[gc] : 0.f;
@@ -195,6 +262,7 @@ __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
     const int beg = a.off[tile], end = a.off[tile + 1];
     const int chunk = (end - beg + a.bpt - 1) / a.bpt;
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
+    float smax = 0.f;
     for (int p = r0 + (int)threadIdx.x; p < r1; p += kThreads) {
         float lx = a.lx[p], ly = a.ly[p], lux = a.lux[p], luy = a.luy[p];
         if (a.xb) {  // D_x^T lrho^{n+1} at x^{n+1}
@@ -220,17 +288,21 @@ __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
         a.ly[p] = ly;
         a.lux[p] = lux;
         a.luy[p] = luy;
-        // gather^T: lE += le W
+        smax = fmaxf(smax, max4abs(lx, ly, lux, luy));
+        // gather^T: lE += le W, quantised: rint(le * W * 2^e) (exact FP64
+        // product of the FP32 factors, one rounding)
         const float gx = 1.f - c.fx, gy = 1.f - c.fy;
         const float w4[4] = {gx * gy, c.fx * gy, gx * c.fy, c.fx * c.fy};
+        const double q0 = dev::clamp_fix_f64((double)le0 * scale);
+        const double q1 = dev::clamp_fix_f64((double)le1 * scale);
         const int li = local(c.i0, orgx, a.nx), lj = local(c.j0, orgy, a.ny);
         if (dev::in_tile(li, W - 2) && dev::in_tile(lj, H - 2)) {
             const int b = lj * W + li;
             const int idx[4] = {b, b + 1, b + W, b + W + 1};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                atomicAdd(&s_lex[idx[k]], (double)(le0 * w4[k]));
-                atomicAdd(&s_ley[idx[k]], (double)(le1 * w4[k]));
+                atomicAdd(&s_lex[idx[k]], dev::fix_mul(q0, (double)w4[k]));
+                atomicAdd(&s_ley[idx[k]], dev::fix_mul(q1, (double)w4[k]));
             }
         } else {
             int i0, i1, j0, j1;
@@ -239,26 +311,29 @@ __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
             const int idx[4] = {j0 * a.nx + i0, j0 * a.nx + i1, j1 * a.nx + i0, j1 * a.nx + i1};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                unsafeAtomicAdd(&a.lE[idx[k]], (double)(le0 * w4[k]));
-                unsafeAtomicAdd(&a.lE[nc + idx[k]], (double)(le1 * w4[k]));
+                atomicAdd(&a.lE[idx[k]], dev::fix_mul(q0, (double)w4[k]));
+                atomicAdd(&a.lE[nc + idx[k]], dev::fix_mul(q1, (double)w4[k]));
             }
         }
     }
+    wave_max_to(smax, &a.bd->smax[a.slot_in ^ 1]);
     __syncthreads();
     for (int c = threadIdx.x; c < R; c += kThreads) {
         const int gc = dev::wrapfar(orgy + c / W, a.ny) * a.nx + dev::wrapfar(orgx + c % W, a.nx);
-        if (s_lex[c] != 0.0) unsafeAtomicAdd(&a.lE[gc], s_lex[c]);
-        if (s_ley[c] != 0.0) unsafeAtomicAdd(&a.lE[nc + gc], s_ley[c]);
+        if (s_lex[c]) atomicAdd(&a.lE[gc], s_lex[c]);
+        if (s_ley[c]) atomicAdd(&a.lE[nc + gc], s_ley[c]);
     }
 }
 
 // lx += coef D_x^T lrho at x (no tile: used at segment boundaries)
+// and the bound of its outputs into *smax
 __global__ void __launch_bounds__(kThreads)
 adj_drho_kernel(const float* __restrict__ x, const float* __restrict__ y,
                 const float* __restrict__ lrho, float* __restrict__ lx, float* __restrict__ ly,
-                int n, int nx, int ny, float coef) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+                int n, int nx, int ny, float coef, unsigned int* __restrict__ smax) {
+    const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p0 < n;
+    const int p = live ? p0 : 0;  // lanes past the end read particle 0 and store nothing
     const Cic c = cic_of(x[p], y[p]);
     int i0, i1, j0, j1;
     dev::wrap_pair(c.i0, nx, i0, i1);
@@ -267,8 +342,12 @@ adj_drho_kernel(const float* __restrict__ x, const float* __restrict__ y,
                         lrho[j1 * nx + i1]};
     float gx, gy;
     grad4(c, v, gx, gy);
-    lx[p] = fmaf(coef, gx, lx[p]);
-    ly[p] = fmaf(coef, gy, ly[p]);
+    const float ox = fmaf(coef, gx, lx[p]), oy = fmaf(coef, gy, ly[p]);
+    if (live) {
+        lx[p] = ox;
+        ly[p] = oy;
+    }
+    wave_max_to(live ? fmaxf(fabsf(ox), fabsf(oy)) : 0.f, smax);
 }
 
 // out[perm[i]] = in[i] for four arrays (back to the previous order)
@@ -375,7 +454,10 @@ struct aw_adjoint {
     DevBuf<float> segx, segE;
     // adjoint state and grid work
     DevBuf<float> lx, ly, lux, luy, tmp4;
-    DevBuf<double> lE, lE0, part;  // adjoint field accumulators, the loss seed
+    DevBuf<unsigned long long> lE;  // int64 adjoint field accumulators
+    DevBuf<float> lE0;              // the loss seed eps0 dx dy E^N
+    DevBuf<double> part;
+    DevBuf<AdjBound> bd;            // the sweep's order-free bounds
     DevBuf<float> lEf, lrho;
     DevBuf<int64_t> sp_end;
     bool forward_done = false;
@@ -547,20 +629,25 @@ struct aw_adjoint {
         forward_done = true;
     }
 
-    // lrho = S^T lE (lE double accumulators, cleared)
-    void adj_solve() {
+    // lrho = S^T lE: the int64 accumulators of the step just swept (cleared,
+    // with the state-bound slot that step read), or the loss seed (slot < 0)
+    void adj_solve(int free_slot) {
         const hipStream_t s = stream;
         const int64_t nc = ncell(*ctx), nk = nkmodes(*ctx);
-        lE_convert_kernel<<<blocks_for(2 * nc), kThreads, 0, s>>>(lE.get(), lEf.get(), 2 * nc,
-                                                                 1.0f);
-        AW_LAUNCH_CHECK();
+        if (free_slot >= 0) {
+            lE_convert_kernel<<<blocks_for(2 * nc), kThreads, 0, s>>>(lE.get(), lEf.get(), 2 * nc,
+                                                                     bd.get(), free_slot);
+            AW_LAUNCH_CHECK();
+        } else {
+            copy_words(lE0.get(), lEf.get(), (size_t)2 * nc * sizeof(float));
+        }
         ctx->ktmp.reserve((size_t)6 * nk);
         float2* ak = reinterpret_cast<float2*>(ctx->ktmp.get());
         float2* rk = ak + 2 * nk;
         fft_exec(*ctx, ctx->r2c2, lEf.get(), ak, s);  // [lEx | lEy] -> [ak_x | ak_y]
         KG k{g.nx, g.ny, g.nx / 2 + 1, 6.28318530717958647692 / g.Lx,
              6.28318530717958647692 / g.Ly, rp.eps0};
-        es_adj_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(ak, rk, k);
+        es_adj_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(ak, rk, k, &bd.get()->rmax);
         AW_LAUNCH_CHECK();
         fft_exec(*ctx, ctx->c2r, rk, lrho.get(), s);
     }
@@ -579,10 +666,13 @@ struct aw_adjoint {
         const float coef = (float)(rp.qm * w_uniform / (g.dx * g.dy));
         const float cx = (float)(rp.dt / g.dx), cy = (float)(rp.dt / g.dy);
         const float a = (float)(rp.qm * rp.dt);
-        // seed: lrho^N = S^T (eps0 dx dy E^N), lx = lu = 0 (the seed is kept, so
-        // the backward sweep can be repeated)
-        copy_words(lE0.get(), lE.get(), (size_t)2 * nc * sizeof(double));
-        adj_solve();
+        // seed: lrho^N = S^T (eps0 dx dy E^N), lx = lu = 0, bounds zero (the seed
+        // is kept, so the backward sweep can be repeated)
+        zero_words(bd.get(), sizeof(AdjBound));
+        adj_solve(-1);
+        const float rfac = (float)(2.0 * (double)nkmodes(*ctx) / (double)nc);
+        int log2n = 0;
+        while (log2n < 62 && ((int64_t)1 << log2n) < n) ++log2n;
         for (auto* b : {&lx, &ly, &lux, &luy}) zero_words(b->get(), (size_t)n * 4);
         segx.reserve((size_t)(K + 1) * 2 * n);
         segE.reserve((size_t)(K + 1) * 2 * nc);
@@ -612,9 +702,9 @@ struct aw_adjoint {
             // this segment's order: apply the pending D_x^T at x^{s1}
             if (pending_drho) {
                 const float* xs = segx.get() + (size_t)(s1 - s0) * 2 * n;
-                adj_drho_kernel<<<blocks_for(n), kThreads, 0, s>>>(xs, xs + n, lrho.get(),
-                                                                  lx.get(), ly.get(), (int)n,
-                                                                  g.nx, g.ny, coef);
+                adj_drho_kernel<<<blocks_for(n), kThreads, 0, s>>>(
+                    xs, xs + n, lrho.get(), lx.get(), ly.get(), (int)n, g.nx, g.ny, coef,
+                    &bd.get()->smax[s1 & 1]);
                 AW_LAUNCH_CHECK();
                 pending_drho = false;
             }
@@ -635,6 +725,10 @@ struct aw_adjoint {
                 aa.Ey = aa.Ex + nc;
                 aa.lrho = lrho.get();
                 aa.lE = lE.get();
+                aa.bd = bd.get();
+                aa.slot_in = (int)((st + 1) & 1);
+                aa.rfac = rfac;
+                aa.log2n = log2n;
                 aa.off = off.get();
                 aa.ntx = ctx->ntx;
                 aa.tx = ctx->tx;
@@ -652,13 +746,14 @@ struct aw_adjoint {
                 const size_t lds = (size_t)aa.W * aa.H * (16 + 12);
                 adj_step_kernel<<<ctx->ntiles * bpt, kThreads, lds, s>>>(aa);
                 AW_LAUNCH_CHECK();
-                adj_solve();           // lrho^st
+                adj_solve(aa.slot_in);  // lrho^st
                 pending_drho = true;   // applied at x^st by the next kernel
             }
             // D_x^T at x^{s0} in this order, then back to the previous segment's order
             const float* x0 = segx.get();
             adj_drho_kernel<<<blocks_for(n), kThreads, 0, s>>>(x0, x0 + n, lrho.get(), lx.get(),
-                                                              ly.get(), (int)n, g.nx, g.ny, coef);
+                                                              ly.get(), (int)n, g.nx, g.ny, coef,
+                                                              &bd.get()->smax[s0 & 1]);
             AW_LAUNCH_CHECK();
             pending_drho = false;
             // undo the sort that produced this segment's order
@@ -758,7 +853,8 @@ aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         a->E.resize((size_t)2 * nc);
         for (auto* v : {&a->lx, &a->ly, &a->lux, &a->luy}) v->resize(n);
         a->lE.resize((size_t)2 * nc);
-        AW_HIP(hipMemset(a->lE.get(), 0, (size_t)2 * nc * sizeof(double)));
+        AW_HIP(hipMemset(a->lE.get(), 0, (size_t)2 * nc * sizeof(unsigned long long)));
+        a->bd.resize(1);
         a->lEf.resize((size_t)2 * nc);
         a->lrho.resize((size_t)nc);
         a->sp_end.resize(ends.size());

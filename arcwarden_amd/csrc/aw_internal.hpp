@@ -253,7 +253,14 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
 // counters fit LDS, else the radix sort of tile keys).
 void bin_tiles(aw_ctx& c, const aw_particles& p, int32_t* perm, int32_t* off, hipStream_t s);
 
+// Argument checks of the C ABI (capi.hip)
+void check_grid(const aw_grid* g);
+void check_particles(const aw_particles* p);
+
 // Deposit helpers (deposit.hip)
+void deposit_rho_acc(const aw_grid& g, const aw_particles& p, double qm, int policy,
+                     int scale_log2, unsigned long long* acc, int num_cu, size_t lds_per_block,
+                     hipStream_t s);
 // counter: ncounter counters cleared by the same launch (stray counts)
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign = false,
@@ -291,6 +298,8 @@ void push_deposit_multi(const void* slots, const int32_t* wg_run, int total_wg, 
 // Contexts (capi.hip)
 aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y);
 int fixed_scale_for(const std::vector<aw_species>& sp, const aw_run_params& rp, bool legacy);
+void autotune_run(const aw_grid& g, const aw_run_params& rp, const aw_species* sp, int nsp,
+                  bool has_b0, int& tile, int& bpt);
 void set_ctx_tiles(aw_ctx& c, int tile_x, int tile_y);
 void sort_tiles(aw_ctx& c, const aw_particles& src, const aw_particles& dst, int32_t* perm,
                 int32_t* off, hipStream_t s);
@@ -348,6 +357,35 @@ __device__ __forceinline__ int cell_of(float x, float y, int nx, int ny) {
     if (ci < 0) ci = 0;
     if (cj < 0) cj = 0;
     return cj * nx + ci;
+}
+
+// The loaders' deterministic draws (include/pic/particles.hpp:190-211): the
+// radical inverse of i in a base, and the hashed uniform in (0, 1) of draw
+// index t on a stream.
+__device__ __forceinline__ double radical_inverse(uint32_t i, uint32_t base) {
+    const double inv = 1.0 / (double)base;
+    double f = inv, r = 0.0;
+    while (i > 0) {
+        r += (double)(i % base) * f;
+        i /= base;
+        f *= inv;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352dU;
+    v ^= v >> 15;
+    v *= 0x846ca68bU;
+    v ^= v >> 16;
+    return v;
+}
+
+__device__ __forceinline__ double rng_uniform(int64_t t, uint32_t stream, uint64_t seed) {
+    const uint32_t h = hash_u32((uint32_t)t * 0x9e3779b9U + stream * 0x85ebca6bU +
+                                (uint32_t)seed);
+    return ((double)h + 0.5) * (1.0 / 4294967296.0);
 }
 
 struct CicF {

@@ -80,10 +80,6 @@ void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_ab
                double coef_signed, double umax, const double* dev, float* c_dcu, double* inv_dcu,
                hipStream_t s, int runs, long long sE, long long sB, int si);
 
-namespace {
-
-thread_local std::string g_last_error;
-
 void check_grid(const aw_grid* g) {
     AW_REQUIRE(g != nullptr, "grid is NULL");
     AW_REQUIRE(g->nx > 0 && g->ny > 0, "grid nx, ny must be positive");
@@ -97,6 +93,10 @@ void check_particles(const aw_particles* p) {
     if (p->n > 0)
         AW_REQUIRE(p->x && p->y && p->ux && p->uy && p->uz, "particle arrays must be non-NULL");
 }
+
+namespace {
+
+thread_local std::string g_last_error;
 
 // also writes the R2C input copy (c.rin) so the solve needs no extra copy
 void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s,
@@ -599,6 +599,128 @@ struct aw_sim {
         AW_HIP(hipStreamSynchronize(s));
     }
 
+    // The caller edited the particles between steps.  The reference's step
+    // deposits its sources from the particles as they are
+    // (include/pic/simulation.hpp:80-93), so such edits are part of its
+    // contract; here the next step's sources come out of the previous push, so
+    // they are rebuilt from the current state: re-sort, then rho (ES, into the
+    // accumulator the next step solves from) or the moments (Darwin, deposited
+    // by the next step).
+    void particles_changed() {
+        if (!initialized)
+            fail(AW_ERR_STATE, "aw_sim_particles_changed before aw_sim_init", __FILE__, __LINE__);
+        const hipStream_t s = stream;
+        do_sort();
+        if (em) {
+            moments_ready = false;
+            return;
+        }
+        ctx->acc.zero(s);
+        deposit_rho_fixed(*ctx, particles(), rp.qm, AW_DEPOSIT_BINNED, tile_off.get(), s);
+        fields_ready = false;
+    }
+
+    // The state a step reads, as checkpoint arrays: the particles, the
+    // fixed-point sources the previous push deposited (ES: rho; Darwin: rho,
+    // J, amu), and on the ES path the field and charge of the last solve.
+    // flags = {model, fields_ready, moments_ready, j_staggered}.
+    int32_t ck_flags[4] = {0, 0, 0, 0};
+    std::vector<aw_checkpoint_array> ck_arrays() {
+        if (!initialized)
+            fail(AW_ERR_STATE, "checkpoint before aw_sim_init", __FILE__, __LINE__);
+        AW_REQUIRE(runs == 1, "batched Darwin runs are not checkpointed");
+        const aw_particles p = particles();
+        const int64_t nc = ncell(*ctx);
+        ck_flags[0] = em ? 1 : 0;
+        ck_flags[1] = fields_ready ? 1 : 0;
+        ck_flags[2] = moments_ready ? 1 : 0;
+        ck_flags[3] = j_staggered ? 1 : 0;
+        std::vector<aw_checkpoint_array> a;
+        auto dev = [&](const char* nm, uint32_t t, uint64_t cnt, void* d) {
+            a.push_back(aw_checkpoint_array{nm, t, cnt, d, 1});
+        };
+        dev("x", AW_CK_F32, (uint64_t)n, p.x);
+        dev("y", AW_CK_F32, (uint64_t)n, p.y);
+        dev("ux", AW_CK_F32, (uint64_t)n, p.ux);
+        dev("uy", AW_CK_F32, (uint64_t)n, p.uy);
+        dev("uz", AW_CK_F32, (uint64_t)n, p.uz);
+        if (p.w) dev("w", AW_CK_F32, (uint64_t)n, p.w);
+        if (em) {
+            dev("moments_fixed", AW_CK_U64, (uint64_t)(8 * nc), acc8.get());
+        } else {
+            dev("rho_fixed", AW_CK_U64, (uint64_t)nc, ctx->acc.get());
+            dev("E", AW_CK_F32, (uint64_t)(2 * nc), E.get());
+            dev("rho", AW_CK_F32, (uint64_t)nc, rho.get());
+        }
+        a.push_back(aw_checkpoint_array{"state_flags", AW_CK_I32, 4, ck_flags, 0});
+        return a;
+    }
+
+    void save(const char* path, const aw_checkpoint_header& h, const char* deck,
+              const char* git) {
+        const std::vector<aw_checkpoint_array> a = ck_arrays();
+        AW_HIP(hipStreamSynchronize(stream));
+        const aw_status st = aw_checkpoint_write(path, &h, deck, git, a.data(), (int32_t)a.size());
+        if (st != AW_OK) fail(st, last_error(), __FILE__, __LINE__);
+    }
+
+    // Loads a checkpoint written by save() into a created simulation of the
+    // same grid, species and model (no aw_sim_init), then re-sorts: results
+    // continue bit for bit (the deposits do not depend on particle order).
+    void restore(const char* path) {
+        AW_REQUIRE(runs == 1, "batched Darwin runs are not checkpointed");
+        aw_checkpoint* ck = nullptr;
+        aw_status st = aw_checkpoint_open(path, &ck);
+        if (st != AW_OK) fail(st, last_error(), __FILE__, __LINE__);
+        std::unique_ptr<aw_checkpoint, aw_status (*)(aw_checkpoint*)> hold(ck, aw_checkpoint_close);
+        int32_t na = 0;
+        aw_checkpoint_info(ck, nullptr, &na, nullptr, nullptr);
+        initialized = true;  // so ck_arrays() lays out the targets
+        const std::vector<aw_checkpoint_array> want = ck_arrays();
+        for (const auto& w : want) {
+            int32_t found = -1;
+            for (int32_t i = 0; i < na && found < 0; ++i) {
+                char name[64];
+                uint32_t t = 0;
+                uint64_t cnt = 0;
+                int32_t present = 0;
+                aw_checkpoint_array_info(ck, i, name, sizeof name, &t, &cnt, &present);
+                if (std::strcmp(name, w.name) == 0) {
+                    if (t != w.dtype || cnt != w.count || !present) {
+                        initialized = false;
+                        fail(AW_ERR_INVALID,
+                             std::string("checkpoint array '") + w.name +
+                                 "' does not match this simulation (dtype, size or payload)",
+                             __FILE__, __LINE__);
+                    }
+                    found = i;
+                }
+            }
+            if (found < 0) {
+                initialized = false;
+                fail(AW_ERR_INVALID, std::string("checkpoint lacks array '") + w.name + "'",
+                     __FILE__, __LINE__);
+            }
+            st = aw_checkpoint_read_array(ck, found, w.data, w.on_device);
+            if (st != AW_OK) {
+                initialized = false;
+                fail(st, last_error(), __FILE__, __LINE__);
+            }
+        }
+        if (ck_flags[0] != (em ? 1 : 0)) {
+            initialized = false;
+            fail(AW_ERR_INVALID, "checkpoint field model differs from this simulation", __FILE__,
+                 __LINE__);
+        }
+        fields_ready = ck_flags[1] != 0;
+        moments_ready = ck_flags[2] != 0;
+        j_staggered = ck_flags[3] != 0;
+        do_sort();
+        strays.zero(stream);
+        stray_pending = false;
+        AW_HIP(hipStreamSynchronize(stream));
+    }
+
     void step(int64_t nstep) {
         if (!initialized) fail(AW_ERR_STATE, "aw_sim_step before aw_sim_init", __FILE__, __LINE__);
         const hipStream_t s = stream;
@@ -731,6 +853,10 @@ aw_status aw_memset_zero(void* p, size_t bytes, aw_stream s) {
 
 aw_status aw_stream_synchronize(aw_stream s) {
     return guard("aw_stream_synchronize", [&] { AW_HIP(hipStreamSynchronize(S(s))); });
+}
+
+aw_status aw_device_synchronize(void) {
+    return guard("aw_device_synchronize", [&] { AW_HIP(hipDeviceSynchronize()); });
 }
 
 void aw_run_params_defaults(aw_run_params* rp) {
@@ -1163,6 +1289,30 @@ static std::unique_ptr<aw_sim> finish_sim(std::unique_ptr<aw_sim> sim, int runs)
     return sim;
 }
 
+}  // extern "C"
+
+namespace aw {
+
+// The autotuner's tile side and workgroups per tile for one electrostatic run
+// (an ensemble member): a temporary simulation of the run is tuned as
+// aw_sim_init tunes one -- loaded once per candidate tile, its fused kernel
+// timed -- or its shape is found in the per-process cache.
+void autotune_run(const aw_grid& g, const aw_run_params& rp, const aw_species* sp, int nsp,
+                  bool has_b0, int& tile, int& bpt) {
+    aw_sim_config c{};
+    c.has_b0 = has_b0 ? 1 : 0;
+    c.field_model = AW_MODEL_ES;
+    c.autotune = 1;
+    std::unique_ptr<aw_sim> sim = finish_sim(make_sim(&g, &rp, sp, nsp, &c), 1);
+    sim->autotune();
+    tile = sim->ctx->tx;
+    bpt = sim->bpt;
+}
+
+}  // namespace aw
+
+extern "C" {
+
 aw_status aw_sim_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
                         int32_t nspecies, const aw_sim_config* cfg, aw_sim** out) {
     return guard("aw_sim_create", [&] {
@@ -1302,6 +1452,35 @@ aw_status aw_sim_diag_now(aw_sim* sim, int64_t step, aw_diag_sample* out) {
         out->max_e = std::sqrt(h[2]);
         out->charge = h[3] * dxdy;
         out->total = out->ke + out->ee;
+    });
+}
+
+aw_status aw_sim_checkpoint_save(aw_sim* sim, const char* path, int64_t step, double time,
+                                 const char* deck, const char* git_hash) {
+    return guard("aw_sim_checkpoint_save", [&] {
+        AW_REQUIRE(sim && path, "NULL argument");
+        aw_checkpoint_header h{};
+        h.version = 1;
+        h.rng_seed = sim->rp.rng_seed;
+        h.step = step;
+        h.time = time;
+        h.sd = 1.0;
+        h.eps_l = 0.0;
+        sim->save(path, h, deck, git_hash);
+    });
+}
+
+aw_status aw_sim_checkpoint_restore(aw_sim* sim, const char* path) {
+    return guard("aw_sim_checkpoint_restore", [&] {
+        AW_REQUIRE(sim && path, "NULL argument");
+        sim->restore(path);
+    });
+}
+
+aw_status aw_sim_particles_changed(aw_sim* sim) {
+    return guard("aw_sim_particles_changed", [&] {
+        AW_REQUIRE(sim, "sim is NULL");
+        sim->particles_changed();
     });
 }
 

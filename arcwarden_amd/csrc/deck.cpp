@@ -7,6 +7,7 @@
 // finalize step turns physical inputs into code-unit RunParams.
 #include "aw.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -252,6 +253,84 @@ void deck_defaults(aw_deck& d) {
 }
 
 // code-unit RunParams from the physical inputs (needs dx = Lx/nx)
+// ---- the dipole background (include/pic/background_b0.hpp:57-146) --------------
+// Along a dipole field line of equatorial distance L R_E, at magnetic
+// latitude lam, B/B_eq = sqrt(1 + 3 sin^2 lam) / cos^6 lam and the arc length
+// from the equator is s = L R_E (sin lam sqrt(1 + 3 sin^2 lam) +
+// asinh(sqrt(3) sin lam)/sqrt(3)) / 2.  The Yee push evaluates B0(s) as an
+// even polynomial in u = (s/s_ref)^2 of degree 5, least-squares fitted to 400
+// midpoint samples of |s| <= s_ref and checked to 2e-4 relative.
+
+double dipole_arc(double lam) {
+    const double t = std::sin(lam);
+    return 0.5 * (t * std::sqrt(1.0 + 3.0 * t * t) + std::asinh(std::sqrt(3.0) * t) / std::sqrt(3.0));
+}
+
+double dipole_b_over_beq(double s, double lre) {
+    const double target = std::fabs(s) / lre;
+    if (target <= 0.0) return 1.0;
+    double lo = 0.0, hi = 0.5 * std::numbers::pi;
+    if (target >= dipole_arc(hi)) throw DeckError("dipole: s beyond the field-line apex");
+    for (int it = 0; it < 200; ++it) {  // s(lam) is increasing: bisection
+        const double mid = 0.5 * (lo + hi);
+        (dipole_arc(mid) < target ? lo : hi) = mid;
+    }
+    const double lam = 0.5 * (lo + hi), t = std::sin(lam), c2 = 1.0 - t * t;
+    return std::sqrt(1.0 + 3.0 * t * t) / (c2 * c2 * c2);
+}
+
+void fit_dipole(aw_run_params& rp, double sref) {
+    constexpr int K = 6, M = 400;
+    rp.b0_sref = sref;
+    double G[K][K] = {}, r[K] = {};  // normal equations of the sampled fit
+    for (int m = 0; m < M; ++m) {
+        const double sv = sref * (m + 0.5) / M, u = (sv / sref) * (sv / sref);
+        const double f = dipole_b_over_beq(sv, rp.b0_lre);
+        double pw[K];
+        pw[0] = 1.0;
+        for (int k = 1; k < K; ++k) pw[k] = pw[k - 1] * u;
+        for (int a = 0; a < K; ++a) {
+            r[a] += pw[a] * f;
+            for (int b = 0; b < K; ++b) G[a][b] += pw[a] * pw[b];
+        }
+    }
+    // Cholesky G = L L^T (G is symmetric positive definite), then two solves
+    double Lc[K][K] = {};
+    for (int a = 0; a < K; ++a) {
+        for (int b = 0; b <= a; ++b) {
+            double v = G[a][b];
+            for (int k = 0; k < b; ++k) v -= Lc[a][k] * Lc[b][k];
+            if (a == b) {
+                if (!(v > 0.0)) throw DeckError("fit_dipole: normal matrix not positive definite");
+                Lc[a][a] = std::sqrt(v);
+            } else {
+                Lc[a][b] = v / Lc[b][b];
+            }
+        }
+    }
+    double z[K];
+    for (int a = 0; a < K; ++a) {
+        double v = r[a];
+        for (int k = 0; k < a; ++k) v -= Lc[a][k] * z[k];
+        z[a] = v / Lc[a][a];
+    }
+    for (int a = K - 1; a >= 0; --a) {
+        double v = z[a];
+        for (int k = a + 1; k < K; ++k) v -= Lc[k][a] * rp.b0_c[k];
+        rp.b0_c[a] = v / Lc[a][a];
+    }
+    double worst = 0.0;
+    for (int m = 0; m <= 100; ++m) {
+        const double sv = sref * m / 100.0, u = (sv / sref) * (sv / sref);
+        double p = rp.b0_c[K - 1];
+        for (int k = K - 2; k >= 0; --k) p = p * u + rp.b0_c[k];
+        const double f = dipole_b_over_beq(sv, rp.b0_lre);
+        worst = std::max(worst, std::fabs(p - f) / f);
+    }
+    if (worst > 2e-4)
+        throw DeckError("fit_dipole: fit residual " + std::to_string(worst) + " exceeds 2e-4");
+}
+
 void finalize(aw_deck& d) {
     if (d.nx <= 0 || d.ny <= 0 || d.Lx <= 0.0 || d.Ly <= 0.0)
         throw DeckError("deck: [grid] nx/ny/Lx/Ly must all be set and positive");
@@ -272,8 +351,12 @@ void finalize(aw_deck& d) {
         if (rp.B0[1] != 0.f || rp.B0[2] != 0.f)
             throw DeckError(
                 "deck: [background] profile=parabolic|dipole requires B0 along x (theta_deg = 0)");
-        if (rp.b0_prof == 2 && rp.b0_lre <= 0.0)
-            throw DeckError("deck: [background] profile=dipole requires lre = L*R_E > 0");
+        if (rp.b0_prof == 2) {
+            if (rp.b0_lre <= 0.0)
+                throw DeckError("deck: [background] profile=dipole requires lre = L*R_E > 0");
+            // 1% past the farther wall, so the wall itself lies inside the fit
+            fit_dipole(rp, 1.01 * std::max(rp.b0_xc, d.Lx - rp.b0_xc));
+        }
     }
     if (d.pump_enable) {  // An et al. (2019) Table I amplitudes, 1e4 e E/(me wpe^2 dx) units
         const double s = d.pump_amp * dx / 1e4;

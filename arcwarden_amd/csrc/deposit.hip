@@ -147,6 +147,47 @@ void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int 
     AW_LAUNCH_CHECK();
 }
 
+// rho of p into the caller's fixed-point accumulator acc (nx*ny cells, scale
+// 2^scale_log2) by the global-atomic or whole-grid LDS policy, with no aw_ctx
+// and no allocation; AUTO picks LDS_GRID when the grid fits LDS.
+void deposit_rho_acc(const aw_grid& g, const aw_particles& p, double qm, int policy,
+                     int scale_log2, unsigned long long* acc, int num_cu, size_t lds_per_block,
+                     hipStream_t s) {
+    if (p.n == 0) return;
+    DepArgs a{};
+    a.x = p.x;
+    a.y = p.y;
+    a.w = p.w;
+    a.n = (int)p.n;
+    a.nx = g.nx;
+    a.ny = g.ny;
+    a.qm = qm;
+    a.area = g.dx * g.dy;
+    a.coef_u = qm * p.w_uniform / a.area;
+    a.scale = std::ldexp(1.0, scale_log2);
+    const size_t sh = (size_t)g.nx * g.ny * 8;
+    if (policy == AW_DEPOSIT_AUTO) policy = sh <= lds_per_block ? AW_DEPOSIT_LDS_GRID
+                                                                : AW_DEPOSIT_GLOBAL;
+    if (policy == AW_DEPOSIT_GLOBAL) {
+        deposit_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, acc);
+        AW_LAUNCH_CHECK();
+    } else if (policy == AW_DEPOSIT_LDS_GRID) {
+        AW_REQUIRE(sh <= lds_per_block, "grid does not fit LDS for AW_DEPOSIT_LDS_GRID");
+        int per_cu = 0;
+        AW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(deposit_ldsgrid_kernel), kThreads, sh));
+        int nb = std::max(1, per_cu) * std::max(1, num_cu);
+        nb = std::min(nb, blocks_for(p.n));
+        deposit_ldsgrid_kernel<<<nb, kThreads, sh, s>>>(a, acc);
+        AW_LAUNCH_CHECK();
+    } else {
+        fail(AW_ERR_UNSUPPORTED, "policy " + std::to_string(policy) +
+                                     " needs binning scratch: use AW_DEPOSIT_GLOBAL, "
+                                     "AW_DEPOSIT_LDS_GRID or AW_DEPOSIT_AUTO",
+             __FILE__, __LINE__);
+    }
+}
+
 int tile_bpt(const aw_ctx& c, int64_t n) {
     // about 2048 particles (8 per lane) per workgroup: enough to amortise the
     // tile load and flush, few enough that the grid holds many workgroups per

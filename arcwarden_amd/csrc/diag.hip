@@ -6,6 +6,10 @@
 // combines the partials in order, so the result is deterministic.
 #include "aw_internal.hpp"
 
+#include <map>
+#include <mutex>
+#include <vector>
+
 namespace aw {
 
 namespace {
@@ -112,6 +116,71 @@ finalize_kernel(const double* __restrict__ kep, int nke, double w_scale,
     }
 }
 
+// ---- module diagnostics (include/pic/diag/*.hpp) ----------------------------
+
+struct PhaseHistArgs {
+    const float *x, *ux, *uz;
+    int64_t n;
+    int nx, nxb, nvb;
+    double bx, bz, vth, vlo, vhi, dx, Lx;
+};
+
+// f(x, v_par / v_th) counts, one integer atomic per particle inside the
+// window.  FP32 (the phase-video binning, include/pic/diag/module.hpp:58-70):
+// x wrapped in cells, v_par = (ux bx + uz bz) / vth, ix = (int)(x / nx * nxb).
+// FP64 (the snapshot binning, include/pic/diag/snapshot.hpp:55-62): x in
+// physical units wrapped by Lx, the same formulas in double.  Both truncate
+// toward zero, so values just below the window's low edge (in (-1, 0) bins)
+// land in bin 0, as the reference's casts do.  -ffp-contract=off keeps every
+// product and sum separately rounded.
+template <bool F64>
+__global__ void __launch_bounds__(kThreads)
+phase_hist_kernel(PhaseHistArgs a, unsigned int* __restrict__ counts) {
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < a.n; i += stride) {
+        int ix, iv;
+        if constexpr (F64) {
+            double xx = (double)a.x[i] * a.dx;
+            if (xx < 0.0) xx += a.Lx;
+            else if (xx >= a.Lx) xx -= a.Lx;
+            const double vp = ((double)a.ux[i] * a.bx + (double)a.uz[i] * a.bz) / a.vth;
+            ix = (int)(xx / a.Lx * (double)a.nxb);
+            iv = (int)((vp - a.vlo) / (a.vhi - a.vlo) * (double)a.nvb);
+        } else {
+            const float fnx = (float)a.nx;
+            float xx = a.x[i];
+            if (xx < 0.f) xx += fnx;
+            else if (xx >= fnx) xx -= fnx;
+            const float vp = (a.ux[i] * (float)a.bx + a.uz[i] * (float)a.bz) / (float)a.vth;
+            ix = (int)(xx / fnx * (float)a.nxb);
+            iv = (int)((vp - (float)a.vlo) / ((float)a.vhi - (float)a.vlo) * (float)a.nvb);
+        }
+        if (ix >= 0 && ix < a.nxb && iv >= 0 && iv < a.nvb)
+            atomicAdd(&counts[(int64_t)iv * a.nxb + ix], 1u);
+    }
+}
+
+// per-block FP64 sums of (a - off)^2 over a fixed slice (deterministic)
+__global__ void __launch_bounds__(kThreads)
+sum_sq_kernel(const float* __restrict__ a, int64_t n, double off, double* __restrict__ part) {
+    __shared__ double sh[kThreads / 64];
+    double acc = 0.0;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b = (int64_t)blockIdx.x * per, e = min(n, b + per);
+    for (int64_t c = b + threadIdx.x; c < e; c += kThreads) {
+        const double d = (double)a[c] - off;
+        acc += d * d;
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void gather_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx,
+                              int64_t m, float* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) dst[k] = src[idx[k]];
+}
+
 }  // namespace
 
 int ke_partials(const aw_particles& p, double* part, hipStream_t s) {
@@ -153,3 +222,86 @@ int reduce_grid_offset() { return kRedBlocks; }
 int reduce_out_offset() { return 4 * kRedBlocks; }
 
 }  // namespace aw
+
+using namespace aw;
+
+namespace {
+
+// per-device partial-sum buffers of aw_sum_sq (allocated on first use)
+std::mutex g_sumsq_mu;
+std::map<int, DevBuf<double>> g_sumsq_part;
+
+}  // namespace
+
+extern "C" {
+
+aw_status aw_phase_hist(const aw_particles* p, const aw_grid* g, int32_t nxb, int32_t nvb,
+                        double bx, double bz, double vth, double vlo, double vhi, int32_t fp64,
+                        uint32_t* counts, aw_stream s) {
+    return guard("aw_phase_hist", [&] {
+        check_grid(g);
+        check_particles(p);
+        AW_REQUIRE(counts, "counts is NULL");
+        AW_REQUIRE(nxb > 0 && nvb > 0, "nxb, nvb must be positive");
+        AW_REQUIRE(vhi > vlo, "vhi must exceed vlo");
+        AW_REQUIRE(vth != 0.0, "vth must be non-zero");
+        if (p->n == 0) return;
+        PhaseHistArgs a{};
+        a.x = p->x;
+        a.ux = p->ux;
+        a.uz = p->uz;
+        a.n = p->n;
+        a.nx = g->nx;
+        a.nxb = nxb;
+        a.nvb = nvb;
+        a.bx = bx;
+        a.bz = bz;
+        a.vth = vth;
+        a.vlo = vlo;
+        a.vhi = vhi;
+        a.dx = g->dx;
+        a.Lx = g->Lx > 0 ? g->Lx : g->nx * g->dx;
+        const int nb = (int)std::min<int64_t>(4096, (p->n + kThreads - 1) / kThreads);
+        if (fp64) phase_hist_kernel<true><<<nb, kThreads, 0, S(s)>>>(a, counts);
+        else phase_hist_kernel<false><<<nb, kThreads, 0, S(s)>>>(a, counts);
+        AW_LAUNCH_CHECK();
+    });
+}
+
+aw_status aw_sum_sq(const float* a, int64_t n, double offset, double* out, aw_stream s) {
+    return guard("aw_sum_sq", [&] {
+        AW_REQUIRE(out, "out is NULL");
+        AW_REQUIRE(n >= 0 && (n == 0 || a), "a is NULL or n < 0");
+        *out = 0.0;
+        if (n == 0) return;
+        int dev = 0;
+        AW_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lk(g_sumsq_mu);
+        DevBuf<double>& part = g_sumsq_part[dev];
+        part.reserve((size_t)kRedBlocks);
+        const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 4095) / 4096));
+        sum_sq_kernel<<<nb, kThreads, 0, S(s)>>>(a, n, offset, part.get());
+        AW_LAUNCH_CHECK();
+        std::vector<double> h((size_t)nb);
+        AW_HIP(hipMemcpyAsync(h.data(), part.get(), (size_t)nb * sizeof(double),
+                              hipMemcpyDeviceToHost, S(s)));
+        AW_HIP(hipStreamSynchronize(S(s)));
+        double t = 0.0;
+        for (double v : h) t += v;
+        *out = t;
+    });
+}
+
+aw_status aw_gather_f32(const float* src, const int32_t* idx, int64_t m, float* dst,
+                        aw_stream s) {
+    return guard("aw_gather_f32", [&] {
+        AW_REQUIRE(m >= 0, "m must be non-negative");
+        if (m == 0) return;
+        AW_REQUIRE(src && idx && dst, "NULL argument");
+        gather_kernel<<<(int)((m + kThreads - 1) / kThreads), kThreads, 0, S(s)>>>(src, idx, m,
+                                                                                  dst);
+        AW_LAUNCH_CHECK();
+    });
+}
+
+}  // extern "C"

@@ -473,6 +473,15 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
             r.has_b0 = d.has_b0 != 0;
             int tile = e->cfg.tile_x;
             if (tile <= 0) tile = r.n >= 64 * nc ? 8 : 16;
+            r.bpt = 0;
+            if (e->cfg.autotune && e->cfg.tile_x <= 0) {  // per run: its fastest tile and workgroups
+                int tt = 0, tb = 0;
+                autotune_run(d.grid, d.rp, d.species, d.nspecies, r.has_b0, tt, tb);
+                if (tt > 0) {
+                    tile = tt;
+                    r.bpt = tb;
+                }
+            }
             r.ctx.reset(make_ctx(&d.grid, tile, tile));
             scale = std::min(scale, fixed_scale_for(r.species, r.rp, false));
             e->runs.push_back(std::move(r));
@@ -486,7 +495,8 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
             r.pbase = pb;
             r.cbase = cb;
             r.obase = ob;
-            r.bpt = e->cfg.blocks_per_tile > 0 ? e->cfg.blocks_per_tile : tile_bpt(*r.ctx, r.n);
+            if (e->cfg.blocks_per_tile > 0) r.bpt = e->cfg.blocks_per_tile;
+            else if (r.bpt <= 0) r.bpt = tile_bpt(*r.ctx, r.n);
             r.wg_begin = wg;
             r.nwg = r.ctx->ntiles * r.bpt;
             pb += r.n;
@@ -588,6 +598,8 @@ aw_status aw_ensemble_set_graph(aw_ensemble* e, int32_t on) {
 aw_status aw_ensemble_init(aw_ensemble* e) {
     return guard("aw_ensemble_init", [&] {
         AW_REQUIRE(e, "ensemble is NULL");
+        AW_REQUIRE(!e->initialized && !e->dar_initialized,
+                   "aw_ensemble_init called twice (create a new ensemble to start over)");
         if (e->dar) {
             darwin_batch_init(e->dar);
             e->dar_initialized = true;

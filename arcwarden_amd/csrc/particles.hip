@@ -12,6 +12,9 @@ namespace aw {
 
 namespace {
 
+using dev::radical_inverse;
+using dev::rng_uniform;
+
 constexpr int kThreads = 256;
 constexpr int kRsRounds = 32;                       // 256 x 32 = 8192 keys / chunk
 constexpr int kRsChunk = kThreads * kRsRounds;
@@ -460,32 +463,6 @@ __global__ void migrate_kernel(float* __restrict__ x, float* __restrict__ y, int
 }
 
 // ---- loaders ---------------------------------------------------------------------
-
-__device__ __forceinline__ double radical_inverse(uint32_t i, uint32_t base) {
-    const double inv = 1.0 / (double)base;
-    double f = inv, r = 0.0;
-    while (i > 0) {
-        r += (double)(i % base) * f;
-        i /= base;
-        f *= inv;
-    }
-    return r;
-}
-
-__device__ __forceinline__ uint32_t hash_u32(uint32_t v) {
-    v ^= v >> 16;
-    v *= 0x7feb352dU;
-    v ^= v >> 15;
-    v *= 0x846ca68bU;
-    v ^= v >> 16;
-    return v;
-}
-
-__device__ __forceinline__ double rng_uniform(int64_t t, uint32_t stream, uint64_t seed) {
-    const uint32_t h = hash_u32((uint32_t)t * 0x9e3779b9U + stream * 0x85ebca6bU +
-                                (uint32_t)seed);
-    return ((double)h + 0.5) * (1.0 / 4294967296.0);
-}
 
 struct SpeciesLoad {
     int ppc, index;

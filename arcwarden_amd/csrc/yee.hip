@@ -1,0 +1,1036 @@
+// yee.hip -- the full-Maxwell (Yee) field model: FDTD field kernels, the
+// particle push on the staggered mesh, the charge-conserving current, the
+// delta-f weight equation, the mirror-equilibrium loader and the run's
+// conservation diagnostics.
+//
+// Reference: MaxwellSimulation (include/pic/simulation_maxwell.hpp:24-227)
+// and include/pic/yee2d.hpp (staggering :1-31, k_faraday / k_ampere :70-97,
+// staggered gather :102-112, yee_advance_particle :145-302, k_binomial3x3
+// :494-508, k_damp_x :531-545, k_cold_fluid :554-578, k_antenna :582-607,
+// k_gauss_residual / k_div_stats / k_wd_stats / k_energy :614-678), the
+// mirror loader (include/pic/particles.hpp:381-449,573-612) and the
+// background profile (include/pic/background_b0.hpp:28-55).
+//
+// Staggering (periodic, cell units):
+//   Ex(i+1/2, j)  Ey(i, j+1/2)  Ez(i, j)  Bx(i, j+1/2)  By(i+1/2, j)  Bz(i+1/2, j+1/2)
+// One step: B half step (Faraday), push with E^n, B^n and the charge-conserving
+// current of the move, periodic wrap, optional binomial current smoothing,
+// antenna and cold-fluid currents, B half step, E full step (Ampere), optional
+// x damping layers.
+//
+// MI355X design: the field kernels are one thread per cell over the
+// row-major grid (coalesced rows); the push is one thread per particle,
+// gathering from the L2-resident field arrays; the current is the library's
+// Esirkepov deposit (int64 fixed point, tile-binned in LDS when dense), so
+// the result is independent of thread order and of the particle order, which
+// this engine never changes.  Every FP32 field and push operation is written
+// out (the build disables contraction; fused multiply-adds are explicit), and
+// the oracle restates the same sequence.
+#include "aw_internal.hpp"
+
+#include <cmath>
+#include <memory>
+#include <vector>
+
+namespace aw {
+
+void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
+                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
+                       hipStream_t s);
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRedBlocks = 256;
+
+inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
+
+// The nine grids of the mesh, each nx*ny floats, row-major.
+struct Mesh {
+    float *ex, *ey, *ez, *bx, *by, *bz, *jx, *jy, *jz;
+    int nx, ny;
+    float dx, dy, c2, dt;
+};
+
+__device__ __forceinline__ int wrap1(int i, int n) {
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+__device__ __forceinline__ int at(const Mesh& m, int i, int j) {
+    return wrap1(j, m.ny) * m.nx + wrap1(i, m.nx);
+}
+
+// B += h * (-curl E)  (Faraday, half step h = dt/2)
+__global__ void __launch_bounds__(kThreads) faraday_kernel(Mesh m, float h) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    const float ez = m.ez[c];
+    const float dez_y = (m.ez[at(m, i, j + 1)] - ez) / m.dy;
+    const float dez_x = (m.ez[at(m, i + 1, j)] - ez) / m.dx;
+    const float dey_x = (m.ey[at(m, i + 1, j)] - m.ey[c]) / m.dx;
+    const float dex_y = (m.ex[at(m, i, j + 1)] - m.ex[c]) / m.dy;
+    m.bx[c] = m.bx[c] - h * dez_y;
+    m.by[c] = m.by[c] + h * dez_x;
+    m.bz[c] = m.bz[c] - h * (dey_x - dex_y);
+}
+
+// E += dt * (c^2 curl B - J)  (Ampere, eps0 = 1)
+__global__ void __launch_bounds__(kThreads) ampere_kernel(Mesh m) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    const float bz = m.bz[c];
+    const float dbz_y = (bz - m.bz[at(m, i, j - 1)]) / m.dy;
+    const float dbz_x = (bz - m.bz[at(m, i - 1, j)]) / m.dx;
+    const float dby_x = (m.by[c] - m.by[at(m, i - 1, j)]) / m.dx;
+    const float dbx_y = (m.bx[c] - m.bx[at(m, i, j - 1)]) / m.dy;
+    m.ex[c] = m.ex[c] + m.dt * (m.c2 * dbz_y - m.jx[c]);
+    m.ey[c] = m.ey[c] + m.dt * (-m.c2 * dbz_x - m.jy[c]);
+    m.ez[c] = m.ez[c] + m.dt * (m.c2 * (dby_x - dbx_y) - m.jz[c]);
+}
+
+// CIC interpolation of a grid whose samples sit at (i + ox, j + oy).
+__device__ __forceinline__ float gather_at(const float* a, const Mesh& m, float x, float y,
+                                           float ox, float oy) {
+    const float xs = x - ox, ys = y - oy;
+    const float fi = floorf(xs), fj = floorf(ys);
+    const int i0 = dev::wrapfar((int)fi, m.nx), j0 = dev::wrapfar((int)fj, m.ny);
+    const int i1 = i0 + 1 == m.nx ? 0 : i0 + 1, j1 = j0 + 1 == m.ny ? 0 : j0 + 1;
+    const float fx = xs - fi, fy = ys - fj;
+    const float gx = 1.f - fx, gy = 1.f - fy;
+    const float v00 = a[j0 * m.nx + i0], v10 = a[j0 * m.nx + i1];
+    const float v01 = a[j1 * m.nx + i0], v11 = a[j1 * m.nx + i1];
+    return fmaf(fx * fy, v11, fmaf(gx * fy, v01, fmaf(fx * gy, v10, (gx * gy) * v00)));
+}
+
+// u' = u + u x t, u += u' x s with t = h B, s = 2t/(1 + t^2)
+// (boris_rotate, include/pic/pusher.hpp:85-101)
+__device__ __forceinline__ void rotate(float& ux, float& uy, float& uz, float bx, float by,
+                                       float bz, float h) {
+    const float tx = h * bx, ty = h * by, tz = h * bz;
+    const float t2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
+    const float sf = 2.0f / (1.0f + t2);
+    const float sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    const float upx = fmaf(uy, tz, fmaf(-uz, ty, ux));
+    const float upy = fmaf(uz, tx, fmaf(-ux, tz, uy));
+    const float upz = fmaf(ux, ty, fmaf(-uy, tx, uz));
+    ux = fmaf(upy, sz, fmaf(-upz, sy, ux));
+    uy = fmaf(upz, sx, fmaf(-upx, sz, uy));
+    uz = fmaf(upx, sy, fmaf(-upy, sx, uz));
+}
+
+// The parabolic (b0_prof = 1) or fitted-polynomial (b0_prof = 2) background
+// field along x and its derivative (include/pic/background_b0.hpp:28-55).
+__device__ __forceinline__ float b0_profile(const aw_run_params& rp, float xph) {
+    const float d = xph - (float)rp.b0_xc;
+    if (rp.b0_prof == 2) {
+        const float sn = d / (float)rp.b0_sref;
+        const float u = sn * sn;
+        float p = (float)rp.b0_c[5];
+        for (int k = 4; k >= 0; --k) p = p * u + (float)rp.b0_c[k];
+        return rp.B0[0] * p;
+    }
+    return rp.B0[0] * (1.f + (float)rp.b0_a * d * d);
+}
+
+__device__ __forceinline__ float b0_profile_dx(const aw_run_params& rp, float xph) {
+    const float d = xph - (float)rp.b0_xc;
+    if (rp.b0_prof == 2) {
+        const float sn = d / (float)rp.b0_sref;
+        const float u = sn * sn;
+        float dp = 5.f * (float)rp.b0_c[5];
+        for (int k = 4; k >= 1; --k) dp = dp * u + (float)k * (float)rp.b0_c[k];
+        return rp.B0[0] * dp * 2.f * d / (float)(rp.b0_sref * rp.b0_sref);
+    }
+    return rp.B0[0] * 2.f * (float)rp.b0_a * d;
+}
+
+__device__ __forceinline__ double ramp_of(double t, double trmp, double toff) {
+    if (t >= toff) return 0.0;
+    if (t < trmp) return t / trmp;
+    if (t > toff - trmp) return (toff - t) / trmp;
+    return 1.0;
+}
+
+struct PushArgs {
+    float *x, *y, *ux, *uy, *uz;
+    const float* w;      // per-particle weight (NULL: w_uniform)
+    float w_uniform;
+    float *wd, *wc;      // delta-f weight and its Kahan compensation
+    double* wdd;         // FP64 delta-f accumulator
+    float *x0, *y0;      // out: positions before the move
+    float* vz;           // out: deposit velocity uz / gamma (rel), else NULL
+    float* wdep;         // out: deposit weight w * wd (delta-f), else NULL
+    int n;
+    double tnow;
+};
+
+// yee_advance_particle (include/pic/yee2d.hpp:145-302): staggered gather of
+// E (+ pump) and the wave B (+ B0 or the mirror profile), kick-rotate-kick
+// (the rotation divided by the mid-kick gamma when rp.rel), the delta-f
+// weight equation at the time-centred u, the move, and the reflecting x
+// walls of bnd_x.  Positions are stored unwrapped (the wrap runs after the
+// current deposit).
+__global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= a.n) return;
+    const float x0 = a.x[t], y0 = a.y[t];
+    float Ex = gather_at(m.ex, m, x0, y0, 0.5f, 0.f);
+    float Ey = gather_at(m.ey, m, x0, y0, 0.f, 0.5f);
+    float Ez = gather_at(m.ez, m, x0, y0, 0.f, 0.f);
+    if (rp.pump && a.tnow < rp.pump_toff) {
+        const double th = rp.pump_k0 * (double)x0 * (double)m.dx - rp.pump_w0 * a.tnow;
+        const double r = ramp_of(a.tnow, rp.pump_trmp, rp.pump_toff);
+        Ex += (float)(rp.pump_ex * cos(th) * r);
+        Ey += (float)(-rp.pump_ey * sin(th) * r);
+        Ez += (float)(rp.pump_ez * cos(th) * r);
+    }
+    const float dBx = gather_at(m.bx, m, x0, y0, 0.f, 0.5f);
+    const float dBy = gather_at(m.by, m, x0, y0, 0.5f, 0.f);
+    const float dBz = gather_at(m.bz, m, x0, y0, 0.5f, 0.5f);
+    float ux = a.ux[t], uy = a.uy[t], uz = a.uz[t];
+    const float uxo = ux, uyo = uy, uzo = uz;  // u^{n-1/2}
+    const float qmh = (float)(rp.qm * 0.5 * rp.dt);
+    ux = fmaf(qmh, Ex, ux);
+    uy = fmaf(qmh, Ey, uy);
+    uz = fmaf(qmh, Ez, uz);
+    const float gmid =
+        rp.rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
+    if (rp.b0_prof) {
+        // B0 along x varying with x; the mirror force enters as the effective
+        // transverse field (include/pic/yee2d.hpp:177-186)
+        const float xph = x0 * m.dx;
+        const float b0 = b0_profile(rp, xph);
+        const float mc = b0_profile_dx(rp, xph) / (2.f * b0 * (float)rp.qm);
+        rotate(ux, uy, uz, dBx + b0, dBy + mc * uz, dBz - mc * uy, qmh * gmid);
+    } else {
+        rotate(ux, uy, uz, dBx + rp.B0[0], dBy + rp.B0[1], dBz + rp.B0[2], qmh * gmid);
+    }
+    ux = fmaf(qmh, Ex, ux);
+    uy = fmaf(qmh, Ey, uy);
+    uz = fmaf(qmh, Ez, uz);
+    const float gnew = rp.rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
+    if (rp.deltaf) {
+        // dwd/dt = -(1 - wd) (q/m) F . dln f0/du at u^n = (u^{n-1/2} + u^{n+1/2})/2,
+        // F = E + v x dB (wave fields and pump), f0 the bi-Maxwellian with B0
+        // along x (include/pic/yee2d.hpp:191-271)
+        const float uxc = 0.5f * (ux + uxo), uyc = 0.5f * (uy + uyo), uzc = 0.5f * (uz + uzo);
+        const float gc =
+            rp.rel ? 1.f / sqrtf(1.f + fmaf(uzc, uzc, fmaf(uyc, uyc, uxc * uxc))) : 1.f;
+        const float Fx = Ex + gc * (uyc * dBz - uzc * dBy);
+        const float Fy = Ey + gc * (uzc * dBx - uxc * dBz);
+        const float Fz = Ez + gc * (uxc * dBy - uyc * dBx);
+        float tperp = (float)rp.df_tperp;
+        float t2 = (float)(rp.df_kappa * rp.df_tperp);
+        if (rp.b0_prof) {
+            const float b = b0_profile(rp, x0 * m.dx) / rp.B0[0];
+            const float par = (1.f - 1.f / b) / (float)rp.df_tpar;
+            tperp = 1.f / (par + (1.f / b) / (float)rp.df_tperp);
+            t2 = 1.f / (par + (1.f / b) / (float)(rp.df_kappa * rp.df_tperp));
+        }
+        float perp = 1.f / tperp;
+        if (rp.df_dist == 1) {  // loss-cone subtracted bi-Maxwellian, capped
+            const float u2p = fmaf(uzc, uzc, uyc * uyc);
+            const float g = (float)rp.df_rho * expf(-0.5f * u2p * (1.f / t2 - 1.f / tperp));
+            const float omg = fmaxf(1.f - g, 1e-3f);
+            perp = fmaxf((1.f / tperp - g / t2) / omg, -10.f / tperp);
+        }
+        const float S = Fx * uxc / (float)rp.df_tpar + (Fy * uyc + Fz * uzc) * perp;
+        const float relax = rp.df_taud > 0.0 ? (float)(1.0 - rp.dt / rp.df_taud) : 1.f;
+        const float hi = 1.f - 1e-4f, lo = -8.f;
+        float wn;
+        if (rp.df_wprec == 2) {
+            const double wo = a.wdd[t];
+            double wnd = (wo + rp.dt * rp.qm * (1.0 - wo) * (double)S) * (double)relax;
+            wnd = fmin(fmax(wnd, (double)lo), (double)hi);
+            a.wdd[t] = wnd;
+            wn = (float)wnd;
+        } else if (rp.df_wprec == 1) {  // Kahan-compensated accumulation
+            const float wd = a.wd[t];
+            const float inc = (float)(rp.dt * rp.qm) * (1.f - wd) * S;
+            const float cmp = a.wc[t];
+            const float yk = inc - cmp;
+            const float sum = wd + yk;
+            a.wc[t] = (sum - wd) - yk;
+            wn = fminf(fmaxf(sum * relax, lo), hi);
+        } else {
+            const float wd = a.wd[t];
+            wn = fminf(fmaxf((wd + (float)(rp.dt * rp.qm) * (1.f - wd) * S) * relax, lo), hi);
+        }
+        a.wd[t] = wn;
+        if (a.wdep) a.wdep[t] = (a.w ? a.w[t] : a.w_uniform) * wn;
+    }
+    float x1 = x0 + (float)((double)(ux * gnew) * rp.dt / (double)m.dx);
+    const float y1 = y0 + (float)((double)(uy * gnew) * rp.dt / (double)m.dy);
+    float uxs = ux, uys = uy, uzs = uz;
+    if (rp.bnd_x) {  // specular walls at the domain ends (include/pic/yee2d.hpp:276-297)
+        const float fnx = (float)m.nx;
+        if (x1 < 0.f) {
+            x1 = -x1;
+            uxs = -ux;
+        } else if (x1 >= fnx) {
+            x1 = 2.f * fnx - x1;
+            uxs = -ux;
+        }
+        if (x1 >= fnx) x1 = nextafterf(fnx, 0.f);
+        if (rp.bnd_x == 2) {  // transverse momentum damped inside the layers
+            const float nd = (float)rp.bnd_nd;
+            float d = 0.f;
+            if (x1 < nd) d = (nd - x1) / nd;
+            else if (x1 > fnx - nd) d = (x1 - (fnx - nd)) / nd;
+            if (d > 0.f) {
+                const float f = expf((float)(-rp.bnd_numax * rp.dt) * d * d);
+                uys = uy * f;
+                uzs = uz * f;
+            }
+        }
+    }
+    a.ux[t] = uxs;
+    a.uy[t] = uys;
+    a.uz[t] = uzs;
+    if (a.vz) a.vz[t] = uz * gnew;
+    a.x0[t] = x0;
+    a.y0[t] = y0;
+    a.x[t] = x1;
+    a.y[t] = y1;
+}
+
+__global__ void wrap_kernel(float* __restrict__ x, float* __restrict__ y, int n, int nx, int ny) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    x[t] = dev::wrap_coord(x[t], (float)nx);
+    y[t] = dev::wrap_coord(y[t], (float)ny);
+}
+
+__global__ void zero_kernel(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// one pass of [1,2,1] x [1,2,1] / 16 smoothing (include/pic/yee2d.hpp:494-508)
+__global__ void __launch_bounds__(kThreads)
+binomial_kernel(const float* __restrict__ src, float* __restrict__ dst, Mesh m) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    const float w[3] = {0.25f, 0.5f, 0.25f};
+    float acc = 0.f;
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+        for (int di = -1; di <= 1; ++di)
+            acc = fmaf(w[di + 1] * w[dj + 1], src[at(m, i + di, j + dj)], acc);
+    dst[c] = acc;
+}
+
+// the rotating transverse antenna current (include/pic/yee2d.hpp:582-607)
+__global__ void antenna_kernel(Mesh m, aw_run_params rp, double t) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx;
+    const double d = i - rp.ant_x0;
+    if (fabs(d) > 4.0 * rp.ant_sigma) return;
+    if (rp.ant_toff > 0 && t >= rp.ant_toff) return;
+    double r = 1.0;
+    if (rp.ant_trmp > 0) {
+        if (t < rp.ant_trmp) r = t / rp.ant_trmp;
+        else if (rp.ant_toff > 0 && t > rp.ant_toff - rp.ant_trmp)
+            r = (rp.ant_toff - t) / rp.ant_trmp;
+    }
+    const double g = rp.ant_amp * r * exp(-d * d / (2.0 * rp.ant_sigma * rp.ant_sigma));
+    const double ph = rp.ant_w0 * t;
+    m.jy[c] += (float)(g * cos(ph));
+    m.jz[c] += (float)(g * sin(ph));
+}
+
+// linearised cold electron fluid on the nodes, ny = 1 (include/pic/yee2d.hpp:554-578)
+__global__ void cold_fluid_kernel(Mesh m, aw_run_params rp, float* __restrict__ vcy,
+                                  float* __restrict__ vcz) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx;
+    const float Ey = m.ey[c], Ez = m.ez[c];
+    const float h = (float)(0.5 * rp.dt * rp.qm);
+    float uy = vcy[c] + h * Ey;
+    float uz = vcz[c] + h * Ez;
+    const float b0 = rp.b0_prof ? b0_profile(rp, i * m.dx) : rp.B0[0];
+    float sn, cs;
+    sincosf((float)(-rp.qm * rp.dt) * b0, &sn, &cs);
+    const float uy2 = uy * cs - uz * sn;
+    const float uz2 = uy * sn + uz * cs;
+    uy = uy2 + h * Ey;
+    uz = uz2 + h * Ez;
+    vcy[c] = uy;
+    vcz[c] = uz;
+    const float jc = (float)(rp.qm * rp.cold_nc);
+    m.jy[c] += jc * uy;
+    m.jz[c] += jc * uz;
+}
+
+// x damping layers: node mask on Ey, Ez, Bx (and the cold fluid), half-node
+// mask on Ex, By, Bz (include/pic/yee2d.hpp:531-545)
+__global__ void damp_kernel(Mesh m, const float* __restrict__ mn, const float* __restrict__ mh,
+                            float* vcy, float* vcz) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx;
+    const float a = mn[i], b = mh[i];
+    if (a == 1.f && b == 1.f) return;
+    m.ey[c] *= a;
+    m.ez[c] *= a;
+    m.bx[c] *= a;
+    m.ex[c] *= b;
+    m.by[c] *= b;
+    m.bz[c] *= b;
+    if (vcy) {
+        vcy[c] *= a;
+        vcz[c] *= a;
+    }
+}
+
+// Gauss residual at the nodes: div E - rho (include/pic/yee2d.hpp:614-623)
+__global__ void gauss_kernel(Mesh m, const float* __restrict__ rho, float* __restrict__ r) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    const int i = c % m.nx, j = c / m.nx;
+    const float dive = (m.ex[c] - m.ex[at(m, i - 1, j)]) / m.dx +
+                       (m.ey[c] - m.ey[at(m, i, j - 1)]) / m.dy;
+    r[c] = dive - rho[c];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kThreads / 64; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
+// Per-block FP64 partials over fixed slices of the cells (deterministic):
+// what = 0: {0.5 |E|^2 dA, 0.5 c^2 |B|^2 dA}; 1: {divB^2, (r - r0)^2}.
+__global__ void __launch_bounds__(kThreads)
+mesh_stats_kernel(Mesh m, int what, const float* __restrict__ r, const float* __restrict__ r0,
+                  double* __restrict__ part) {
+    __shared__ double sh[kThreads / 64];
+    const int64_t nc = (int64_t)m.nx * m.ny;
+    const int64_t per = (nc + gridDim.x - 1) / gridDim.x;
+    const int64_t b = (int64_t)blockIdx.x * per, e = min(nc, b + per);
+    double s0 = 0.0, s1 = 0.0;
+    const double dA = (double)m.dx * (double)m.dy;
+    for (int64_t c = b + threadIdx.x; c < e; c += kThreads) {
+        if (what == 0) {
+            s0 += 0.5 * ((double)m.ex[c] * m.ex[c] + (double)m.ey[c] * m.ey[c] +
+                         (double)m.ez[c] * m.ez[c]) * dA;
+            s1 += 0.5 * (double)m.c2 *
+                  ((double)m.bx[c] * m.bx[c] + (double)m.by[c] * m.by[c] +
+                   (double)m.bz[c] * m.bz[c]) * dA;
+        } else {
+            const int i = (int)(c % m.nx), j = (int)(c / m.nx);
+            const float divb = (m.bx[at(m, i + 1, j)] - m.bx[c]) / m.dx +
+                               (m.by[at(m, i, j + 1)] - m.by[c]) / m.dy;
+            s0 += (double)divb * divb;
+            const double d = (double)r[c] - r0[c];
+            s1 += d * d;
+        }
+    }
+    const double t0 = block_sum(s0, sh), t1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = t0;
+        part[2 * blockIdx.x + 1] = t1;
+    }
+}
+
+// per-block {sum wd, sum wd^2, max |wd|} over fixed particle slices
+__global__ void __launch_bounds__(kThreads)
+wd_stats_kernel(const float* __restrict__ wd, int64_t n, double* __restrict__ part) {
+    __shared__ double sh[kThreads / 64];
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b = (int64_t)blockIdx.x * per, e = min(n, b + per);
+    double s1 = 0.0, s2 = 0.0, mx = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += kThreads) {
+        const double v = wd[i];
+        s1 += v;
+        s2 += v * v;
+        mx = fmax(mx, fabs(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    __shared__ double shm[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+    const double t1 = block_sum(s1, sh), t2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) m = fmax(m, shm[w]);
+        part[3 * blockIdx.x] = t1;
+        part[3 * blockIdx.x + 1] = t2;
+        part[3 * blockIdx.x + 2] = m;
+    }
+}
+
+// the diagnostics' view of the fields: wave E + pump, wave B + B0, and ELx = Ex
+// (tools/arcsim.cu:121-142)
+__global__ void diag_fields_kernel(Mesh m, aw_run_params rp, double t, int pump,
+                                   float* __restrict__ out) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    const int nc = m.nx * m.ny;
+    if (c >= nc) return;
+    float ex = m.ex[c], ey = m.ey[c], ez = m.ez[c];
+    if (pump) {
+        const int i = c % m.nx;
+        const double th = rp.pump_k0 * (i * (double)m.dx) - rp.pump_w0 * t;
+        const double r = ramp_of(t, rp.pump_trmp, rp.pump_toff);
+        ex += (float)(rp.pump_ex * cos(th) * r);
+        ey += (float)(-rp.pump_ey * sin(th) * r);
+        ez += (float)(rp.pump_ez * cos(th) * r);
+    }
+    out[c] = ex;
+    out[nc + c] = ey;
+    out[2 * nc + c] = ez;
+    out[3 * nc + c] = m.bx[c] + rp.B0[0];
+    out[4 * nc + c] = m.by[c] + rp.B0[1];
+    out[5 * nc + c] = m.bz[c] + rp.B0[2];
+    out[6 * nc + c] = m.ex[c];
+}
+
+struct MirrorLoad {
+    const int32_t* off;  // nx + 1 marker offsets per cell
+    int nx;
+    float dx;
+    double uth_par, uth_perp_eq, lc_rho, lc_kappa;
+    float weight;
+    uint64_t seed;
+    int dist;
+    int64_t n;
+};
+
+// mirror_init_kernel (include/pic/particles.hpp:390-449): equal-weight
+// markers of the (E, mu)-mapped bi-Maxwellian (or loss-cone) in the profile,
+// marker t in cell c with off[c] <= t < off[c+1], hashed draws of index t.
+__global__ void mirror_load_kernel(MirrorLoad L, aw_run_params rp, float* __restrict__ x,
+                                   float* __restrict__ y, float* __restrict__ ux,
+                                   float* __restrict__ uy, float* __restrict__ uz,
+                                   float* __restrict__ w) {
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= L.n) return;
+    int lo = 0, hi = L.nx;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (L.off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const int ti = (int)t;
+    float xv = (float)lo + (float)dev::rng_uniform(ti, 0, L.seed);
+    if (xv >= (float)L.nx) xv = nextafterf((float)L.nx, 0.f);
+    const double b = (double)b0_profile(rp, xv * L.dx) / (double)rp.B0[0];
+    const double tpa = L.uth_par * L.uth_par, tpe = L.uth_perp_eq * L.uth_perp_eq;
+    const double tp = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / tpe);
+    const double r1 = fmax(dev::rng_uniform(ti, 1, L.seed), 1e-12);
+    const double r2 = dev::rng_uniform(ti, 2, L.seed);
+    const double r4 = dev::rng_uniform(ti, 4, L.seed);
+    const double two_pi = 6.28318530717958647692;
+    const double upar = L.uth_par * sqrt(-2.0 * log(r1)) * cos(two_pi * r2);
+    double uperp;
+    if (L.dist == 1) {  // rejection off the Rayleigh(Tp) envelope
+        const double t2 = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / (L.lc_kappa * tpe));
+        const double dinv = 0.5 * (1.0 / t2 - 1.0 / tp);
+        double u2 = 0.0;
+        for (int k = 0; k < 64; ++k) {
+            const double rm = fmax(dev::rng_uniform(ti, 10 + 2 * k, L.seed), 1e-12);
+            const double ra = dev::rng_uniform(ti, 11 + 2 * k, L.seed);
+            u2 = -2.0 * tp * log(rm);
+            if (ra < 1.0 - L.lc_rho * exp(-u2 * dinv)) break;
+        }
+        uperp = sqrt(u2);
+    } else {
+        const double r3 = fmax(dev::rng_uniform(ti, 3, L.seed), 1e-12);
+        uperp = sqrt(tp) * sqrt(-2.0 * log(r3));
+    }
+    x[t] = xv;
+    y[t] = 0.5f;
+    ux[t] = (float)upar;
+    uy[t] = (float)(uperp * cos(two_pi * r4));
+    uz[t] = (float)(uperp * sin(two_pi * r4));
+    if (w) w[t] = L.weight;
+}
+
+// Per-cell marker offsets of the mirror load (include/pic/particles.hpp:573-597):
+// count(c) = lround(ppc * n(x_c)/n_eq) at the cell centre.
+std::vector<int32_t> mirror_offsets(const aw_grid& g, const aw_run_params& rp,
+                                    const aw_species& q) {
+    auto prof = [&](float xph) {  // the host twin of b0_profile
+        const float d = xph - (float)rp.b0_xc;
+        if (rp.b0_prof == 2) {
+            const float sn = d / (float)rp.b0_sref;
+            const float u = sn * sn;
+            float p = (float)rp.b0_c[5];
+            for (int k = 4; k >= 0; --k) p = p * u + (float)rp.b0_c[k];
+            return rp.B0[0] * p;
+        }
+        return rp.B0[0] * (1.f + (float)rp.b0_a * d * d);
+    };
+    const double tpa = q.uth[0] * q.uth[0], tpe = q.uth[1] * q.uth[1];
+    std::vector<int32_t> off((size_t)g.nx + 1, 0);
+    int64_t tot = 0;
+    for (int c = 0; c < g.nx; ++c) {
+        const double b = (double)prof((c + 0.5f) * (float)g.dx) / (double)rp.B0[0];
+        const double t1 = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / tpe);
+        double nfac = t1 / tpe;
+        if (q.dist == 1) {
+            const double t2 = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / (q.lc_kappa * tpe));
+            nfac = (t1 - q.lc_rho * t2) / (tpe * (1.0 - q.lc_rho * q.lc_kappa));
+        }
+        off[c] = (int32_t)tot;
+        tot += std::lround(q.ppc * nfac);
+        AW_REQUIRE(tot < (int64_t)INT32_MAX, "mirror load: marker count must fit int32");
+    }
+    off[g.nx] = (int32_t)tot;
+    return off;
+}
+
+void check_mirror(const aw_grid& g, const aw_run_params& rp, const aw_species& q) {
+    AW_REQUIRE(g.ny == 1, "initialize_mirror: ny must be 1");
+    AW_REQUIRE(rp.b0_prof != 0, "initialize_mirror: rp.b0_prof required");
+    AW_REQUIRE(q.uth[1] == q.uth[2], "initialize_mirror: gyrotropy uth[1] == uth[2] required");
+    AW_REQUIRE(q.dist != 1 || (q.lc_rho >= 0.0 && q.lc_rho <= 1.0 && q.lc_kappa > 0.0 &&
+                               q.lc_kappa < 1.0),
+               "initialize_mirror: losscone needs 0<=rho<=1, 0<kappa<1");
+    AW_REQUIRE(q.ppc > 0, "species ppc must be positive");
+}
+
+}  // namespace
+}  // namespace aw
+
+using namespace aw;
+
+struct aw_maxwell {
+    aw_grid g{};
+    aw_run_params rp{};
+    std::unique_ptr<aw_ctx> ctx;  // Esirkepov binning and the rho accumulator
+    hipStream_t stream = nullptr;
+    DevBuf<float> F;              // ex ey ez bx by bz jx jy jz
+    DevBuf<float> jtmp;           // smoothing scratch
+    DevBuf<float> mask;           // [node | half-node] x damping masks
+    DevBuf<float> vc;             // [vcy | vcz] cold fluid
+    DevBuf<float> x0, y0, vz, wdep;
+    DevBuf<float> rho, rres, r0, dfields;
+    DevBuf<double> part;
+    bool have_ref = false;
+    int64_t nstep = 0;
+
+    int64_t nc() const { return (int64_t)g.nx * g.ny; }
+
+    Mesh mesh() const {
+        Mesh m{};
+        float* f = F.get();
+        const int64_t n = nc();
+        m.ex = f;
+        m.ey = f + n;
+        m.ez = f + 2 * n;
+        m.bx = f + 3 * n;
+        m.by = f + 4 * n;
+        m.bz = f + 5 * n;
+        m.jx = f + 6 * n;
+        m.jy = f + 7 * n;
+        m.jz = f + 8 * n;
+        m.nx = g.nx;
+        m.ny = g.ny;
+        m.dx = (float)g.dx;
+        m.dy = (float)g.dy;
+        m.c2 = (float)(rp.c * rp.c);
+        m.dt = (float)rp.dt;
+        return m;
+    }
+
+    ~aw_maxwell() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void zero(float* p, int64_t n) {
+        if (n <= 0) return;
+        zero_kernel<<<(int)((n + kThreads - 1) / kThreads), kThreads, 0, stream>>>(p, n);
+        AW_LAUNCH_CHECK();
+    }
+
+    void build_masks() {
+        std::vector<float> h(2 * (size_t)g.nx);
+        const int nd = rp.bnd_nd;
+        auto val = [&](double xi) {
+            double d = 0.0;
+            if (xi < nd) d = (nd - xi) / (double)nd;
+            else if (xi > g.nx - nd) d = (xi - (g.nx - nd)) / (double)nd;
+            return d <= 0.0 ? 1.0 : std::exp(-rp.bnd_numax * d * d * rp.dt);
+        };
+        for (int i = 0; i < g.nx; ++i) {
+            h[i] = (float)val(i);
+            h[g.nx + i] = (float)val(i + 0.5);
+        }
+        mask.resize(h.size());
+        AW_HIP(hipMemcpy(mask.get(), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+
+    void smooth(float* a) {  // rp.jfilter binomial passes in place
+        if (rp.jfilter <= 0) return;
+        const Mesh m = mesh();
+        float *src = a, *dst = jtmp.get();
+        for (int k = 0; k < rp.jfilter; ++k) {
+            binomial_kernel<<<blocks_for(nc()), kThreads, 0, stream>>>(src, dst, m);
+            AW_LAUNCH_CHECK();
+            std::swap(src, dst);
+        }
+        if (src != a)
+            AW_HIP(hipMemcpyAsync(a, src, (size_t)nc() * sizeof(float), hipMemcpyDeviceToDevice,
+                                  stream));
+    }
+
+    // MaxwellSimulation::step_at (include/pic/simulation_maxwell.hpp:52-109)
+    void step(aw_particles& p, const aw_deltaf_state* df, double tnow) {
+        const Mesh m = mesh();
+        const int nb = blocks_for(nc());
+        const float h = 0.5f * (float)rp.dt;
+        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h);
+        AW_LAUNCH_CHECK();
+        if (p.n > 0) {
+            AW_REQUIRE(!rp.deltaf || (df && df->wd), "delta-f run without a weight array");
+            AW_REQUIRE(!rp.deltaf || rp.df_wprec != 1 || df->wc,
+                       "df_wprec = 1 needs the compensation array");
+            AW_REQUIRE(!rp.deltaf || rp.df_wprec != 2 || df->wdd,
+                       "df_wprec = 2 needs the FP64 weight array");
+            const size_t n = (size_t)p.n;
+            x0.reserve(n);
+            y0.reserve(n);
+            // the current's uz/gamma is taken before the walls damp u (bnd_x = 2)
+            const bool own_vz = rp.rel || rp.bnd_x;
+            if (own_vz) vz.reserve(n);
+            if (rp.deltaf) wdep.reserve(n);
+            zero(m.jx, 3 * nc());
+            PushArgs a{};
+            a.x = p.x;
+            a.y = p.y;
+            a.ux = p.ux;
+            a.uy = p.uy;
+            a.uz = p.uz;
+            a.w = p.w;
+            a.w_uniform = (float)p.w_uniform;
+            a.wd = rp.deltaf ? df->wd : nullptr;
+            a.wc = rp.deltaf ? df->wc : nullptr;
+            a.wdd = rp.deltaf ? df->wdd : nullptr;
+            a.x0 = x0.get();
+            a.y0 = y0.get();
+            a.vz = own_vz ? vz.get() : nullptr;
+            a.wdep = rp.deltaf ? wdep.get() : nullptr;
+            a.n = (int)p.n;
+            a.tnow = tnow;
+            yee_push_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(a, m, rp);
+            AW_LAUNCH_CHECK();
+            aw_particles dep = p;
+            if (own_vz) dep.uz = vz.get();
+            if (rp.deltaf) dep.w = wdep.get();
+            deposit_esirkepov(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, AW_DEPOSIT_AUTO,
+                              m.jx, m.jy, m.jz, stream);
+            wrap_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(p.x, p.y, (int)p.n, g.nx,
+                                                                 g.ny);
+            AW_LAUNCH_CHECK();
+            smooth(m.jx);
+            smooth(m.jy);
+            smooth(m.jz);
+        }
+        if (rp.ant_amp != 0.0) {
+            if (p.n == 0) zero(m.jx, 3 * nc());
+            antenna_kernel<<<nb, kThreads, 0, stream>>>(m, rp, tnow);
+            AW_LAUNCH_CHECK();
+        }
+        if (rp.cold_nc > 0.0) {
+            if (p.n == 0 && rp.ant_amp == 0.0) zero(m.jx, 3 * nc());
+            cold_fluid_kernel<<<nb, kThreads, 0, stream>>>(m, rp, vc.get(), vc.get() + nc());
+            AW_LAUNCH_CHECK();
+        }
+        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h);
+        AW_LAUNCH_CHECK();
+        ampere_kernel<<<nb, kThreads, 0, stream>>>(m);
+        AW_LAUNCH_CHECK();
+        if (rp.bnd_x) {
+            damp_kernel<<<nb, kThreads, 0, stream>>>(m, mask.get(), mask.get() + g.nx,
+                                                     rp.cold_nc > 0.0 ? vc.get() : nullptr,
+                                                     rp.cold_nc > 0.0 ? vc.get() + nc() : nullptr);
+            AW_LAUNCH_CHECK();
+        }
+        ++nstep;
+    }
+
+    // sum of the per-block pairs of mesh_stats_kernel, in order
+    void mesh_stats(int what, double out[2]) {
+        part.reserve(2 * (size_t)kRedBlocks);
+        const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (nc() + 1023) / 1024));
+        mesh_stats_kernel<<<nb, kThreads, 0, stream>>>(mesh(), what, rres.get(), r0.get(),
+                                                       part.get());
+        AW_LAUNCH_CHECK();
+        std::vector<double> h(2 * (size_t)nb);
+        AW_HIP(hipMemcpyAsync(h.data(), part.get(), h.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, stream));
+        AW_HIP(hipStreamSynchronize(stream));
+        out[0] = out[1] = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            out[0] += h[2 * b];
+            out[1] += h[2 * b + 1];
+        }
+    }
+
+    // MaxwellSimulation::residuals (include/pic/simulation_maxwell.hpp:125-160)
+    void residuals(const aw_particles& p, const float* wd, double out[2]) {
+        const int64_t n = nc();
+        rho.reserve((size_t)n);
+        rres.reserve((size_t)n);
+        r0.reserve((size_t)n);
+        jtmp.reserve((size_t)n);
+        if (p.n > 0) {
+            aw_particles q = p;
+            if (wd) {  // delta-f: the perturbation charge, weights w wd
+                wdep.reserve((size_t)p.n);
+                scale_weights(p, wd, wdep.get());
+                q.w = wdep.get();
+            }
+            ctx->acc.zero(stream);
+            deposit_rho_fixed(*ctx, q, rp.qm, AW_DEPOSIT_AUTO, nullptr, stream);
+            fix_to_float_add(ctx->acc.get(), rho.get(), n, ctx->fix_log2, true, stream, true);
+        } else {
+            zero(rho.get(), n);
+        }
+        smooth(rho.get());
+        gauss_kernel<<<blocks_for(n), kThreads, 0, stream>>>(mesh(), rho.get(), rres.get());
+        AW_LAUNCH_CHECK();
+        if (!have_ref) {
+            AW_HIP(hipMemcpyAsync(r0.get(), rres.get(), (size_t)n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, stream));
+            have_ref = true;
+        }
+        double s[2];
+        mesh_stats(1, s);
+        out[0] = std::sqrt(s[0] / (double)n);
+        out[1] = std::sqrt(s[1] / (double)n);
+    }
+
+    void scale_weights(const aw_particles& p, const float* wd, float* out);
+};
+
+namespace aw {
+namespace {
+
+__global__ void scale_weights_kernel(const float* __restrict__ w, float wu,
+                                     const float* __restrict__ wd, float* __restrict__ out,
+                                     int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t < n) out[t] = (w ? w[t] : wu) * wd[t];
+}
+
+}  // namespace
+}  // namespace aw
+
+void aw_maxwell::scale_weights(const aw_particles& p, const float* wd, float* out) {
+    scale_weights_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(p.w, (float)p.w_uniform, wd,
+                                                                   out, p.n);
+    AW_LAUNCH_CHECK();
+}
+
+extern "C" {
+
+aw_status aw_maxwell_create(const aw_grid* g, const aw_run_params* rp, aw_maxwell** out) {
+    return guard("aw_maxwell_create", [&] {
+        check_grid(g);
+        AW_REQUIRE(rp && out, "NULL argument");
+        AW_REQUIRE(rp->dt > 0 && rp->c > 0, "dt and c must be positive");
+        AW_REQUIRE(!rp->bnd_x || rp->bnd_nd > 0, "bnd_x needs bnd_nd > 0");
+        AW_REQUIRE(rp->cold_nc <= 0.0 || g->ny == 1,
+                   "cold_nc: the linearized cold fluid is ny = 1 only (Ey/Ez stagger "
+                   "degeneracy)");
+        *out = nullptr;
+        auto m = std::make_unique<aw_maxwell>();
+        m->g = *g;
+        if (m->g.Lx <= 0) m->g.Lx = g->nx * g->dx;
+        if (m->g.Ly <= 0) m->g.Ly = g->ny * g->dy;
+        m->rp = *rp;
+        m->ctx.reset(make_ctx(&m->g, 0, 0));
+        AW_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        const int64_t nc = m->nc();
+        m->F.resize((size_t)(9 * nc));
+        m->zero(m->F.get(), 9 * nc);
+        if (rp->jfilter > 0) m->jtmp.resize((size_t)nc);
+        if (rp->bnd_x) m->build_masks();
+        if (rp->cold_nc > 0.0) {
+            m->vc.resize((size_t)(2 * nc));
+            m->zero(m->vc.get(), 2 * nc);
+        }
+        AW_HIP(hipStreamSynchronize(m->stream));
+        *out = m.release();
+    });
+}
+
+aw_status aw_maxwell_destroy(aw_maxwell* m) {
+    return guard("aw_maxwell_destroy", [&] { delete m; });
+}
+
+aw_status aw_maxwell_stream(aw_maxwell* m, aw_stream* out) {
+    return guard("aw_maxwell_stream", [&] {
+        AW_REQUIRE(m && out, "NULL argument");
+        *out = (aw_stream)m->stream;
+    });
+}
+
+aw_status aw_maxwell_synchronize(aw_maxwell* m) {
+    return guard("aw_maxwell_synchronize", [&] {
+        AW_REQUIRE(m, "NULL argument");
+        AW_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+aw_status aw_maxwell_mesh(aw_maxwell* m, aw_yee_mesh* out) {
+    return guard("aw_maxwell_mesh", [&] {
+        AW_REQUIRE(m && out, "NULL argument");
+        const Mesh v = m->mesh();
+        out->ex = v.ex;
+        out->ey = v.ey;
+        out->ez = v.ez;
+        out->bx = v.bx;
+        out->by = v.by;
+        out->bz = v.bz;
+        out->jx = v.jx;
+        out->jy = v.jy;
+        out->jz = v.jz;
+        out->vcy = m->rp.cold_nc > 0.0 ? m->vc.get() : nullptr;
+        out->vcz = m->rp.cold_nc > 0.0 ? m->vc.get() + m->nc() : nullptr;
+    });
+}
+
+aw_status aw_maxwell_step(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df) {
+    return guard("aw_maxwell_step", [&] {
+        AW_REQUIRE(m && p, "NULL argument");
+        check_particles(p);
+        m->step(*p, df, (double)m->nstep * m->rp.dt);
+    });
+}
+
+aw_status aw_maxwell_step_at(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df,
+                             double tnow) {
+    return guard("aw_maxwell_step_at", [&] {
+        AW_REQUIRE(m && p, "NULL argument");
+        check_particles(p);
+        m->step(*p, df, tnow);
+    });
+}
+
+aw_status aw_maxwell_field_energy(aw_maxwell* m, double* we, double* wb) {
+    return guard("aw_maxwell_field_energy", [&] {
+        AW_REQUIRE(m && we && wb, "NULL argument");
+        double s[2];
+        m->mesh_stats(0, s);
+        *we = s[0];
+        *wb = s[1];
+    });
+}
+
+aw_status aw_maxwell_residuals(aw_maxwell* m, const aw_particles* p, const float* wd,
+                               double* divb_rms, double* gauss_drift_rms) {
+    return guard("aw_maxwell_residuals", [&] {
+        AW_REQUIRE(m && p && divb_rms && gauss_drift_rms, "NULL argument");
+        check_particles(p);
+        double out[2];
+        m->residuals(*p, wd, out);
+        *divb_rms = out[0];
+        *gauss_drift_rms = out[1];
+    });
+}
+
+aw_status aw_maxwell_diag_fields(aw_maxwell* m, double t, aw_fields* out) {
+    return guard("aw_maxwell_diag_fields", [&] {
+        AW_REQUIRE(m && out, "NULL argument");
+        const int64_t nc = m->nc();
+        m->dfields.reserve((size_t)(7 * nc));
+        const int pump = m->rp.pump && t < m->rp.pump_toff;
+        diag_fields_kernel<<<blocks_for(nc), kThreads, 0, m->stream>>>(m->mesh(), m->rp, t, pump,
+                                                                     m->dfields.get());
+        AW_LAUNCH_CHECK();
+        float* f = m->dfields.get();
+        *out = aw_fields{};
+        out->Ex = f;
+        out->Ey = f + nc;
+        out->Ez = f + 2 * nc;
+        out->Bx = f + 3 * nc;
+        out->By = f + 4 * nc;
+        out->Bz = f + 5 * nc;
+        out->ELx = f + 6 * nc;
+    });
+}
+
+aw_status aw_wd_stats(const float* wd, int64_t n, double* sum, double* rms, double* max_abs,
+                      aw_stream s) {
+    return guard("aw_wd_stats", [&] {
+        AW_REQUIRE(sum && rms && max_abs, "NULL argument");
+        *sum = *rms = *max_abs = 0.0;
+        if (n <= 0) return;
+        AW_REQUIRE(wd, "wd is NULL");
+        const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 4095) / 4096));
+        DevBuf<double> part((size_t)(3 * nb));
+        wd_stats_kernel<<<nb, kThreads, 0, S(s)>>>(wd, n, part.get());
+        AW_LAUNCH_CHECK();
+        std::vector<double> h(3 * (size_t)nb);
+        AW_HIP(hipMemcpyAsync(h.data(), part.get(), h.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, S(s)));
+        AW_HIP(hipStreamSynchronize(S(s)));
+        double s1 = 0.0, s2 = 0.0, mx = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            s1 += h[3 * b];
+            s2 += h[3 * b + 1];
+            mx = std::max(mx, h[3 * b + 2]);
+        }
+        *sum = s1;
+        *rms = std::sqrt(s2 / (double)n);
+        *max_abs = mx;
+    });
+}
+
+aw_status aw_mirror_count(const aw_grid* g, const aw_run_params* rp, const aw_species* q,
+                          int64_t* n) {
+    return guard("aw_mirror_count", [&] {
+        check_grid(g);
+        AW_REQUIRE(rp && q && n, "NULL argument");
+        check_mirror(*g, *rp, *q);
+        *n = mirror_offsets(*g, *rp, *q).back();
+    });
+}
+
+aw_status aw_load_mirror(const aw_grid* g, const aw_run_params* rp, const aw_species* q,
+                         aw_particles* p, aw_stream s) {
+    return guard("aw_load_mirror", [&] {
+        check_grid(g);
+        AW_REQUIRE(rp && q && p, "NULL argument");
+        check_mirror(*g, *rp, *q);
+        check_particles(p);
+        const std::vector<int32_t> off = mirror_offsets(*g, *rp, *q);
+        AW_REQUIRE(p->n == off.back(), "particle count must equal aw_mirror_count");
+        if (p->n == 0) return;
+        DevBuf<int32_t> doff(off.size());
+        AW_HIP(hipMemcpyAsync(doff.get(), off.data(), off.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, S(s)));
+        MirrorLoad L{};
+        L.off = doff.get();
+        L.nx = g->nx;
+        L.dx = (float)g->dx;
+        L.uth_par = q->uth[0];
+        L.uth_perp_eq = q->uth[1];
+        L.lc_rho = q->lc_rho;
+        L.lc_kappa = q->lc_kappa;
+        L.weight = (float)(q->density * g->dx * g->dy / q->ppc);
+        L.seed = rp->rng_seed;
+        L.dist = q->dist;
+        L.n = p->n;
+        mirror_load_kernel<<<blocks_for(p->n), kThreads, 0, S(s)>>>(L, *rp, p->x, p->y, p->ux,
+                                                                  p->uy, p->uz, p->w);
+        AW_LAUNCH_CHECK();
+        AW_HIP(hipStreamSynchronize(S(s)));  // doff lives in this call
+    });
+}
+
+}  // extern "C"

@@ -29,6 +29,11 @@ struct Particles {
     double w_uniform = 0.0;  // weight of every particle when w is empty
     bool sorted = false;
     DeviceArray<int> tile_off;  // tile offsets after sort_by_tile (ntiles + 1)
+    // delta-f marker weight and its precision-study accumulators (Yee branch)
+    DeviceArray<float> wd, wc;
+    DeviceArray<double> wdd;
+    bool has_wd = false;
+    int wprec = 0;
 
     Particles() = default;
 
@@ -81,6 +86,46 @@ struct Particles {
         if (n == 0) allocate(g, rp.ppc);
         aw_particles p = views();
         check(aw_load_legacy(g.ctx(), &rp, &p, to_aw(s)));  // w = rp.weight
+        sorted = false;
+    }
+
+    // Particles::enable_deltaf (:484-491): wd = 0, plus the Kahan compensation
+    // (wprec 1) or the FP64 accumulator (wprec 2).
+    void enable_deltaf(hipStream_t s = nullptr, int wprec_ = 0) {
+        wd = DeviceArray<float>(n);
+        wd.zero(s);
+        has_wd = true;
+        wprec = wprec_;
+        if (wprec == 1) {
+            wc = DeviceArray<float>(n);
+            wc.zero(s);
+        }
+        if (wprec == 2) {
+            wdd = DeviceArray<double>(n);
+            wdd.zero(s);
+        }
+    }
+
+    aw_deltaf_state deltaf_state() {
+        aw_deltaf_state d{};
+        d.wd = wd.size() ? wd.data() : nullptr;
+        d.wc = wc.size() ? wc.data() : nullptr;
+        d.wdd = wdd.size() ? wdd.data() : nullptr;
+        return d;
+    }
+
+    // Particles::initialize_mirror (:573-612): the mirror-equilibrium load of
+    // one species in the rp.b0_prof background (ny = 1); the marker count
+    // follows the profile.
+    void initialize_mirror(const Species& q, const Grid& g, const RunParams& rp,
+                           hipStream_t s) {
+        const aw_grid cg = g.to_c();
+        const aw_species cq = q.to_c();
+        int64_t count = 0;
+        check(aw_mirror_count(&cg, &rp, &cq, &count));
+        allocate_n(static_cast<std::size_t>(count));
+        aw_particles p = views();
+        check(aw_load_mirror(&cg, &rp, &cq, &p, to_aw(s)));
         sorted = false;
     }
 

@@ -68,6 +68,27 @@ public:
     // Load particles, compute the t=0 field, roll velocities back half a step
     // (ES); Darwin deposits the t=0 moments.
     void init() {
+        create();
+        check(aw_sim_init(sim_));
+    }
+
+    // Writes a restart file (ARCW v1 with the state arrays) after the steps
+    // run so far; deck is stored verbatim in the header.
+    void save_checkpoint(const std::string& path, long step, const std::string& deck = "",
+                         const std::string& git_hash = "") {
+        check(aw_sim_checkpoint_save(sim_, path.c_str(), step, step * p_.dt, deck.c_str(),
+                                     git_hash.c_str()));
+    }
+
+    // Instead of init(): the state of a file written by save_checkpoint for
+    // the same grid, species and configuration; continue with step(saved step).
+    void restore_checkpoint(const std::string& path) {
+        create();
+        check(aw_sim_checkpoint_restore(sim_, path.c_str()));
+    }
+
+private:
+    void create() {
         aw_sim_config c{};
         c.has_b0 = Cfg::has_b0 ? 1 : 0;
         c.deposit_policy = Cfg::aw_deposit;
@@ -88,9 +109,9 @@ public:
         }
         check(aw_sim_create(&g, &p_, sp.empty() ? nullptr : sp.data(),
                             static_cast<int32_t>(sp.size()), &c, &sim_));
-        check(aw_sim_init(sim_));
     }
 
+public:
     void step(long n) { check(aw_sim_step(sim_, n)); }
 
     void run() {

@@ -156,13 +156,26 @@ class Simulation:
         out["w_uniform"] = p.w_uniform
         return out
 
+    def save_checkpoint(self, path: str, step: int, deck: str = "", git_hash: str = "") -> None:
+        """Restart file (ARCW v1) of the state after the steps run so far."""
+        L.call("aw_sim_checkpoint_save", self._h, str(path).encode(), int(step),
+               C.c_double(step * self.rp.dt), deck.encode(), git_hash.encode())
+
+    def restore_checkpoint(self, path: str) -> None:
+        """Instead of init(): load a file written by save_checkpoint for the same
+        grid, species and configuration; continue with step(saved step)."""
+        L.call("aw_sim_checkpoint_restore", self._h, str(path).encode())
+
     def set_particles(self, arrays: dict) -> None:
+        """Writes x, y, ux, uy, uz (host arrays in the current particle order)
+        and rebuilds the next step's sources from them."""
         self.synchronize()
         p = self.particle_views()
         for k in ("x", "y", "ux", "uy", "uz"):
             a = np.asarray(arrays[k], dtype=np.float32)
             assert a.size == p.n
             h2d(getattr(p, k), a)
+        L.call("aw_sim_particles_changed", self._h)
 
     def fields(self) -> dict:
         self.synchronize()

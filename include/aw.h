@@ -59,8 +59,8 @@ typedef struct aw_grid {
 } aw_grid;
 
 /* RunParams -- include/pic/config.hpp:127-313, member for member (bool as
- * int32).  Only the electrostatic / Darwin / pump / loader members are read
- * by this library; the Yee-branch members are carried so decks round-trip. */
+ * int32).  The Yee-branch members (jfilter, bnd_*, deltaf and df_*, rel,
+ * cold_nc, b0_*, ant_*) are read by the aw_maxwell_* engine. */
 typedef struct aw_run_params {
     double dt, n0, vth, vd, wpe, wce;
     float B0[3];
@@ -182,8 +182,9 @@ typedef struct aw_deck {
  * applies the finalize step (c from dx_wpe_c, B0 from wce/theta_deg, pump
  * amplitudes and k0, B0-profile centre).  Host only (no GPU work).  Errors
  * carry the reference's messages ("deck: cannot open <path>", "deck: bad
- * section at line N", ...).  The dipole-profile polynomial fit (Yee-branch
- * input, bg::fit_dipole) is not computed: rp.b0_c keeps its defaults. */
+ * section at line N", ...).  profile = dipole also fits rp.b0_c / b0_sref
+ * (bg::fit_dipole, include/pic/background_b0.hpp:98-146): the even degree-5
+ * polynomial in (s/s_ref)^2 of the dipole B/B_eq, checked to 2e-4. */
 aw_status aw_deck_load(const char* path, aw_deck* out);
 
 /* ---- errors -------------------------------------------------------------- */
@@ -201,6 +202,9 @@ aw_status aw_device_free(void* p);
 aw_status aw_copy(void* dst, const void* src, size_t bytes);
 aw_status aw_memset_zero(void* p, size_t bytes, aw_stream s);
 aw_status aw_stream_synchronize(aw_stream s);
+/* cudaDeviceSynchronize for callers without the HIP headers: waits for all
+ * work on the current device (the diagnostics modules read state this way). */
+aw_status aw_device_synchronize(void);
 
 /* Default-constructed RunParams / Species: the member initialisers of
  * include/pic/config.hpp:127-313 and include/pic/species.hpp:20-45. */
@@ -380,6 +384,28 @@ aw_status aw_reduce_energies(aw_ctx* ctx, const aw_particles* p, const float* Ex
                              const float* Ey, const float* rho, double eps0,
                              aw_diag_sample* out, aw_stream s);
 
+/* Module diagnostics (include/pic/diag) reduce on the device: */
+
+/* f(x, v_par/v_th) counts over all particles, v_par = ux*bx + uz*bz (b the
+ * unit B0 direction in the x-z plane): counts[iv*nxb + ix] += 1 (uint32,
+ * the caller zeroes).  fp64 = 0: the binning of phase_hist_kernel
+ * (include/pic/diag/module.hpp:58-70; FP32, x wrapped in cells,
+ * ix = (int)(x/nx*nxb)); fp64 = 1: the binning of Snapshot's host loop
+ * (include/pic/diag/snapshot.hpp:55-62; FP64, x*dx wrapped by Lx).  Both
+ * compute iv = (int)((v_par - vlo)/(vhi - vlo)*nvb) and skip bins outside
+ * [0, nxb) x [0, nvb) after the truncating cast, as the reference does. */
+aw_status aw_phase_hist(const aw_particles* p, const aw_grid* g, int32_t nxb, int32_t nvb,
+                        double bx, double bz, double vth, double vlo, double vhi, int32_t fp64,
+                        uint32_t* counts, aw_stream s);
+/* diag::sum_sq (include/pic/diag/module.hpp:41-46) on the device: FP64
+ * sum of (a[i] - offset)^2 over n floats, in a fixed order
+ * (deterministic).  Blocks until *out (host) holds the result. */
+aw_status aw_sum_sq(const float* a, int64_t n, double offset, double* out, aw_stream s);
+/* dst[k] = src[idx[k]] for k < m: PhaseFrames' fixed particle sample
+ * (include/pic/diag/phase_frames.hpp:36-50) gathered on the device. */
+aw_status aw_gather_f32(const float* src, const int32_t* idx, int64_t m, float* dst,
+                        aw_stream s);
+
 /* ---- simulation (Simulation<Cfg>, include/pic/simulation.hpp:38-140) ------------- */
 
 typedef struct aw_sim aw_sim;
@@ -432,6 +458,24 @@ aw_status aw_sim_sources(aw_sim* sim, aw_sources* out);
 aw_status aw_sim_history(aw_sim* sim, aw_diag_sample* out, int64_t cap, int64_t* count);
 /* Diagnostics::compute now (blocking). */
 aw_status aw_sim_diag_now(aw_sim* sim, int64_t step, aw_diag_sample* out);
+/* Restart files of a Simulation (no reference counterpart beyond the file
+ * layout above, which the reference defines but fills with headers only):
+ * save writes the particles, the fixed-point sources the next step reads
+ * (ES: rho; Darwin: rho, J, amu) and the ES field, with the header's step /
+ * time / rng_seed and the deck text.  restore loads such a file into a
+ * created (not initialised) simulation of the same grid, species and model;
+ * stepping then continues with the same rho, fields and particle multiset bit
+ * for bit as the run that saved it (the particles are re-sorted).  Plain runs
+ * only (no Darwin batches). */
+aw_status aw_sim_checkpoint_save(aw_sim* sim, const char* path, int64_t step, double time,
+                                 const char* deck, const char* git_hash);
+aw_status aw_sim_checkpoint_restore(aw_sim* sim, const char* path);
+/* The caller wrote new particle values into the buffers of aw_sim_particles
+ * (positions in [0, nx) x [0, ny)).  The reference's step deposits its
+ * sources from the particles as they are (include/pic/simulation.hpp:80-93),
+ * so tools that edit sim.particles() between steps keep working: the next
+ * step re-sorts and takes rho (and, Darwin, J and amu) from the edited state. */
+aw_status aw_sim_particles_changed(aw_sim* sim);
 /* Forces a re-sort before the next step (tests; normally cadence/adaptive). */
 aw_status aw_sim_request_sort(aw_sim* sim);
 /* Event timing of the dominant particle kernel, recorded on the sim stream
@@ -445,6 +489,121 @@ aw_status aw_sim_kernel_time(aw_sim* sim, double* total_ms, int64_t* launches,
  * cfg.autotune was set. */
 aw_status aw_sim_exec_config(aw_sim* sim, int32_t* tile_x, int32_t* tile_y,
                              int32_t* blocks_per_tile);
+
+/* ---- checkpoint / restart (include/pic/checkpoint.hpp:1-131) ----------------------
+ * The ARCW v1 file: "ARCW", u32 version, u64 rng_seed, i64 step, f64 time,
+ * f64 S_d, f64 eps_L; u64-length deck text; u64-length git hash; u32 array
+ * count and per array u32-length name, u32 dtype, u64 count; then the
+ * payloads in manifest order (a schema-only file stops before them). */
+
+#define AW_CK_F32 0u
+#define AW_CK_F64 1u
+#define AW_CK_I32 2u
+#define AW_CK_U64 3u
+
+typedef struct aw_checkpoint_header { /* CheckpointHeader (:40-52) */
+    uint32_t version;                 /* read back; written as 1 */
+    uint64_t rng_seed;
+    int64_t step;
+    double time, sd, eps_l;
+} aw_checkpoint_header;
+
+typedef struct aw_checkpoint_array {  /* CheckpointArray (:34-38) + its payload */
+    const char* name;
+    uint32_t dtype;                   /* AW_CK_* */
+    uint64_t count;
+    void* data;                       /* NULL: manifest entry only */
+    int32_t on_device;                /* 1: device memory (staged through pinned host) */
+} aw_checkpoint_array;
+
+typedef struct aw_checkpoint aw_checkpoint;
+
+/* checkpoint_write_header (:65-90) plus the payloads of every array with
+ * data (host or device).  deck / git_hash may be NULL (empty). */
+aw_status aw_checkpoint_write(const char* path, const aw_checkpoint_header* h, const char* deck,
+                              const char* git_hash, const aw_checkpoint_array* arrays,
+                              int32_t narrays);
+/* checkpoint_read_header (:92-127): "checkpoint: bad magic", "checkpoint:
+ * version N newer than reader", "checkpoint: short read" as in the
+ * reference; payloads are located, not read. */
+aw_status aw_checkpoint_open(const char* path, aw_checkpoint** out);
+aw_status aw_checkpoint_close(aw_checkpoint* ck);
+aw_status aw_checkpoint_info(const aw_checkpoint* ck, aw_checkpoint_header* h, int32_t* narrays,
+                             int64_t* deck_len, int64_t* git_len);
+/* which = 0: the deck text, 1: the git hash (cap >= length + 1). */
+aw_status aw_checkpoint_text(const aw_checkpoint* ck, int32_t which, char* buf, int64_t cap);
+aw_status aw_checkpoint_array_info(const aw_checkpoint* ck, int32_t i, char* name, int64_t cap,
+                                   uint32_t* dtype, uint64_t* count, int32_t* present);
+aw_status aw_checkpoint_read_array(const aw_checkpoint* ck, int32_t i, void* dst,
+                                   int32_t on_device);
+
+/* ---- full-Maxwell (Yee) field model (MaxwellSimulation,
+ *      include/pic/simulation_maxwell.hpp:24-227; include/pic/yee2d.hpp) ------------- */
+
+typedef struct aw_maxwell aw_maxwell;
+
+/* The mesh arrays (device, nx*ny floats each, row-major): E, B (the wave
+ * field; B0 and the pump are applied in the push), J of the last step, and
+ * the cold-fluid velocities (NULL unless rp.cold_nc > 0).  Staggering:
+ * Ex(i+1/2,j) Ey(i,j+1/2) Ez(i,j) Bx(i,j+1/2) By(i+1/2,j) Bz(i+1/2,j+1/2),
+ * Jx, Jy, Jz on the E sites (include/pic/yee2d.hpp:1-31). */
+typedef struct aw_yee_mesh {
+    float *ex, *ey, *ez, *bx, *by, *bz, *jx, *jy, *jz;
+    float *vcy, *vcz;
+} aw_yee_mesh;
+
+/* Delta-f marker weights (Particles::enable_deltaf, include/pic/particles.hpp:
+ * 484-491): wd (always), wc (the Kahan compensation, rp.df_wprec = 1) and
+ * wdd (FP64 accumulator, rp.df_wprec = 2); device arrays of n. */
+typedef struct aw_deltaf_state {
+    float* wd;
+    float* wc;
+    double* wdd;
+} aw_deltaf_state;
+
+/* MaxwellSimulation(g, rp): a zeroed mesh, the damping masks of rp.bnd_x and
+ * the cold fluid of rp.cold_nc (ny = 1 only), on its own stream. */
+aw_status aw_maxwell_create(const aw_grid* g, const aw_run_params* rp, aw_maxwell** out);
+aw_status aw_maxwell_destroy(aw_maxwell* m);
+aw_status aw_maxwell_stream(aw_maxwell* m, aw_stream* out);
+aw_status aw_maxwell_synchronize(aw_maxwell* m);
+aw_status aw_maxwell_mesh(aw_maxwell* m, aw_yee_mesh* out);
+/* MaxwellSimulation::step / step_at (:51-110): B half step; staggered gather
+ * (+ pump, + B0 or the rp.b0_prof mirror field), Boris (rp.rel: relativistic),
+ * delta-f weight update (rp.deltaf, df may be NULL otherwise), move, walls
+ * (rp.bnd_x); the charge-conserving current of the move (int64 fixed point,
+ * independent of thread and particle order); periodic wrap; rp.jfilter
+ * binomial passes; antenna and cold-fluid currents; B half step; E full
+ * step; x damping.  The caller's particle arrays are updated in place and
+ * never reordered.  step uses t = (steps so far) * dt. */
+aw_status aw_maxwell_step(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df);
+aw_status aw_maxwell_step_at(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df,
+                             double tnow);
+/* MaxwellSimulation::field_energy (:112-120): eps0/2 sum |E|^2 dA and
+ * c^2/2 sum |B|^2 dA (FP64, deterministic; blocking). */
+aw_status aw_maxwell_field_energy(aw_maxwell* m, double* we, double* wb);
+/* MaxwellSimulation::residuals (:122-161): rms div B and the rms drift of the
+ * Gauss residual div E - rho since the first call (rho the CIC node charge of
+ * p, times wd when wd is non-NULL, smoothed like J).  Blocking. */
+aw_status aw_maxwell_residuals(aw_maxwell* m, const aw_particles* p, const float* wd,
+                               double* divb_rms, double* gauss_drift_rms);
+/* The Fields the diagnostics modules read on the Yee branch (tools/arcsim.cu:
+ * 121-142): Ex..Ez (wave + pump at time t), Bx..Bz (wave + B0), ELx = wave
+ * Ex; the other members NULL.  The arrays live in the engine until the next
+ * call. */
+aw_status aw_maxwell_diag_fields(aw_maxwell* m, double t, aw_fields* out);
+/* MaxwellSimulation::wd_stats (:166-181): sum wd, rms wd, max |wd| (FP64,
+ * deterministic; blocking). */
+aw_status aw_wd_stats(const float* wd, int64_t n, double* sum, double* rms, double* max_abs,
+                      aw_stream s);
+/* Particles::initialize_mirror (include/pic/particles.hpp:573-612): the
+ * marker count of the mirror-equilibrium load of species q in the rp.b0_prof
+ * background (ny = 1, uth[1] == uth[2]), then the load itself into p (p->n
+ * must equal the count; p->w receives the equal weights when non-NULL). */
+aw_status aw_mirror_count(const aw_grid* g, const aw_run_params* rp, const aw_species* q,
+                          int64_t* n);
+aw_status aw_load_mirror(const aw_grid* g, const aw_run_params* rp, const aw_species* q,
+                         aw_particles* p, aw_stream s);
 
 /* ---- batched ensembles ------------------------------------------------------------ */
 
@@ -479,7 +638,12 @@ typedef struct aw_ensemble_stats {
 
 /* cfg: tile_x (0 -> 8 at >= 64 particles per cell, else 16), blocks_per_tile,
  * sort_every (0: re-sort only when strays exceed stray_frac), stray_frac
- * (0 -> 0.02); the other members are ignored.
+ * (0 -> 0.02), autotune (1: each run takes the tile side and workgroups per
+ * tile that the autotuner picks for it stepped alone -- tuned once per
+ * distinct grid / particle count / weights / B0 shape, cached for the
+ * process; explicit tile_x / blocks_per_tile still win); the other members
+ * are ignored.  The choice changes no result (order-free deposits).
+ * aw_ensemble_init may be called once per ensemble.
  * cfg->field_model = AW_MODEL_DARWIN builds a Darwin ensemble: a uniform
  * batch whose runs share the grid, the species layout (count, ppc, density)
  * and every step parameter (dt, qm, eps0, c, n0, B0, pump, dump_every) and

@@ -927,6 +927,348 @@ void orc_load_species(const orc_grid* g, const orc_species_init* sp, float* x, f
 /* diagnostics                                                               */
 /* ------------------------------------------------------------------------ */
 
+/* f(x, v_par/v_th) counts.  fp64 = 0: phase_hist_kernel
+ * (include/pic/diag/module.hpp:58-70) -- FP32, x wrapped in cells,
+ * vp = (ux*bx + uz*bz)/vth, ix = (int)(x/nx*nxb); fp64 = 1: Snapshot's host
+ * histogram (include/pic/diag/snapshot.hpp:55-62) -- FP64, x*dx wrapped by
+ * Lx, ix = (int)(xx/Lx*nxb).  iv = (int)((vp - vlo)/(vhi - vlo)*nvb); the
+ * casts truncate toward zero and out-of-window bins are skipped.
+ * counts[iv*nxb + ix] += 1. */
+void orc_phase_hist(const orc_grid* g, const float* x, const float* ux, const float* uz,
+                    int64_t n, int32_t nxb, int32_t nvb, double bx, double bz, double vth,
+                    double vlo, double vhi, int32_t fp64, uint32_t* counts) {
+    const double Lx = g->nx * g->dx;
+    for (int64_t i = 0; i < n; ++i) {
+        int ix, iv;
+        if (fp64) {
+            double xx = (double)x[i] * g->dx;
+            if (xx < 0.0) xx += Lx;
+            else if (xx >= Lx) xx -= Lx;
+            const double vp = ((double)ux[i] * bx + (double)uz[i] * bz) / vth;
+            ix = (int)(xx / Lx * (double)nxb);
+            iv = (int)((vp - vlo) / (vhi - vlo) * (double)nvb);
+        } else {
+            const float fnx = (float)g->nx;
+            float xx = x[i];
+            if (xx < 0.f) xx += fnx;
+            else if (xx >= fnx) xx -= fnx;
+            const float vp = (ux[i] * (float)bx + uz[i] * (float)bz) / (float)vth;
+            ix = (int)(xx / fnx * (float)nxb);
+            iv = (int)((vp - (float)vlo) / ((float)vhi - (float)vlo) * (float)nvb);
+        }
+        if (ix >= 0 && ix < nxb && iv >= 0 && iv < nvb) counts[(int64_t)iv * nxb + ix] += 1u;
+    }
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* full-Maxwell (Yee) branch: FP32 restatement of the mesh updates and the   */
+/* push, FP64 reductions                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* F holds nine nx*ny grids: ex ey ez bx by bz jx jy jz.  Staggering
+ * include/pic/yee2d.hpp:7-9; periodic one-period wrap. */
+static inline int yat(const orc_grid* g, int i, int j) {
+    if (i < 0) i += g->nx; else if (i >= g->nx) i -= g->nx;
+    if (j < 0) j += g->ny; else if (j >= g->ny) j -= g->ny;
+    return j * g->nx + i;
+}
+
+/* k_faraday (include/pic/yee2d.hpp:70-81): B += h (-curl E); every update
+ * reads the E of the previous state, so the order of cells is free. */
+void orc_yee_faraday_f32(const orc_grid* g, float* F, float h) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const float* ex = F; const float* ey = F + nc; const float* ez = F + 2 * nc;
+    float* bx = F + 3 * nc; float* by = F + 4 * nc; float* bz = F + 5 * nc;
+    const float dx = (float)g->dx, dy = (float)g->dy;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const int c = j * g->nx + i;
+            const float e = ez[c];
+            const float dez_y = (ez[yat(g, i, j + 1)] - e) / dy;
+            const float dez_x = (ez[yat(g, i + 1, j)] - e) / dx;
+            const float dey_x = (ey[yat(g, i + 1, j)] - ey[c]) / dx;
+            const float dex_y = (ex[yat(g, i, j + 1)] - ex[c]) / dy;
+            bx[c] = bx[c] - h * dez_y;
+            by[c] = by[c] + h * dez_x;
+            bz[c] = bz[c] - h * (dey_x - dex_y);
+        }
+}
+
+/* k_ampere (include/pic/yee2d.hpp:84-97): E += dt (c^2 curl B - J). */
+void orc_yee_ampere_f32(const orc_grid* g, float* F, float c2, float dt) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    float* ex = F; float* ey = F + nc; float* ez = F + 2 * nc;
+    const float* bx = F + 3 * nc; const float* by = F + 4 * nc; const float* bz = F + 5 * nc;
+    const float* jx = F + 6 * nc; const float* jy = F + 7 * nc; const float* jz = F + 8 * nc;
+    const float dx = (float)g->dx, dy = (float)g->dy;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const int c = j * g->nx + i;
+            const float b = bz[c];
+            const float dbz_y = (b - bz[yat(g, i, j - 1)]) / dy;
+            const float dbz_x = (b - bz[yat(g, i - 1, j)]) / dx;
+            const float dby_x = (by[c] - by[yat(g, i - 1, j)]) / dx;
+            const float dbx_y = (bx[c] - bx[yat(g, i, j - 1)]) / dy;
+            ex[c] = ex[c] + dt * (c2 * dbz_y - jx[c]);
+            ey[c] = ey[c] + dt * (-c2 * dbz_x - jy[c]);
+            ez[c] = ez[c] + dt * (c2 * (dby_x - dbx_y) - jz[c]);
+        }
+}
+
+/* one [1,2,1]x[1,2,1]/16 pass (include/pic/yee2d.hpp:494-508) */
+void orc_yee_binomial_f32(const orc_grid* g, const float* src, float* dst) {
+    const float w[3] = {0.25f, 0.5f, 0.25f};
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            float acc = 0.f;
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di)
+                    acc = fmaf(w[di + 1] * w[dj + 1], src[yat(g, i + di, j + dj)], acc);
+            dst[j * g->nx + i] = acc;
+        }
+}
+
+/* CIC interpolation of samples at (i + ox, j + oy) (gather_stag,
+ * include/pic/yee2d.hpp:102-112), as a fixed fma chain */
+static inline float ygather(const orc_grid* g, const float* a, float x, float y, float ox,
+                            float oy) {
+    const float xs = x - ox, ys = y - oy;
+    const float fi = floorf(xs), fj = floorf(ys);
+    const int i0 = wrapfar((int)fi, g->nx), j0 = wrapfar((int)fj, g->ny);
+    const int i1 = i0 + 1 == g->nx ? 0 : i0 + 1, j1 = j0 + 1 == g->ny ? 0 : j0 + 1;
+    const float fx = xs - fi, fy = ys - fj, gx = 1.f - fx, gy = 1.f - fy;
+    const float v00 = a[j0 * g->nx + i0], v10 = a[j0 * g->nx + i1];
+    const float v01 = a[j1 * g->nx + i0], v11 = a[j1 * g->nx + i1];
+    return fmaf(fx * fy, v11, fmaf(gx * fy, v01, fmaf(fx * gy, v10, (gx * gy) * v00)));
+}
+
+static inline void yrotate(float* u, float bx, float by, float bz, float h) {
+    const float tx = h * bx, ty = h * by, tz = h * bz;
+    const float t2 = fmaf(tz, tz, fmaf(ty, ty, tx * tx));
+    const float sf = 2.0f / (1.0f + t2);
+    const float sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    const float upx = fmaf(u[1], tz, fmaf(-u[2], ty, u[0]));
+    const float upy = fmaf(u[2], tx, fmaf(-u[0], tz, u[1]));
+    const float upz = fmaf(u[0], ty, fmaf(-u[1], tx, u[2]));
+    u[0] = fmaf(upy, sz, fmaf(-upz, sy, u[0]));
+    u[1] = fmaf(upz, sx, fmaf(-upx, sz, u[1]));
+    u[2] = fmaf(upx, sy, fmaf(-upy, sx, u[2]));
+}
+
+/* background_b0.hpp:28-55 */
+float orc_b0_profile(const orc_yee_params* p, float xph) {
+    const float d = xph - (float)p->b0_xc;
+    if (p->b0_prof == 2) {
+        const float sn = d / (float)p->b0_sref, u = sn * sn;
+        float q = (float)p->b0_c[5];
+        for (int k = 4; k >= 0; --k) q = q * u + (float)p->b0_c[k];
+        return p->B0[0] * q;
+    }
+    return p->B0[0] * (1.f + (float)p->b0_a * d * d);
+}
+
+static float b0_profile_dx(const orc_yee_params* p, float xph) {
+    const float d = xph - (float)p->b0_xc;
+    if (p->b0_prof == 2) {
+        const float sn = d / (float)p->b0_sref, u = sn * sn;
+        float dp = 5.f * (float)p->b0_c[5];
+        for (int k = 4; k >= 1; --k) dp = dp * u + (float)k * (float)p->b0_c[k];
+        return p->B0[0] * dp * 2.f * d / (float)(p->b0_sref * p->b0_sref);
+    }
+    return p->B0[0] * 2.f * (float)p->b0_a * d;
+}
+
+static double yramp(double t, double trmp, double toff) {
+    if (t >= toff) return 0.0;
+    if (t < trmp) return t / trmp;
+    if (t > toff - trmp) return (toff - t) / trmp;
+    return 1.0;
+}
+
+/* yee_advance_particle (include/pic/yee2d.hpp:145-302) in the library's FP32
+ * operation order: staggered gather of E (+ pump) and the wave B (+ B0 or the
+ * mirror field), kick, rotate with qmh / gamma_mid (rel), kick, delta-f weight
+ * update at the time-centred u, move (unwrapped), x walls.  Writes u, x, y,
+ * wd (wc, wdd), and x0, y0 (old positions), vz (uz/gamma, when non-NULL) and
+ * wdep (w wd, when non-NULL). */
+void orc_yee_push_f32(const orc_grid* g, const orc_yee_params* p, double tnow, const float* F,
+                      float* x, float* y, float* ux, float* uy, float* uz, const float* w,
+                      float w_uniform, float* wd, float* wc, double* wdd, float* x0o,
+                      float* y0o, float* vzo, float* wdep, int64_t n) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const float *ex = F, *ey = F + nc, *ez = F + 2 * nc;
+    const float *bx = F + 3 * nc, *by = F + 4 * nc, *bz = F + 5 * nc;
+    const float dx = (float)g->dx, dy = (float)g->dy, fnx = (float)g->nx;
+    for (int64_t t = 0; t < n; ++t) {
+        const float x0 = x[t], y0 = y[t];
+        float Ex = ygather(g, ex, x0, y0, 0.5f, 0.f);
+        float Ey = ygather(g, ey, x0, y0, 0.f, 0.5f);
+        float Ez = ygather(g, ez, x0, y0, 0.f, 0.f);
+        if (p->pump && tnow < p->pump_toff) {
+            const double th = p->pump_k0 * (double)x0 * (double)dx - p->pump_w0 * tnow;
+            const double r = yramp(tnow, p->pump_trmp, p->pump_toff);
+            Ex += (float)(p->pump_ex * cos(th) * r);
+            Ey += (float)(-p->pump_ey * sin(th) * r);
+            Ez += (float)(p->pump_ez * cos(th) * r);
+        }
+        const float dBx = ygather(g, bx, x0, y0, 0.f, 0.5f);
+        const float dBy = ygather(g, by, x0, y0, 0.5f, 0.f);
+        const float dBz = ygather(g, bz, x0, y0, 0.5f, 0.5f);
+        float u[3] = {ux[t], uy[t], uz[t]};
+        const float uo[3] = {u[0], u[1], u[2]};
+        const float qmh = (float)(p->qm * 0.5 * p->dt);
+        u[0] = fmaf(qmh, Ex, u[0]);
+        u[1] = fmaf(qmh, Ey, u[1]);
+        u[2] = fmaf(qmh, Ez, u[2]);
+        const float gmid = p->rel ? 1.f / sqrtf(1.f + fmaf(u[2], u[2], fmaf(u[1], u[1], u[0] * u[0])))
+                                  : 1.f;
+        if (p->b0_prof) {
+            const float xph = x0 * dx;
+            const float b0 = orc_b0_profile(p, xph);
+            const float mc = b0_profile_dx(p, xph) / (2.f * b0 * (float)p->qm);
+            yrotate(u, dBx + b0, dBy + mc * u[2], dBz - mc * u[1], qmh * gmid);
+        } else {
+            yrotate(u, dBx + p->B0[0], dBy + p->B0[1], dBz + p->B0[2], qmh * gmid);
+        }
+        u[0] = fmaf(qmh, Ex, u[0]);
+        u[1] = fmaf(qmh, Ey, u[1]);
+        u[2] = fmaf(qmh, Ez, u[2]);
+        const float gnew = p->rel ? 1.f / sqrtf(1.f + fmaf(u[2], u[2], fmaf(u[1], u[1], u[0] * u[0])))
+                                  : 1.f;
+        if (p->deltaf) {
+            const float uxc = 0.5f * (u[0] + uo[0]), uyc = 0.5f * (u[1] + uo[1]),
+                        uzc = 0.5f * (u[2] + uo[2]);
+            const float gc = p->rel ? 1.f / sqrtf(1.f + fmaf(uzc, uzc, fmaf(uyc, uyc, uxc * uxc)))
+                                    : 1.f;
+            const float Fx = Ex + gc * (uyc * dBz - uzc * dBy);
+            const float Fy = Ey + gc * (uzc * dBx - uxc * dBz);
+            const float Fz = Ez + gc * (uxc * dBy - uyc * dBx);
+            float tperp = (float)p->df_tperp, t2 = (float)(p->df_kappa * p->df_tperp);
+            if (p->b0_prof) {
+                const float b = orc_b0_profile(p, x0 * dx) / p->B0[0];
+                const float par = (1.f - 1.f / b) / (float)p->df_tpar;
+                tperp = 1.f / (par + (1.f / b) / (float)p->df_tperp);
+                t2 = 1.f / (par + (1.f / b) / (float)(p->df_kappa * p->df_tperp));
+            }
+            float perp = 1.f / tperp;
+            if (p->df_dist == 1) {
+                const float u2p = fmaf(uzc, uzc, uyc * uyc);
+                const float gg = (float)p->df_rho * expf(-0.5f * u2p * (1.f / t2 - 1.f / tperp));
+                const float omg = fmaxf(1.f - gg, 1e-3f);
+                perp = fmaxf((1.f / tperp - gg / t2) / omg, -10.f / tperp);
+            }
+            const float S = Fx * uxc / (float)p->df_tpar + (Fy * uyc + Fz * uzc) * perp;
+            const float relax = p->df_taud > 0.0 ? (float)(1.0 - p->dt / p->df_taud) : 1.f;
+            const float hi = 1.f - 1e-4f, lo = -8.f;
+            float wn;
+            if (p->df_wprec == 2) {
+                const double wo = wdd[t];
+                double v = (wo + p->dt * p->qm * (1.0 - wo) * (double)S) * (double)relax;
+                v = fmin(fmax(v, (double)lo), (double)hi);
+                wdd[t] = v;
+                wn = (float)v;
+            } else if (p->df_wprec == 1) {
+                const float wo = wd[t];
+                const float inc = (float)(p->dt * p->qm) * (1.f - wo) * S;
+                const float yk = inc - wc[t];
+                const float sum = wo + yk;
+                wc[t] = (sum - wo) - yk;
+                wn = fminf(fmaxf(sum * relax, lo), hi);
+            } else {
+                const float wo = wd[t];
+                wn = fminf(fmaxf((wo + (float)(p->dt * p->qm) * (1.f - wo) * S) * relax, lo), hi);
+            }
+            wd[t] = wn;
+            if (wdep) wdep[t] = (w ? w[t] : w_uniform) * wn;
+        }
+        float x1 = x0 + (float)((double)(u[0] * gnew) * p->dt / (double)dx);
+        const float y1 = y0 + (float)((double)(u[1] * gnew) * p->dt / (double)dy);
+        float us[3] = {u[0], u[1], u[2]};
+        if (p->bnd_x) {
+            if (x1 < 0.f) { x1 = -x1; us[0] = -u[0]; }
+            else if (x1 >= fnx) { x1 = 2.f * fnx - x1; us[0] = -u[0]; }
+            if (x1 >= fnx) x1 = nextafterf(fnx, 0.f);
+            if (p->bnd_x == 2) {
+                const float nd = (float)p->bnd_nd;
+                float d = 0.f;
+                if (x1 < nd) d = (nd - x1) / nd;
+                else if (x1 > fnx - nd) d = (x1 - (fnx - nd)) / nd;
+                if (d > 0.f) {
+                    const float f = expf((float)(-p->bnd_numax * p->dt) * d * d);
+                    us[1] = u[1] * f;
+                    us[2] = u[2] * f;
+                }
+            }
+        }
+        ux[t] = us[0]; uy[t] = us[1]; uz[t] = us[2];
+        if (vzo) vzo[t] = u[2] * gnew;
+        x0o[t] = x0; y0o[t] = y0;
+        x[t] = x1; y[t] = y1;
+    }
+}
+
+/* k_energy (include/pic/yee2d.hpp:663-678): {eps0/2 |E|^2 dA, c^2/2 |B|^2 dA} */
+void orc_yee_energy_f64(const orc_grid* g, const float* F, double c2, double* out2) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const double dA = g->dx * g->dy;
+    double we = 0.0, wb = 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+        for (int q = 0; q < 3; ++q) we += 0.5 * (double)F[q * nc + c] * F[q * nc + c] * dA;
+        for (int q = 3; q < 6; ++q) wb += 0.5 * c2 * (double)F[q * nc + c] * F[q * nc + c] * dA;
+    }
+    out2[0] = we;
+    out2[1] = wb;
+}
+
+/* rms div B on the Yee mesh (include/pic/yee2d.hpp:640-643) */
+double orc_yee_divb_rms(const orc_grid* g, const float* F) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const float* bx = F + 3 * nc; const float* by = F + 4 * nc;
+    double s = 0.0;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const int c = j * g->nx + i;
+            const double d = ((double)bx[yat(g, i + 1, j)] - bx[c]) / g->dx +
+                             ((double)by[yat(g, i, j + 1)] - by[c]) / g->dy;
+            s += d * d;
+        }
+    return sqrt(s / (double)nc);
+}
+
+/* div E at the nodes (include/pic/yee2d.hpp:614-623), FP64 */
+void orc_yee_div_e(const orc_grid* g, const float* F, double* out) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const float* ex = F; const float* ey = F + nc;
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const int c = j * g->nx + i;
+            out[c] = ((double)ex[c] - ex[yat(g, i - 1, j)]) / g->dx +
+                     ((double)ey[c] - ey[yat(g, i, j - 1)]) / g->dy;
+        }
+}
+
+/* marker offsets of the mirror load (include/pic/particles.hpp:585-597) */
+int64_t orc_mirror_offsets(const orc_grid* g, const orc_yee_params* p, int32_t ppc,
+                           double uth_par, double uth_perp, int32_t dist, double lc_rho,
+                           double lc_kappa, int32_t* off) {
+    const double tpa = uth_par * uth_par, tpe = uth_perp * uth_perp;
+    int64_t tot = 0;
+    for (int c = 0; c < g->nx; ++c) {
+        const double b = (double)orc_b0_profile(p, (c + 0.5f) * (float)g->dx) / (double)p->B0[0];
+        const double t1 = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / tpe);
+        double nfac = t1 / tpe;
+        if (dist == 1) {
+            const double t2 = 1.0 / ((1.0 - 1.0 / b) / tpa + (1.0 / b) / (lc_kappa * tpe));
+            nfac = (t1 - lc_rho * t2) / (tpe * (1.0 - lc_rho * lc_kappa));
+        }
+        off[c] = (int32_t)tot;
+        tot += lround(ppc * nfac);
+    }
+    off[g->nx] = (int32_t)tot;
+    return tot;
+}
+
 /* Diagnostics::compute (include/pic/diagnostics.hpp:58-108):
  * out5 = {ke, ee, total, charge, max|E|}. */
 void orc_energies(const orc_grid* g, const float* ux, const float* uy, const float* uz,

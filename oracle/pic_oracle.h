@@ -133,6 +133,33 @@ void orc_apply_modes_f32(const orc_grid* g, float* x, int64_t n, const double* a
                          int32_t nmodes);
 
 /* ---- diagnostics (double) -------------------------------------------------- */
+/* ---- full-Maxwell (Yee) branch ------------------------------------------ */
+typedef struct {
+    double qm, dt, c;
+    float B0[3];
+    int32_t rel, pump, deltaf, df_wprec, df_dist, b0_prof, bnd_x, bnd_nd;
+    double pump_ex, pump_ey, pump_ez, pump_k0, pump_w0, pump_trmp, pump_toff;
+    double df_tpar, df_tperp, df_kappa, df_rho, df_taud;
+    double b0_a, b0_xc, b0_sref, b0_c[6];
+    double bnd_numax;
+} orc_yee_params;
+void orc_yee_faraday_f32(const orc_grid* g, float* F, float h);
+void orc_yee_ampere_f32(const orc_grid* g, float* F, float c2, float dt);
+void orc_yee_binomial_f32(const orc_grid* g, const float* src, float* dst);
+float orc_b0_profile(const orc_yee_params* p, float xph);
+void orc_yee_push_f32(const orc_grid* g, const orc_yee_params* p, double tnow, const float* F,
+                      float* x, float* y, float* ux, float* uy, float* uz, const float* w,
+                      float w_uniform, float* wd, float* wc, double* wdd, float* x0o,
+                      float* y0o, float* vzo, float* wdep, int64_t n);
+void orc_yee_energy_f64(const orc_grid* g, const float* F, double c2, double* out2);
+double orc_yee_divb_rms(const orc_grid* g, const float* F);
+void orc_yee_div_e(const orc_grid* g, const float* F, double* out);
+int64_t orc_mirror_offsets(const orc_grid* g, const orc_yee_params* p, int32_t ppc,
+                           double uth_par, double uth_perp, int32_t dist, double lc_rho,
+                           double lc_kappa, int32_t* off);
+void orc_phase_hist(const orc_grid* g, const float* x, const float* ux, const float* uz,
+                    int64_t n, int32_t nxb, int32_t nvb, double bx, double bz, double vth,
+                    double vlo, double vhi, int32_t fp64, uint32_t* counts);
 void orc_energies(const orc_grid* g, const float* ux, const float* uy, const float* uz,
                   const float* w, double w_uniform, int64_t n, const float* Ex,
                   const float* Ey, const float* rho, double eps0, double* out5);

@@ -30,6 +30,21 @@ class OrcSpecies(C.Structure):
                 ("ufl", C.c_double * 3), ("noisy", C.c_int32), ("seed", C.c_uint64)]
 
 
+class OrcYee(C.Structure):
+    _fields_ = [("qm", C.c_double), ("dt", C.c_double), ("c", C.c_double),
+                ("B0", C.c_float * 3),
+                ("rel", C.c_int32), ("pump", C.c_int32), ("deltaf", C.c_int32),
+                ("df_wprec", C.c_int32), ("df_dist", C.c_int32), ("b0_prof", C.c_int32),
+                ("bnd_x", C.c_int32), ("bnd_nd", C.c_int32),
+                ("pump_ex", C.c_double), ("pump_ey", C.c_double), ("pump_ez", C.c_double),
+                ("pump_k0", C.c_double), ("pump_w0", C.c_double), ("pump_trmp", C.c_double),
+                ("pump_toff", C.c_double),
+                ("df_tpar", C.c_double), ("df_tperp", C.c_double), ("df_kappa", C.c_double),
+                ("df_rho", C.c_double), ("df_taud", C.c_double),
+                ("b0_a", C.c_double), ("b0_xc", C.c_double), ("b0_sref", C.c_double),
+                ("b0_c", C.c_double * 6), ("bnd_numax", C.c_double)]
+
+
 _L = None
 
 
@@ -322,6 +337,16 @@ def load_species(g, specs, noisy, seed, mode_amp=None):
     return arrs, weights
 
 
+def phase_hist(g, state, nxb, nvb, bx, bz, vth, vlo, vhi, fp64):
+    """uint32 counts [nvb, nxb] of f(x, v_par/v_th) (see orc_phase_hist)."""
+    x, ux, uz = _f(state["x"]), _f(state["ux"]), _f(state["uz"])
+    out = np.zeros(nvb * nxb, np.uint32)
+    lib().orc_phase_hist(C.byref(g), _p(x), _p(ux), _p(uz), C.c_int64(x.size), C.c_int32(nxb),
+                         C.c_int32(nvb), C.c_double(bx), C.c_double(bz), C.c_double(vth),
+                         C.c_double(vlo), C.c_double(vhi), C.c_int32(int(fp64)), _p(out))
+    return out.reshape(nvb, nxb)
+
+
 def energies(g, state, Ex, Ey, rho, w_uniform, eps0=1.0, w=None):
     out = np.empty(5, np.float64)
     ux, uy, uz = _f(state["ux"]), _f(state["uy"]), _f(state["uz"])
@@ -366,3 +391,93 @@ def es_adjoint_f64(g, state, w_uniform, nsteps, qm=-1.0, dt=0.1, eps0=1.0, grad=
                                     _p(ux), _p(uy), C.c_int64(n), C.c_int32(nsteps), *ptrs,
                                     C.c_int32(nt))
     return (loss, out) if grad else loss
+
+
+# ---- full-Maxwell (Yee) branch -------------------------------------------------
+
+def yee_params(**kw) -> OrcYee:
+    """OrcYee from keyword values (RunParams member names); B0 a 3-sequence."""
+    p = OrcYee()
+    p.df_kappa, p.df_tpar, p.df_tperp, p.pump_toff = 1.0, 1.0, 1.0, 1e30
+    for k, v in kw.items():
+        if k in ("B0", "b0_c"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def yee_faraday(g, F, h):
+    F = _f(F).copy()
+    lib().orc_yee_faraday_f32(C.byref(g), _p(F), C.c_float(h))
+    return F
+
+
+def yee_ampere(g, F, c2, dt):
+    F = _f(F).copy()
+    lib().orc_yee_ampere_f32(C.byref(g), _p(F), C.c_float(c2), C.c_float(dt))
+    return F
+
+
+def yee_binomial(g, a):
+    a = _f(a).ravel()
+    out = np.empty_like(a)
+    lib().orc_yee_binomial_f32(C.byref(g), _p(a), _p(out))
+    return out
+
+
+def yee_push(g, yp, tnow, F, state, w=None, w_uniform=0.0, wd=None, wc=None, wdd=None):
+    """One Yee push (orc_yee_push_f32) of copies of the state; returns
+    (new state, x0, y0, vz, wdep)."""
+    st = {k: _f(state[k]).copy() for k in ("x", "y", "ux", "uy", "uz")}
+    n = st["x"].size
+    wd = _f(wd).copy() if wd is not None else None
+    wc = _f(wc).copy() if wc is not None else None
+    wdd = _d(wdd).copy() if wdd is not None else None
+    x0, y0, vz = (np.empty(n, np.float32) for _ in range(3))
+    wdep = np.empty(n, np.float32) if yp.deltaf else None
+    lib().orc_yee_push_f32(C.byref(g), C.byref(yp), C.c_double(tnow), _p(_f(F)),
+                           *(_p(st[k]) for k in ("x", "y", "ux", "uy", "uz")),
+                           _p(_f(w) if w is not None else None), C.c_float(w_uniform),
+                           _p(wd), _p(wc), _p(wdd), _p(x0), _p(y0), _p(vz), _p(wdep),
+                           C.c_int64(n))
+    if wd is not None:
+        st["wd"] = wd
+    if wc is not None:
+        st["wc"] = wc
+    if wdd is not None:
+        st["wdd"] = wdd
+    return st, x0, y0, vz, wdep
+
+
+def yee_energy(g, F, c2):
+    out = np.empty(2, np.float64)
+    lib().orc_yee_energy_f64(C.byref(g), _p(_f(F)), C.c_double(c2), _p(out))
+    return out
+
+
+def yee_divb_rms(g, F):
+    lib().orc_yee_divb_rms.restype = C.c_double
+    return lib().orc_yee_divb_rms(C.byref(g), _p(_f(F)))
+
+
+def yee_div_e(g, F):
+    out = np.empty(g.nx * g.ny, np.float64)
+    lib().orc_yee_div_e(C.byref(g), _p(_f(F)), _p(out))
+    return out.reshape(g.ny, g.nx)
+
+
+def b0_profile(yp, xph):
+    lib().orc_b0_profile.restype = C.c_float
+    return lib().orc_b0_profile(C.byref(yp), C.c_float(xph))
+
+
+def mirror_offsets(g, yp, ppc, uth_par, uth_perp, dist=0, lc_rho=0.0, lc_kappa=0.5):
+    off = np.empty(g.nx + 1, np.int32)
+    lib().orc_mirror_offsets.restype = C.c_int64
+    lib().orc_mirror_offsets(C.byref(g), C.byref(yp), C.c_int32(ppc), C.c_double(uth_par),
+                             C.c_double(uth_perp), C.c_int32(dist), C.c_double(lc_rho),
+                             C.c_double(lc_kappa), _p(off))
+    return off

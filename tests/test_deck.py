@@ -140,3 +140,29 @@ def test_deck_profile_centre(tmp_path):
     assert d.rp.b0_prof == 1 and d.rp.b0_xc == 4.0  # defaults to Lx / 2
     assert list(d.rp.B0) == [2.0, 0.0, 0.0]
     assert isinstance(d, Deck)
+
+
+def test_deck_dipole_profile_fit(tmp_path):
+    """profile = dipole fits the even polynomial of B/B_eq along a dipole field
+    line (include/pic/background_b0.hpp:57-146): s_ref = 1.01 max(xc, Lx - xc),
+    and the fit reproduces the closed form -- B/B_eq = sqrt(1 + 3 sin^2 l)/cos^6 l
+    at arc length s = L R_E (sin l sqrt(1 + 3 sin^2 l) + asinh(sqrt(3) sin l)/sqrt(3))/2
+    -- within 2e-4 over the whole span."""
+    lre = 40.0
+    text = GRID + SPECIES + f"[background]\nwce = 1\nprofile = dipole\nlre = {lre}\n"
+    d = load_deck(_write(tmp_path, text))
+    assert d.rp.b0_prof == 2
+    assert d.rp.b0_sref == pytest.approx(1.01 * max(d.rp.b0_xc, d.Lx - d.rp.b0_xc), rel=1e-15)
+    lam = np.linspace(0.0, 0.9, 2000)
+    t = np.sin(lam)
+    s = lre * 0.5 * (t * np.sqrt(1 + 3 * t * t) + np.arcsinh(np.sqrt(3) * t) / np.sqrt(3))
+    b = np.sqrt(1 + 3 * t * t) / np.cos(lam) ** 6
+    keep = s <= d.rp.b0_sref
+    u = (s[keep] / d.rp.b0_sref) ** 2
+    poly = np.polyval(list(d.rp.b0_c)[::-1], u)
+    assert np.max(np.abs(poly - b[keep]) / b[keep]) < 2e-4
+    assert d.rp.b0_c[0] == pytest.approx(1.0, abs=2e-4)
+    # the profile needs lre
+    bad = GRID + SPECIES + "[background]\nwce = 1\nprofile = dipole\n"
+    with pytest.raises(Exception, match="lre"):
+        load_deck(_write(tmp_path, bad))

@@ -127,3 +127,70 @@ def test_torch_autograd_function(gpu):
     dv0 = U.grad[0, 0] - U.grad[1, 0]
     assert torch.isfinite(dv0)
     run.close()
+
+
+def test_config4_gradient_matches_fd_and_fp64_adjoint(gpu, oracle):
+    """The differentiable two-stream workload itself: 128^2, dx 0.5, 64 ppc
+    (1,048,576 particles), beams +-0.2, vth 0.02, 16 x-displacement modes with
+    amplitudes 1e-3 U[-1, 1] (seed 20260630), dt 0.1, 100 steps, checkpoints
+    every 10, loss EE(100).  Every dL/dA_m and dL/dv0 is compared on its own
+    (relative error per component) with the FP64 host adjoint of the same
+    initial state, and A_1, A_8, A_16 and v0 also with central differences of
+    the FP64 forward run, which pin the FP64 adjoint at this size."""
+    nx, ppc, nsteps, K = 128, 64, 100, 10
+    g = L.grid(nx, nx, nx * DX, nx * DX)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260630)
+    sp = [L.species("beam_plus", 0.5, ppc // 2, (VTH,) * 3, (V0, 0, 0)),
+          L.species("beam_minus", 0.5, ppc // 2, (VTH,) * 3, (-V0, 0, 0))]
+    A = 1e-3 * np.random.default_rng(20260630).uniform(-1, 1, 16)
+    run = AdjointRun(g, rp, sp, len(A), checkpoint_every=K)
+    loss = run.forward(A, None, nsteps)
+    gA, gU = run.backward()
+    st = run.initial_state()
+    run.close()
+    base = AdjointRun(g, rp, sp, 0, checkpoint_every=K)
+    base.forward(np.zeros(0), None, 1)
+    xb = base.initial_state()["x"].astype(np.float64)
+    base.close()
+    og = oracle.grid(nx, nx, DX, DX)
+    w = 0.5 * DX * DX / (ppc // 2)
+    loss64, gr = oracle.es_adjoint_f64(og, st, w, nsteps)
+    k1 = 2 * np.pi / nx
+    n = xb.size
+    modes = [np.sin((m + 1) * k1 * xb) / DX for m in range(len(A))]
+    gA64 = np.array([np.dot(gr["x"], s) for s in modes])
+    dv0_64 = gr["ux"][: n // 2].sum() - gr["ux"][n // 2:].sum()
+    dv0 = gU[0, 0] - gU[1, 0]
+    del gr
+
+    def fwd(dx_dir=None, du=0.0, h=0.0):
+        s = {k: st[k].astype(np.float64) for k in ("x", "y", "ux", "uy")}
+        if dx_dir is not None:
+            s["x"] = np.mod(s["x"] + h * dx_dir, nx)
+        if du:
+            s["ux"][: n // 2] += du
+            s["ux"][n // 2:] -= du
+        return oracle.es_adjoint_f64(og, s, w, nsteps, grad=False)
+
+    fd = {}
+    hA, hv = 1e-5, 1e-5
+    for m in (0, 7, 15):
+        fd[f"A{m + 1}"] = (fwd(modes[m], h=hA) - fwd(modes[m], h=-hA)) / (2 * hA)
+    fd["v0"] = (fwd(du=hv) - fwd(du=-hv)) / (2 * hv)
+    adj64 = {f"A{m + 1}": gA64[m] for m in (0, 7, 15)}
+    adj64["v0"] = dv0_64
+    gpu = {f"A{m + 1}": gA[m] for m in (0, 7, 15)}
+    gpu["v0"] = dv0
+    rel = lambda a, b: abs(a - b) / abs(b)
+    print("loss", loss, loss64, rel(loss, loss64))
+    for k in fd:
+        print(k, "fd", fd[k], "adj64", adj64[k], "gpu", gpu[k], "fd-vs-adj64",
+              rel(adj64[k], fd[k]), "gpu-vs-fd", rel(gpu[k], fd[k]))
+    errA = np.abs(gA - gA64) / np.abs(gA64)
+    print("per-mode relative error vs FP64 adjoint", errA, "dv0", rel(dv0, dv0_64))
+    assert rel(loss, loss64) < 1e-4
+    for k in fd:
+        assert rel(adj64[k], fd[k]) < 1e-5, k      # FP64 adjoint pinned by FD at this size
+        assert rel(gpu[k], fd[k]) < 1e-4, k        # the gate: GPU gradient vs FD
+    assert errA.max() < 1e-4                        # every mode on its own
+    assert rel(dv0, dv0_64) < 1e-4

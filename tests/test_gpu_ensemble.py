@@ -158,3 +158,30 @@ def test_random_ensemble_generator_and_invariants(gpu):
         assert max(q) - min(q) <= 1e-6 * abs(q[0])
         assert q[0] == pytest.approx(-small[i].grid.nx * small[i].grid.ny * 0.25, rel=1e-6)
     ens.close()
+
+
+def test_ensemble_autotune_changes_no_result(gpu):
+    """cfg.autotune picks each run's tile and workgroups per tile (the
+    autotuner of a run stepped alone); rho, fields and the particle multisets
+    stay bit-identical to the heuristic configuration."""
+    runs = _runs()
+    a = Ensemble(runs, SimConfig(sort_every=5), graph=True)
+    b = Ensemble(runs, SimConfig(sort_every=5, autotune=True), graph=True)
+    for e in (a, b):
+        e.init()
+        e.run(0, 12)
+    for i in range(len(runs)):
+        assert np.array_equal(a.rho(i), b.rho(i)), i
+        fa, fb = a.fields(i), b.fields(i)
+        assert np.array_equal(fa["Ex"], fb["Ex"]) and np.array_equal(fa["Ey"], fb["Ey"]), i
+        assert np.array_equal(_multiset(a.particles(i)), _multiset(b.particles(i))), i
+    a.close()
+    b.close()
+
+
+def test_ensemble_init_twice_is_refused(gpu):
+    e = Ensemble(_runs()[:2], SimConfig(sort_every=5))
+    e.init()
+    with pytest.raises(L.AwError, match="twice"):
+        e.init()
+    e.close()

@@ -727,3 +727,234 @@ def test_es_adjoint_matches_central_differences(oracle):
         assert abs(fd - gA[m]) <= 1e-4 * abs(gA[m]) + 1e-12, m
     fd = (f(state(A, v0 + h)) - f(state(A, v0 - h))) / (2 * h)
     assert abs(fd - gv) <= 1e-4 * abs(gv)
+
+
+# ---- module diagnostics: the phase-space histogram ------------------------------
+
+def test_phase_hist_hand_values(oracle):
+    """Particles placed by hand on a 16-cell grid (dx 0.5): the bins follow
+    from the binning rules of include/pic/diag/module.hpp:58-70 (FP32) and
+    include/pic/diag/snapshot.hpp:55-62 (FP64), including the periodic wrap,
+    the truncating cast that puts values just below the window into bin 0 and
+    the oblique B0 projection."""
+    g = oracle.grid(16, 4, 0.5, 0.5)
+    c30, s30 = np.cos(np.radians(30)), np.sin(np.radians(30))
+    cases = [  # x (cells), ux, uz, b=(bx,bz), expected (iv, ix) or None
+        (3.2, 0.3, 0.0, (1.0, 0.0), (5, 1)),
+        (-0.25, -4.99, 0.0, (1.0, 0.0), (0, 7)),     # wrapped to 15.75 cells
+        (16.5, 0.0, 0.0, (1.0, 0.0), (5, 0)),        # wrapped to 0.5 cells
+        (8.0, -5.05, 0.0, (1.0, 0.0), (0, 4)),       # (int)(-0.05) = 0: bin 0
+        (8.0, -6.0, 0.0, (1.0, 0.0), None),          # (int)(-1.0) = -1: skipped
+        (8.0, 5.0, 0.0, (1.0, 0.0), None),           # iv = nvb: skipped
+        (2.0, 0.2, 0.4, (c30, s30), (5, 1)),         # vp = 0.373
+        (10.0, 2.0, -3.0, (c30, s30), (5, 5)),       # vp = 1.732 - 1.5 = 0.232
+    ]
+    for fp64 in (0, 1):
+        for x, ux, uz, (bx, bz), exp in cases:
+            st = {"x": np.array([x], np.float32), "ux": np.array([ux], np.float32),
+                  "uz": np.array([uz], np.float32)}
+            h = oracle.phase_hist(g, st, 8, 10, bx, bz, 1.0, -5.0, 5.0, fp64)
+            vp = ux * bx + uz * bz
+            if exp is None:
+                assert h.sum() == 0, (x, ux, uz, fp64)
+            else:
+                iv = int((vp + 5.0) / 10.0 * 10)
+                assert exp[0] == iv
+                assert h.sum() == 1 and h[exp] == 1, (x, ux, uz, fp64, np.argwhere(h))
+
+
+def test_phase_hist_matches_histogram2d(oracle):
+    """A random sample inside the window against numpy's histogram2d on the
+    same bin edges: the counts agree except for values within rounding of a
+    bin edge."""
+    rng = np.random.default_rng(3)
+    nx, n = 64, 200_000
+    g = oracle.grid(nx, 8, 0.5, 0.5)
+    st = {"x": rng.uniform(0, nx, n).astype(np.float32),
+          "ux": rng.normal(0, 0.05, n).astype(np.float32),
+          "uz": rng.normal(0, 0.05, n).astype(np.float32)}
+    bx, bz, vth = np.cos(0.4), np.sin(0.4), 0.05
+    vp = (st["ux"].astype(np.float64) * bx + st["uz"].astype(np.float64) * bz) / vth
+    keep = np.abs(vp) < 5.9
+    st = {k: v[keep] for k, v in st.items()}
+    vp = vp[keep]
+    ref, _, _ = np.histogram2d(vp, st["x"].astype(np.float64) * 0.5, bins=[200, 256],
+                               range=[[-6.0, 6.0], [0.0, nx * 0.5]])
+    for fp64 in (0, 1):
+        h = oracle.phase_hist(g, st, 256, 200, bx, bz, vth, -6.0, 6.0, fp64)
+        assert h.sum() == keep.sum()
+        assert np.abs(h.astype(np.int64) - ref.astype(np.int64)).sum() <= 1e-4 * keep.sum()
+
+
+# ---- full-Maxwell (Yee) branch ---------------------------------------------------
+
+def _yee_vacuum_omega(oracle, mx, my, nx, ny, dx, dy, c, dt, nsteps):
+    """Frequency of a standing Ez mode under the oracle's Yee updates, from the
+    zero crossings of its projection (tests/test_yee_vacuum.cu)."""
+    g = oracle.grid(nx, ny, dx, dy)
+    kx, ky = 2 * np.pi * mx / (nx * dx), 2 * np.pi * my / (ny * dy)
+    X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dy)
+    mode = np.cos(kx * X + ky * Y).ravel()
+    nc = nx * ny
+    F = np.zeros(9 * nc, np.float32)
+    F[2 * nc:3 * nc] = mode
+    prev, cross = None, []
+    for n in range(1, nsteps + 1):
+        F = oracle.yee_faraday(g, F, 0.5 * dt)
+        F = oracle.yee_faraday(g, F, 0.5 * dt)
+        F = oracle.yee_ampere(g, F, c * c, dt)
+        a = float(np.dot(F[2 * nc:3 * nc].astype(np.float64), mode))
+        if prev is not None and np.sign(a) != np.sign(prev):
+            cross.append((n - 1) * dt + dt * prev / (prev - a))   # linear interpolation
+        prev = a
+    half_periods = len(cross) - 1
+    return np.pi * half_periods / (cross[-1] - cross[0])
+
+
+@pytest.mark.parametrize("mx,my", [(2, 0), (1, 2)])
+def test_yee_vacuum_dispersion(oracle, mx, my):
+    """Vacuum waves on the oracle's Yee mesh follow the discrete dispersion
+    sin^2(w dt/2)/(c dt)^2 = sin^2(kx dx/2)/dx^2 + sin^2(ky dy/2)/dy^2
+    (tests/test_yee_vacuum.cu, 3e-4): pins the curl signs and the staggering."""
+    nx, ny, dx, dy, c = 32, 16, 1.0, 1.0, 1.0
+    dt = 0.5 / (c * np.sqrt(1 / dx ** 2 + 1 / dy ** 2))
+    kx, ky = 2 * np.pi * mx / (nx * dx), 2 * np.pi * my / (ny * dy)
+    s = np.sqrt(np.sin(kx * dx / 2) ** 2 / dx ** 2 + np.sin(ky * dy / 2) ** 2 / dy ** 2)
+    w_yee = 2.0 / dt * np.arcsin(c * dt * s)
+    periods = 20
+    nsteps = int(periods * 2 * np.pi / w_yee / dt)
+    w = _yee_vacuum_omega(oracle, mx, my, nx, ny, dx, dy, c, dt, nsteps)
+    assert abs(w - w_yee) / w_yee < 3e-4, (w, w_yee)
+    # and not the continuum c|k| (the staggering is the discrete one)
+    assert abs(w_yee - c * np.hypot(kx, ky)) / w_yee > 1e-3
+
+
+def test_yee_faraday_keeps_div_b(oracle):
+    """Faraday on the Yee mesh leaves div B unchanged: from B = 0 and random E,
+    div B stays at FP32 rounding after many updates."""
+    rng = np.random.default_rng(1)
+    g = oracle.grid(24, 20, 0.7, 0.4)
+    nc = 24 * 20
+    F = np.zeros(9 * nc, np.float32)
+    F[:3 * nc] = rng.normal(0, 1, 3 * nc)
+    for _ in range(50):
+        F = oracle.yee_faraday(g, F, 0.05)
+        F = oracle.yee_ampere(g, F, 4.0, 0.1)
+    bscale = np.sqrt(np.mean(F[3 * nc:6 * nc].astype(np.float64) ** 2)) / 0.4
+    assert oracle.yee_divb_rms(g, F) < 1e-5 * bscale
+
+
+def _one(x, y, ux, uy, uz):
+    return {k: np.array([v], np.float32) for k, v in
+            (("x", x), ("y", y), ("ux", ux), ("uy", uy), ("uz", uz))}
+
+
+def test_yee_push_closed_forms(oracle):
+    """The oracle's Yee push: a uniform Ex kicks by qm dt Ex per step; a
+    linear Ex sampled at its staggered sites (i + 1/2) is gathered as the
+    same linear function of the particle position; gyration in a uniform Bz
+    conserves |u| and turns by 2 atan(qm Bz dt / 2); the move is u dt/dx."""
+    nx, ny, dx, dy = 16, 8, 0.5, 0.25
+    g = oracle.grid(nx, ny, dx, dy)
+    nc = nx * ny
+    yp = oracle.yee_params(qm=-1.0, dt=0.1, c=10.0)
+    F = np.zeros(9 * nc, np.float32)
+    F[:nc] = 0.3
+    st, x0, y0, vz, _ = oracle.yee_push(g, yp, 0.0, F, _one(5.3, 2.6, 0.0, 0.0, 0.0))
+    assert st["ux"][0] == pytest.approx(-0.1 * 0.3, rel=1e-6)
+    assert st["x"][0] == pytest.approx(5.3 + st["ux"][0] * 0.1 / dx, rel=1e-6)
+    assert x0[0] == np.float32(5.3) and y0[0] == np.float32(2.6)
+    # linear Ex(i + 1/2) = a (i + 1/2): the gather at x is a x (x away from the wrap)
+    a = 0.02
+    F = np.zeros(9 * nc, np.float32)
+    F[:nc] = np.tile(a * (np.arange(nx) + 0.5), ny)
+    for x in (3.25, 7.9, 10.5):
+        st, *_ = oracle.yee_push(g, yp, 0.0, F, _one(x, 3.1, 0.0, 0.0, 0.0))
+        assert st["ux"][0] == pytest.approx(-0.1 * a * x, rel=1e-5)
+    # gyration in B0 = (0, 0, 0.5)
+    yp = oracle.yee_params(qm=-1.0, dt=0.1, c=10.0, B0=(0.0, 0.0, 0.5))
+    F = np.zeros(9 * nc, np.float32)
+    st, *_ = oracle.yee_push(g, yp, 0.0, F, _one(4.0, 4.0, 0.2, 0.0, 0.05))
+    ang = np.arctan2(st["uy"][0], st["ux"][0])
+    assert abs(ang) == pytest.approx(2 * np.arctan(0.5 * 0.1 / 2), rel=1e-5)
+    assert np.hypot(st["ux"][0], st["uy"][0]) == pytest.approx(0.2, rel=1e-6)
+    assert st["uz"][0] == np.float32(0.05)
+
+
+def test_yee_deltaf_weight_hand_value(oracle):
+    """One delta-f weight step (include/pic/yee2d.hpp:191-271) for a particle
+    moving along B0 = x in a uniform Ex: u^n = u^{n-1/2} + qm dt/2 Ex, F = Ex,
+    S = Ex u^n_x / Tpar, wd' = wd + dt qm (1 - wd) S."""
+    nx, ny = 16, 4
+    g = oracle.grid(nx, ny, 0.5, 0.5)
+    nc = nx * ny
+    tpar, tperp, ex, u0, dt, qm = 0.04, 0.09, 1e-3, 0.3, 0.05, -1.0
+    yp = oracle.yee_params(qm=qm, dt=dt, c=10.0, B0=(0.5, 0.0, 0.0), deltaf=1,
+                           df_tpar=tpar, df_tperp=tperp)
+    F = np.zeros(9 * nc, np.float32)
+    F[:nc] = ex
+    wd0 = np.array([0.01], np.float32)
+    st, *_ , wdep = oracle.yee_push(g, yp, 0.0, F, _one(5.0, 1.0, u0, 0.0, 0.0),
+                                    w_uniform=2.0, wd=wd0)
+    un = u0 + qm * dt * ex                      # u^{n+1/2}
+    uc = 0.5 * (u0 + un)
+    S = ex * uc / tpar
+    expect = 0.01 + dt * qm * (1 - 0.01) * S
+    assert st["wd"][0] == pytest.approx(expect, rel=1e-5)
+    assert wdep[0] == pytest.approx(2.0 * expect, rel=1e-5)
+
+
+def test_yee_gauss_law_is_kept(oracle):
+    """The oracle's Yee step with the FP64 Esirkepov current keeps
+    div E - rho constant (continuity + Ampere), to FP32 rounding of E."""
+    rng = np.random.default_rng(2)
+    nx, ny, dx, dy = 16, 12, 0.5, 0.4
+    g = oracle.grid(nx, ny, dx, dy)
+    nc = nx * ny
+    yp = oracle.yee_params(qm=-1.0, dt=0.05, c=4.0, B0=(0.3, 0.0, 0.2))
+    n = 4000
+    st = {"x": rng.uniform(0, nx, n).astype(np.float32),
+          "y": rng.uniform(0, ny, n).astype(np.float32)}
+    for k in ("ux", "uy", "uz"):
+        st[k] = rng.normal(0, 0.2, n).astype(np.float32)
+    w = 0.2
+    F = np.zeros(9 * nc, np.float32)
+    r_first = None
+    for step in range(8):
+        F = oracle.yee_faraday(g, F, 0.5 * yp.dt)
+        st, x0, y0, _, _ = oracle.yee_push(g, yp, 0.0, F, st, w_uniform=w)
+        jx, jy, jz = oracle.esirkepov_f64(g, x0, y0, st["x"], st["y"], st["uz"], -1.0, yp.dt,
+                                          w_uniform=w)
+        st["x"] = np.mod(st["x"], nx).astype(np.float32)
+        st["y"] = np.mod(st["y"], ny).astype(np.float32)
+        F[6 * nc:] = np.concatenate([jx.ravel(), jy.ravel(), jz.ravel()]).astype(np.float32)
+        F = oracle.yee_faraday(g, F, 0.5 * yp.dt)
+        F = oracle.yee_ampere(g, F, 16.0, yp.dt)
+        rho = oracle.deposit_rho_f64(g, st["x"], st["y"], -1.0, w_uniform=w)
+        r = oracle.yee_div_e(g, F) - rho
+        if r_first is None:
+            r_first = r - 0.0
+        drift = np.sqrt(np.mean((r - r_first) ** 2))
+    assert drift < 1e-5 * np.sqrt(np.mean(rho ** 2))
+
+
+def test_mirror_offsets(oracle):
+    """Marker counts of the mirror load: a flat profile (b0_a = 0) gives ppc in
+    every cell; the parabolic profile gives lround(ppc T_perp(x)/T_perp_eq)
+    with 1/T_perp(x) = (1 - 1/b)/T_par + (1/b)/T_perp_eq, b = B0(x)/B0_eq,
+    symmetric about the centre."""
+    nx, dx = 40, 0.5
+    g = oracle.grid(nx, 1, dx, dx)
+    flat = oracle.yee_params(qm=-1.0, dt=0.1, c=10.0, B0=(0.5, 0, 0), b0_prof=1, b0_a=0.0,
+                             b0_xc=nx * dx / 2)
+    off = oracle.mirror_offsets(g, flat, 32, 0.1, 0.2)
+    assert (np.diff(off) == 32).all()
+    par = oracle.yee_params(qm=-1.0, dt=0.1, c=10.0, B0=(0.5, 0, 0), b0_prof=1, b0_a=0.01,
+                            b0_xc=nx * dx / 2)
+    off = oracle.mirror_offsets(g, par, 32, 0.1, 0.2)
+    cnt = np.diff(off)
+    xc = (np.arange(nx) + 0.5) * dx
+    b = 1 + 0.01 * (xc - nx * dx / 2) ** 2
+    tp = 1 / ((1 - 1 / b) / 0.01 + (1 / b) / 0.04)
+    assert (cnt == np.round(32 * tp / 0.04)).all()
+    assert (cnt == cnt[::-1]).all() and cnt[nx // 2] == 32
