@@ -1,5 +1,5 @@
-// adjoint.hip -- discrete adjoint of the electrostatic run (no B0) with
-// checkpointing.
+// adjoint.hip -- discrete adjoint of the electrostatic run (unmagnetised or
+// with B0 along z) with checkpointing.
 //
 // No reference counterpart (the reference has no gradients).  The forward
 // run is the Simulation step of this library (fused gather + push + move +
@@ -7,12 +7,16 @@
 // energy EE = eps0/2 sum |E^N|^2 dx dy at the last step.  The reverse sweep
 // is the transpose of every linear piece, step by step:
 //   x^{n+1} = x^n + c u^{n+1/2}        ->  lu += c lx
-//   u^{n+1/2} = u^{n-1/2} + a e(x^n)   ->  le = a lu;  lx += (de/dx)^T le;
+//   u^{n+1/2} = R (u^{n-1/2} + h e) + h e   (Boris, e = e(x^n), h = qm dt/2,
+//                                          R the rotation of B0 z; R = I without B0)
+//                                      ->  le = h (I + R^T) lu;  lu = R^T lu;
+//                                          lx += (de/dx)^T le;
 //                                          lE^n = gather^T le (a deposit)
 //   E^n = S rho^n,  rho^n = D(x^n)     ->  lrho = S^T lE;  lx += D_x^T lrho
 //                                          (a gather with CIC derivative weights)
-// with c = dt/dx, a = qm dt, S^T = IFFT(conj(H) FFT(.)), H = -i k/(eps0 k^2);
-// the leapfrog half step back is folded into step 0 (net kick a/2).
+// with c = dt/dx, S^T = IFFT(conj(H) FFT(.)), H = -i k/(eps0 k^2); the
+// leapfrog half step back (the same Boris form with dt -> -dt/2) is
+// transposed after step 0.  B0 along z leaves uz out of the run.
 //
 // Checkpoints every K steps hold (x, y, ux, uy) and the tile order: the
 // particles are re-sorted by tile at every checkpoint, each reverse segment
@@ -150,7 +154,9 @@ struct AdjArgs {
     const int32_t* off;
     int ntx, tx, ty, W, H, bpt;
     int nx, ny, n;
-    float cx, cy, a_kick, coef;
+    float cx, cy, coef;
+    float h, rc, rs;     // the step's Boris: u' = R (u + h e) + h e, R = [[rc, rs], [-rs, rc]]
+    float h0, rc0, rs0;  // the half step back, transposed after step 0 (h0 = 0: none)
 };
 
 struct Cic {
@@ -225,8 +231,9 @@ __device__ __forceinline__ int adj_exponent(const AdjArgs& a) {
     const double U = (double)__uint_as_float(a.bd->smax[a.slot_in]);
     const double R = a.xb ? (double)__uint_as_float(a.bd->rmax) * (double)a.rfac : 0.0;
     const double c = fmax(fabs((double)a.cx), fabs((double)a.cy));
-    const double B =
-        fabs((double)a.a_kick) * (U + c * (U + fabs((double)a.coef) * 2.0 * R)) * (1.0 + 1e-3);
+    // |le| <= (|h| + |h0|) (1 + sqrt 2) sqrt 2 |lu'| (rotations keep the 2-norm)
+    const double B = 4.0 * (fabs((double)a.h) + fabs((double)a.h0)) *
+                     (U + c * (U + fabs((double)a.coef) * 2.0 * R)) * (1.0 + 1e-3);
     if (!(B > 0.0) || !(B < 1e300)) return 0;
     int k;
     (void)frexp(B, &k);  // B < 2^k
@@ -275,7 +282,20 @@ __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
         }
         lux = fmaf(a.cx, lx, lux);
         luy = fmaf(a.cy, ly, luy);
-        const float le0 = a.a_kick * lux, le1 = a.a_kick * luy;
+        // transpose of the Boris velocity update (and, after step 0, of the
+        // half step back): le = h (I + R^T) lu, lu = R^T lu
+        float rx = a.rc * lux - a.rs * luy, ry = a.rs * lux + a.rc * luy;
+        float le0 = a.h * (lux + rx), le1 = a.h * (luy + ry);
+        lux = rx;
+        luy = ry;
+        if (a.h0 != 0.f) {
+            rx = a.rc0 * lux - a.rs0 * luy;
+            ry = a.rs0 * lux + a.rc0 * luy;
+            le0 = le0 + a.h0 * (lux + rx);  // unfused: without B0 this is exactly (qm dt/2) lu
+            le1 = le1 + a.h0 * (luy + ry);
+            lux = rx;
+            luy = ry;
+        }
         const Cic c = cic_of(a.xn[p], a.yn[p]);
         float ve[4], vf[4], dxx, dxy, dyx, dyy;
         corners(a.Ex, s_ex, W, H, orgx, orgy, a.nx, a.ny, c, ve);
@@ -461,6 +481,7 @@ struct aw_adjoint {
     DevBuf<float> lEf, lrho;
     DevBuf<int64_t> sp_end;
     bool forward_done = false;
+    bool has_b0 = false;  // B0 along z (the run is unmagnetised otherwise)
 
     ~aw_adjoint() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -562,7 +583,7 @@ struct aw_adjoint {
     void fwd_step(int64_t step) {
         solve_current();
         aw_run_params r = rp;
-        push_deposit_es(*ctx, parts(cur), off.get(), E.get(), E.get() + ncell(*ctx), r, false,
+        push_deposit_es(*ctx, parts(cur), off.get(), E.get(), E.get() + ncell(*ctx), r, has_b0,
                         strays.get(), nullptr, bpt, stream);
         (void)step;
     }
@@ -598,7 +619,7 @@ struct aw_adjoint {
         sort(ck_perm.get(), off.get());
         deposit_rho();
         solve_current();
-        push_es(*ctx, parts(cur), E.get(), E.get() + ncell(*ctx), rp, false, true, s);
+        push_es(*ctx, parts(cur), E.get(), E.get() + ncell(*ctx), rp, has_b0, true, s);
         // rho^0 again for step 0 (solve_current consumed the accumulator)
         deposit_rho();
         for (int64_t st = 0; st < steps; ++st) {
@@ -665,7 +686,12 @@ struct aw_adjoint {
         const int64_t nc = ncell(*ctx);
         const float coef = (float)(rp.qm * w_uniform / (g.dx * g.dy));
         const float cx = (float)(rp.dt / g.dx), cy = (float)(rp.dt / g.dy);
-        const float a = (float)(rp.qm * rp.dt);
+        // the Boris steps of the forward run: the step and the half step back
+        const BorisConsts bs = boris_make(rp.qm, rp.dt, rp.B0, has_b0);
+        const BorisConsts bh = boris_make(rp.qm, -0.5 * rp.dt, rp.B0, has_b0);
+        auto rot_c = [](const BorisConsts& b) {
+            return (float)(1.0 - (double)b.t[2] * (double)b.s[2]);
+        };
         // seed: lrho^N = S^T (eps0 dx dy E^N), lx = lu = 0, bounds zero (the seed
         // is kept, so the backward sweep can be repeated)
         zero_words(bd.get(), sizeof(AdjBound));
@@ -695,7 +721,7 @@ struct aw_adjoint {
                 // the push saves x^n, y^n into the store as it moves them
                 float* Ej = segE.get() + (size_t)j * 2 * nc;
                 solve_current(Ej);
-                push_deposit_es_save(*ctx, p, off.get(), Ej, Ej + nc, rp, false, strays.get(),
+                push_deposit_es_save(*ctx, p, off.get(), Ej, Ej + nc, rp, has_b0, strays.get(),
                                      nullptr, bpt, xj, s);
             }
             // the last position of this segment is the first of the next one, in
@@ -741,7 +767,12 @@ struct aw_adjoint {
                 aa.n = (int)n;
                 aa.cx = cx;
                 aa.cy = cy;
-                aa.a_kick = st == 0 ? 0.5f * a : a;
+                aa.h = bs.qmh;
+                aa.rc = rot_c(bs);
+                aa.rs = bs.s[2];
+                aa.h0 = st == 0 ? bh.qmh : 0.f;
+                aa.rc0 = rot_c(bh);
+                aa.rs0 = bh.s[2];
                 aa.coef = coef;
                 const size_t lds = (size_t)aa.W * aa.H * (16 + 12);
                 adj_step_kernel<<<ctx->ntiles * bpt, kThreads, lds, s>>>(aa);
@@ -802,7 +833,7 @@ struct aw_adjoint {
             for (size_t sidx = 0; sidx < species.size(); ++sidx) {
                 grad_ufl[3 * sidx] = tot[n_modes + 2 * sidx];
                 grad_ufl[3 * sidx + 1] = tot[n_modes + 2 * sidx + 1];
-                grad_ufl[3 * sidx + 2] = 0.0;  // uz does not enter the B0-free run
+                grad_ufl[3 * sidx + 2] = 0.0;  // uz does not enter (E in the plane, B0 along z)
             }
     }
 };
@@ -817,6 +848,8 @@ aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         AW_REQUIRE(n_modes >= 0 && n_modes <= 32, "n_modes must be in [0, 32]");
         AW_REQUIRE(checkpoint_every > 0, "checkpoint_every must be positive");
         AW_REQUIRE(rp->dt > 0 && rp->eps0 > 0, "dt and eps0 must be positive");
+        AW_REQUIRE(rp->B0[0] == 0.f && rp->B0[1] == 0.f,
+                   "the adjoint takes B0 along z only (rp.B0[0] = rp.B0[1] = 0)");
         *out = nullptr;
         auto a = std::make_unique<aw_adjoint>();
         a->g = *g;
@@ -824,6 +857,7 @@ aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         a->species.assign(sp, sp + nspecies);
         a->n_modes = n_modes;
         a->K = checkpoint_every;
+        a->has_b0 = rp->B0[2] != 0.f;
         const int64_t nc = (int64_t)g->nx * g->ny;
         std::vector<int64_t> ends;
         double w0 = -1.0;

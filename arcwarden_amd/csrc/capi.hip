@@ -168,6 +168,7 @@ struct aw_sim {
     std::vector<std::vector<double>> rmodes;        // mode amplitudes of run r
     size_t kcap = 0;                                // KE partial slots per run
     bool initialized = false, fields_ready = false, sort_requested = false;
+    bool range_error = false;  // a Darwin moment left its fixed-point range
     int64_t steps_since_sort = 0, nsorts = 0;
     unsigned long long* stray_host = nullptr;
     hipEvent_t stray_ev = nullptr;
@@ -446,8 +447,10 @@ struct aw_sim {
         if (++stray_age >= kStrayLag) AW_HIP(hipEventSynchronize(stray_ev));
         else if (hipEventQuery(stray_ev) != hipSuccess) return;
         stray_pending = false;
+        const unsigned long long v = *stray_host;
+        if (v >= kRangeFlag) range_error = true;
         const double frac = cfg.stray_frac > 0 ? cfg.stray_frac : 0.02;
-        if ((double)*stray_host > frac * (double)n * runs) sort_requested = true;
+        if ((double)(v % kRangeFlag) > frac * (double)n * runs) sort_requested = true;
     }
 
     // Tile geometry and workgroups per tile of the particle kernels.
@@ -725,6 +728,12 @@ struct aw_sim {
         if (!initialized) fail(AW_ERR_STATE, "aw_sim_step before aw_sim_init", __FILE__, __LINE__);
         const hipStream_t s = stream;
         poll_strays();
+        if (range_error)
+            fail(AW_ERR_INVALID,
+                 "Darwin moments left the fixed-point range: a particle exceeded the |u| <= "
+                 "2 max(1, c) bound the moment scales assume (contributions saturated, no sum "
+                 "wrapped)",
+                 __FILE__, __LINE__);
         if (em) {
             step_em(nstep);
             return;
@@ -1146,7 +1155,9 @@ aw_status aw_deposit_rho_j_amu(aw_ctx* ctx, const aw_particles* p, double qm, aw
         check_particles(p);
         float* const o[8] = {out->rho, out->Jx, out->Jy, out->Jz,
                              out->amu0, out->amu1, out->amu2, out->amu3};
-        for (float* q : o) AW_REQUIRE(q, "sources: rho, J and amu grids must be non-NULL");
+        bool any = false;
+        for (float* q : o) any = any || q;
+        AW_REQUIRE(any, "sources: no rho, J or amu grid given");
         deposit_rho_j_amu(*ctx, *p, qm, o, S(s));
     });
 }
@@ -1261,6 +1272,18 @@ static std::unique_ptr<aw_sim> make_sim(const aw_grid* g, const aw_run_params* r
     sim->ctx->fix_log2 = cfg->fixed_scale > 0
                              ? cfg->fixed_scale
                              : fixed_scale_for(sim->species, *rp, cfg->legacy_load != 0);
+    if (cfg->fixed_scale <= 0) {
+        // No int64 cell sum of rho can wrap: even every particle's largest
+        // contribution, |qm| w / (dx dy) 2^S, summed in one cell stays below 2^62.
+        double wmax = cfg->legacy_load ? std::fabs(rp->weight) : 0.0;
+        for (const auto& s : sim->species)
+            wmax = std::max(wmax, std::fabs(s.density) * g->dx * g->dy / s.ppc);
+        const double cmax = std::fabs(rp->qm) * wmax / (g->dx * g->dy);
+        int& S = sim->ctx->fix_log2;
+        while (S > 8 && (double)std::max<int64_t>(1, sim->n) * cmax * std::ldexp(1.0, S) >=
+                            std::ldexp(1.0, 62))
+            --S;
+    }
     return sim;
 }
 

@@ -77,6 +77,19 @@ __device__ __forceinline__ void select_run(EmArgs& a) {
     if (a.ke_part) a.ke_part += r * a.bk;
 }
 
+// Range guard of the fixed-point moments: the static scales put the largest
+// contribution allowed by |u| <= 2 max(1, c) at 2^48.  A moment beyond 2^50
+// (|u| four times that bound for J, twice for amu) marks the thread's stray
+// count with kRangeBit; the count reaches the host as stray_count +
+// kRangeFlag, and the simulation reports the overflow instead of letting
+// saturated (clamped) contributions pile up into a wrapped int64 sum.
+constexpr float kRangeLimit = 1125899906842624.0f;  // 2^50
+constexpr unsigned int kRangeBit = 0x80000000u;
+
+__device__ __forceinline__ unsigned long long stray_word(unsigned int nstray) {
+    return (unsigned long long)(nstray & ~kRangeBit) + ((nstray & kRangeBit) ? kRangeFlag : 0ull);
+}
+
 // the 8 moment values of one particle (already scaled); cq = a.cq
 __device__ __forceinline__ void moments8(const float cq[3], float mw, float ux, float uy, float uz,
                                          float v[8]) {
@@ -100,8 +113,13 @@ __device__ __forceinline__ void deposit_q(unsigned long long* s_acc, int R, int 
                                           int orgx, int orgy, const dev::CicD& d,
                                           const float* v_in, unsigned int& nstray) {
     double v[NQ];
+    bool over = false;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) v[q] = dev::clamp_fix_f64((double)v_in[q]);
+    for (int q = 0; q < NQ; ++q) {
+        over = over || fabsf(v_in[q]) > kRangeLimit;
+        v[q] = dev::clamp_fix_f64((double)v_in[q]);
+    }
+    if (over) nstray |= kRangeBit;
     const int mi = local_index(d.i0, orgx, nx), mj = local_index(d.j0, orgy, ny);
     if (dev::in_tile(mi, W - 2) && dev::in_tile(mj, H - 2)) {
         const int b = mj * W + mi;
@@ -250,7 +268,7 @@ __global__ void __launch_bounds__(kThreads) em_moments_kernel(EmArgs a) {
     }
     __syncthreads();
     flush_q<8>(smem, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
-    if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
+    if (nstray && a.strays) atomicAdd(a.strays, stray_word(nstray));
 }
 
 template <bool PER_W>
@@ -295,7 +313,7 @@ __global__ void __launch_bounds__(kThreads) dcu_kernel(EmArgs a) {
     }
     __syncthreads();
     flush_q<3>(s_acc, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
-    if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
+    if (nstray && a.strays) atomicAdd(a.strays, stray_word(nstray));
 }
 
 // One particle of em_push_kernel, its SoA values already loaded.
@@ -381,7 +399,7 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
     }
     __syncthreads();
     if constexpr (DEP) flush_q<8>(s_acc, f.R, f.W, a.acc, a.ncell, a.nx, a.ny, f.orgx, f.orgy);
-    if (nstray && a.strays) atomicAdd(a.strays, (unsigned long long)nstray);
+    if (nstray && a.strays) atomicAdd(a.strays, stray_word(nstray));
     if constexpr (KE) {
         if (threadIdx.x == 0) {
             double t = 0.0;
@@ -597,8 +615,8 @@ __global__ void convert_add_kernel(const unsigned long long* __restrict__ acc, O
                                    int nq, const double* __restrict__ inv) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    for (int q = 0; q < nq; ++q)
-        out.o[q][c] += (float)((double)(long long)acc[(size_t)q * ncell + c] * inv[q]);
+    for (int q = 0; q < nq; ++q)  // a NULL output skips its quantity
+        if (out.o[q]) out.o[q][c] += (float)((double)(long long)acc[(size_t)q * ncell + c] * inv[q]);
 }
 
 }  // namespace

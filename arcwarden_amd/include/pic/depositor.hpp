@@ -34,40 +34,29 @@ struct Depositor {
         check(aw_deposit_rho_j_amu(g.ctx(), &v, rp.qm, &o, to_aw(s)));
     }
 
-    // rho and J only (:369-408); amu goes to a scratch block.
+    // rho and J only (:369-408): the amu outputs are left out.  Stream-ordered,
+    // no allocation or synchronisation per call.
     static void charge_current(Particles& p, Sources& src, const Grid& g, const RunParams& rp,
                                hipStream_t s) {
-        DeviceArray<float> scratch(4 * static_cast<std::size_t>(g.real_size()));
-        scratch.zero(s);
         aw_particles v = p.views();
         aw_sources o = src.c_view();
-        const std::size_t n = static_cast<std::size_t>(g.real_size());
-        o.amu0 = scratch.data();
-        o.amu1 = scratch.data() + n;
-        o.amu2 = scratch.data() + 2 * n;
-        o.amu3 = scratch.data() + 3 * n;
+        o.amu0 = o.amu1 = o.amu2 = o.amu3 = nullptr;
+        o.dcux = o.dcuy = o.dcuz = nullptr;
         check(aw_deposit_rho_j_amu(g.ctx(), &v, rp.qm, &o, to_aw(s)));
-        stream_synchronize(s);
     }
     static void charge_current_sorted(Particles& p, Sources& src, const Grid& g,
                                       const RunParams& rp, hipStream_t s) {
         charge_current(p, src, g, rp, s);
     }
 
-    // amu only (:412-422); rho and J go to a scratch block.
+    // amu only (:412-422): rho and J are left out.
     static void deposit_amu(Particles& p, Sources& src, const Grid& g, const RunParams& rp,
                             hipStream_t s) {
-        DeviceArray<float> scratch(4 * static_cast<std::size_t>(g.real_size()));
-        scratch.zero(s);
         aw_particles v = p.views();
         aw_sources o = src.c_view();
-        const std::size_t n = static_cast<std::size_t>(g.real_size());
-        o.rho = scratch.data();
-        o.Jx = scratch.data() + n;
-        o.Jy = scratch.data() + 2 * n;
-        o.Jz = scratch.data() + 3 * n;
+        o.rho = o.Jx = o.Jy = o.Jz = nullptr;
+        o.dcux = o.dcuy = o.dcuz = nullptr;
         check(aw_deposit_rho_j_amu(g.ctx(), &v, rp.qm, &o, to_aw(s)));
-        stream_synchronize(s);
     }
 
     // dcu += qm^2 w/(dx dy) (E + u x B) W with E = f.Ex, Ey, Ez (E_L + pump)

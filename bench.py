@@ -56,12 +56,36 @@ def traffic_from_profiles():
     return None
 
 
+def cpu_threads() -> int:
+    """Threads for the host baselines: OMP_NUM_THREADS when set (the GPU box
+    sets it to its CPU share per GPU, 16), else every hardware thread."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return os.cpu_count() or 1
+
+
+def cpu_info() -> dict:
+    """The host the CPU baselines ran on: model name, hardware threads and
+    OMP_NUM_THREADS."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "hardware_concurrency": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(state, g_nx, g_ny, dx, w_uniform, rho, budget_s):
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
     import pic_oracle as orc
     orc.build()
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     g = orc.grid(g_nx, g_ny, dx, dx)
     pp = orc.push_params(-1.0, 0.1, (0.0, 0.0, 0.5), True)
     st = {k: np.ascontiguousarray(state[k], np.float32) for k in ("x", "y", "ux", "uy", "uz")}
@@ -74,11 +98,20 @@ def cpu_baseline(state, g_nx, g_ny, dx, w_uniform, rho, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 10:
             break
-    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle FP32 ES step (solve+push+wrap+deposit), OpenMP x{threads}, "
-                      f"{steps} step(s) of the full {g_nx}x{g_ny}x256ppc particle set "
-                      f"({n} particles) taken from the GPU state after init"}
+    out = {"value": n * steps / el, "unit": "particle-steps/s", "cores": threads,
+           "kind": "port",
+           "sample": f"oracle FP32 ES step (solve+push+wrap+deposit), OpenMP x{threads}, "
+                     f"{steps} step(s) of the full {g_nx}x{g_ny}x256ppc particle set "
+                     f"({n} particles) taken from the GPU state after init"}
+    out.update(cpu_info())
+    # the same step on one thread (the scalar host path), one step of the state
+    st1 = {k: v.copy() for k, v in st.items()}
+    t0 = time.perf_counter()
+    orc.es_step_f32(g, pp, st1, w_uniform, 40, r.copy(), nthreads=1)
+    el1 = time.perf_counter() - t0
+    out["scalar"] = {"value": n / el1, "unit": "particle-steps/s", "cores": 1,
+                     "sample": f"the same oracle step on one thread, 1 step of {n} particles"}
+    return out
 
 
 def two_stream_bench(steps: int = 200, cpu: bool = True):
@@ -121,7 +154,8 @@ def two_stream_bench(steps: int = 200, cpu: bool = True):
         out["cpu_baseline"] = {"value": n * steps / cel, "unit": "particle-steps/s", "cores": 1,
                                "kind": "port",
                                "sample": "oracle scalar FP32 ES step, one thread, the full "
-                                         "200-step run from the GPU's initial state"}
+                                         "200-step run from the GPU's initial state",
+                               **cpu_info()}
         out["gpu_vs_cpu_scalar"] = (n * steps / el) / (n * steps / cel)
     out["ee_final"] = ee_gpu
     return out
@@ -181,7 +215,7 @@ def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps:
         sys.path.insert(0, str(ROOT / "oracle"))
         import pic_oracle as orc
         orc.build()
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         cps, cel, desc, shapes, subset = 0, 0.0, [], set(), []
         for r in runs:
             shape = (r.grid.nx, sum(sp.ppc for sp in r.species))
@@ -209,7 +243,8 @@ def ensemble_bench(steps: int, graph: bool = True, cpu: bool = False, cpu_steps:
                    "kind": "port",
                    "sample": f"oracle FP32 ES step (solve+push+wrap+deposit), OpenMP x{threads}, "
                              f"{cpu_steps} steps of each of four runs of the ensemble "
-                             f"({', '.join(desc)}) from their GPU initial states"}
+                             f"({', '.join(desc)}) from their GPU initial states",
+                   **cpu_info()}
     return {"runs": len(runs), "particles": ntot, "steps": steps,
             "value": ntot * steps / bat, "unit": "particle-steps/s",
             "sequential_value": ntot * steps / seq, "speedup_vs_sequential": seq / bat,
@@ -261,7 +296,7 @@ def adjoint_bench(nsteps: int = 100, cpu_steps: int = 0):
         g = run.grid
         og = orc.grid(g.nx, g.ny, g.dx, g.dy)
         w = run.species[0].density * g.dx * g.dy / run.species[0].ppc
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         c0 = time.perf_counter()
         orc.es_adjoint_f64(og, st, w, cpu_steps, nthreads=threads)
         cel = time.perf_counter() - c0
@@ -269,7 +304,7 @@ def adjoint_bench(nsteps: int = 100, cpu_steps: int = 0):
                "cores": threads, "kind": "port",
                "sample": f"oracle FP64 host adjoint (forward + reverse sweep), OpenMP "
                          f"x{threads}, {cpu_steps} steps of the same 1,048,576-particle "
-                         f"initial state"}
+                         f"initial state", **cpu_info()}
     run.close()
     plain = t4 - t3
     n = run.n

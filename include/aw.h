@@ -359,7 +359,9 @@ aw_status aw_form_e(aw_ctx* ctx, aw_fields* f, const aw_run_params* rp, double t
  *   rho += c_p W, J += c_p (ux, uy, uz) W,
  *   amu0 += c_p (ux^2 - uy^2)/2 W, amu1 += c_p ux uy W, amu2 += c_p uz ux W,
  *   amu3 += c_p uz uy W
- * into out->rho, Jx, Jy, Jz, amu0..amu3 (all non-NULL).  Accumulates. */
+ * into out->rho, Jx, Jy, Jz, amu0..amu3; a NULL member is not deposited
+ * (charge_current passes no amu, deposit_amu no rho / J).  Accumulates;
+ * stream-ordered, no allocation after the context's first call. */
 aw_status aw_deposit_rho_j_amu(aw_ctx* ctx, const aw_particles* p, double qm, aw_sources* out,
                                aw_stream s);
 /* Depositor<Cfg>::deposit_dcu (include/pic/depositor.hpp:275-324,426-437):
@@ -675,7 +677,9 @@ aw_status aw_ensemble_history(aw_ensemble* e, int32_t run, aw_diag_sample* out, 
 /* ---- adjoint (gradients of the field energy) -------------------------------------- */
 
 /* No reference counterpart (the reference computes no gradients): the
- * discrete adjoint of the electrostatic run without B0.  The forward run is
+ * discrete adjoint of the electrostatic run, unmagnetised or with B0 along z
+ * (rp.B0[2]; the Boris rotation is transposed in the reverse sweep).  The
+ * forward run is
  * this library's ES step (fused push + fixed-point deposit, spectral Poisson
  * solve) from a species load with x-displacement modes
  * x += sum_m A_m sin(m 2 pi x / Lx) (physical units, m = 1..n_modes); the

@@ -1450,7 +1450,34 @@ void orc_solve_es_adj_f64(const orc_grid* g, double eps0, const double* lEx,
  * loss = EE(N) = eps0/2 sum |E^N|^2 dx dy.  With gx != NULL also the
  * discrete adjoint: gx, gy = dL/dx^0, dL/dy^0 (cell units), gux, guy =
  * dL/du^0.  States x^n and fields E^n are kept for the reverse sweep. */
-double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
+/* Boris step for B0 along z in FP64: u' = R(u + h e) + h e, R the rotation
+ * with t = h B0z, s = 2t/(1 + t^2): R = [[1 - t s, s], [-s, 1 - t s]]
+ * (include/pic/pusher.hpp:35-80 with only the z component of B0). */
+typedef struct { double h, c, s; } rot_z;
+
+static rot_z rot_z_make(double h, double b0z) {
+    const double t = h * b0z, sf = 2.0 * t / (1.0 + t * t);
+    rot_z r = {h, 1.0 - t * sf, sf};
+    return r;
+}
+
+static inline void boris_z_d(double* ux, double* uy, double e0, double e1, const rot_z* r) {
+    const double vx = *ux + r->h * e0, vy = *uy + r->h * e1;
+    *ux = r->c * vx + r->s * vy + r->h * e0;
+    *uy = -r->s * vx + r->c * vy + r->h * e1;
+}
+
+/* its transpose: le += h (I + R^T) lu, lu = R^T lu */
+static inline void boris_z_adj_d(double* lx, double* ly, double* le0, double* le1,
+                                 const rot_z* r) {
+    const double tx = r->c * *lx - r->s * *ly, ty = r->s * *lx + r->c * *ly;
+    *le0 += r->h * (*lx + tx);
+    *le1 += r->h * (*ly + ty);
+    *lx = tx;
+    *ly = ty;
+}
+
+double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0, double b0z,
                           double w_uniform, const double* x0, const double* y0,
                           const double* ux0, const double* uy0, int64_t n, int32_t nsteps,
                           double* gx, double* gy, double* gux, double* guy, int32_t nthreads) {
@@ -1458,6 +1485,9 @@ double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
     const int64_t nc = (int64_t)g->nx * g->ny;
     const double area = g->dx * g->dy, coef = qm * w_uniform / area;
     const double a = qm * dt, cx = dt / g->dx, cy = dt / g->dy;
+    (void)a;
+    const rot_z R1 = rot_z_make(qm * 0.5 * dt, b0z);          /* the step */
+    const rot_z Rh = rot_z_make(qm * 0.5 * (-0.5 * dt), b0z);  /* the half step back */
     const int adj = gx != NULL;
     const int64_t keep = adj ? (int64_t)nsteps + 1 : 1;
     double* X = (double*)malloc(sizeof(double) * (size_t)(n * keep));
@@ -1482,7 +1512,6 @@ double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
         double* ey = EY + nc * ks;
         orc_solve_es_f64(g, eps0, rho, ex, ey);
         if (s == nsteps) break;
-        const double kick = s == 0 ? 0.5 : 1.0;  /* half step back folded into step 0 */
         double* xn = adj ? X + n * (ks + 1) : X;
         double* yn = adj ? Y + n * (ks + 1) : Y;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -1490,9 +1519,11 @@ double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
             const cic_grad c = cic_grad_d(g, xs[p], ysv[p]);
             double e0 = 0.0, e1 = 0.0;
             for (int k = 0; k < 4; ++k) { e0 += c.w[k] * ex[c.c[k]]; e1 += c.w[k] * ey[c.c[k]]; }
-            /* u^{-1/2} = u^0 - a/2 e, then u^{1/2} = u^{-1/2} + a e: net a/2 e at step 0 */
-            ux[p] += kick * a * e0;
-            uy[p] += kick * a * e1;
+            /* step 0 starts with the half step back, u^{-1/2} = R'(u^0 + h' e) + h' e
+             * (Pusher::half_step_back, dt -> -dt/2), then every step is
+             * u^{n+1/2} = R(u^{n-1/2} + h e) + h e with the Boris rotation R of B0 z */
+            if (s == 0) boris_z_d(&ux[p], &uy[p], e0, e1, &Rh);
+            boris_z_d(&ux[p], &uy[p], e0, e1, &R1);
             xn[p] = wrap_d(xs[p] + cx * ux[p], fnx);
             yn[p] = wrap_d(ysv[p] + cy * uy[p], fny);
         }
@@ -1527,12 +1558,14 @@ double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
             const double* ysv = Y + n * s;
             const double* ex = EX + nc * s;
             const double* ey = EY + nc * s;
-            const double kick = s == 0 ? 0.5 : 1.0;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
             for (int64_t p = 0; p < n; ++p) {
                 gux[p] += cx * gx[p];
                 guy[p] += cy * gy[p];
-                const double le0 = kick * a * gux[p], le1 = kick * a * guy[p];
+                /* transpose of u' = R(u + h e) + h e: le = h (I + R^T) lu', lu = R^T lu' */
+                double le0 = 0.0, le1 = 0.0;
+                boris_z_adj_d(&gux[p], &guy[p], &le0, &le1, &R1);
+                if (s == 0) boris_z_adj_d(&gux[p], &guy[p], &le0, &le1, &Rh);
                 const cic_grad c = cic_grad_d(g, xs[p], ysv[p]);
                 double dxx = 0, dxy = 0, dyx = 0, dyy = 0;  /* d e_comp / d pos */
                 for (int k = 0; k < 4; ++k) {

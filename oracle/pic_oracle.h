@@ -174,7 +174,7 @@ int32_t orc_max_threads(void);
 /* ---- electrostatic run in FP64 and its discrete adjoint (no B0) ---------- */
 void orc_solve_es_adj_f64(const orc_grid* g, double eps0, const double* lEx,
                           const double* lEy, double* lrho);
-double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0,
+double orc_es_adjoint_f64(const orc_grid* g, double qm, double dt, double eps0, double b0z,
                           double w_uniform, const double* x0, const double* y0,
                           const double* ux0, const double* uy0, int64_t n, int32_t nsteps,
                           double* gx, double* gy, double* gux, double* guy, int32_t nthreads);

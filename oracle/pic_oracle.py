@@ -375,8 +375,8 @@ def max_threads() -> int:
 
 
 def es_adjoint_f64(g, state, w_uniform, nsteps, qm=-1.0, dt=0.1, eps0=1.0, grad=True,
-                   nthreads=None):
-    """FP64 electrostatic run (no B0) from `state` (x, y in cells, ux, uy at
+                   nthreads=None, b0z=0.0):
+    """FP64 electrostatic run (B0 = b0z along z) from `state` (x, y in cells, ux, uy at
     t = 0 before the half step back) for nsteps steps.  Returns the loss
     EE(nsteps) and, with grad, dict(x, y, ux, uy) of dL/d(initial state)."""
     x, y, ux, uy = (_d(state[k]).copy() for k in ("x", "y", "ux", "uy"))
@@ -387,7 +387,8 @@ def es_adjoint_f64(g, state, w_uniform, nsteps, qm=-1.0, dt=0.1, eps0=1.0, grad=
     f = lib().orc_es_adjoint_f64
     f.restype = C.c_double
     loss = f(C.byref(g), C.c_double(qm), C.c_double(dt),
-                                    C.c_double(eps0), C.c_double(w_uniform), _p(x), _p(y),
+                                    C.c_double(eps0), C.c_double(b0z), C.c_double(w_uniform),
+                                    _p(x), _p(y),
                                     _p(ux), _p(uy), C.c_int64(n), C.c_int32(nsteps), *ptrs,
                                     C.c_int32(nt))
     return (loss, out) if grad else loss

@@ -16,9 +16,10 @@ pytestmark = pytest.mark.gpu
 NX, DX, PPC, V0, VTH = 32, 0.5, 16, 0.2, 0.02
 
 
-def _setup(n_modes=4, seed=11, nx=NX, ppc=PPC):
+def _setup(n_modes=4, seed=11, nx=NX, ppc=PPC, b0z=0.0):
     g = L.grid(nx, nx, nx * DX, nx * DX)
-    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=seed)
+    rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=seed,
+                      B0=(0.0, 0.0, b0z))
     sp = [L.species("beam_plus", 0.5, ppc // 2, (VTH,) * 3, (V0, 0, 0)),
           L.species("beam_minus", 0.5, ppc // 2, (VTH,) * 3, (-V0, 0, 0))]
     A = 3e-3 * np.random.default_rng(seed).uniform(-1, 1, n_modes)
@@ -42,8 +43,11 @@ def test_forward_loss_equals_engine_field_energy(gpu):
     assert loss == pytest.approx(ee, rel=1e-12)
 
 
-def test_adjoint_matches_fp64_host_adjoint(gpu, oracle):
-    g, rp, sp, A = _setup()
+@pytest.mark.parametrize("b0z", [0.0, 0.45])
+def test_adjoint_matches_fp64_host_adjoint(gpu, oracle, b0z):
+    """dL/dA_m, dL/dv0 and the per-particle gradients against the FP64 host
+    adjoint, unmagnetised and with B0 along z."""
+    g, rp, sp, A = _setup(b0z=b0z)
     nsteps = 30
     run = AdjointRun(g, rp, sp, len(A), checkpoint_every=10)
     loss = run.forward(A, None, nsteps)
@@ -57,7 +61,7 @@ def test_adjoint_matches_fp64_host_adjoint(gpu, oracle):
     run.close()
     og = oracle.grid(NX, NX, DX, DX)
     w = 0.5 * DX * DX / (PPC // 2)
-    loss64, gr = oracle.es_adjoint_f64(og, st, w, nsteps)
+    loss64, gr = oracle.es_adjoint_f64(og, st, w, nsteps, b0z=b0z)
     assert loss == pytest.approx(loss64, rel=1e-4)
     for k in ("x", "y", "ux", "uy"):
         assert rel_l2(sg[k], gr[k]) < 1e-3, k
@@ -194,3 +198,11 @@ def test_config4_gradient_matches_fd_and_fp64_adjoint(gpu, oracle):
         assert rel(gpu[k], fd[k]) < 1e-4, k        # the gate: GPU gradient vs FD
     assert errA.max() < 1e-4                        # every mode on its own
     assert rel(dv0, dv0_64) < 1e-4
+
+
+
+def test_adjoint_rejects_in_plane_b0(gpu):
+    g, rp, sp, A = _setup()
+    rp.B0[0] = 0.3
+    with pytest.raises(L.AwError, match="along z"):
+        AdjointRun(g, rp, sp, len(A))

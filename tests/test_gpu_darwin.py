@@ -347,3 +347,25 @@ def test_darwin_exec_config_changes_no_result(gpu, tile, bpt, autotune, current)
     assert np.array_equal(_multiset(a), _multiset(b))
     a.close()
     b.close()
+
+
+def test_darwin_moment_range_guard(gpu):
+    """The Darwin moment scales assume |u| <= 2 max(1, c).  A particle far
+    beyond it (uz = 1e4 with c = 5, B0 along z so uz stays out of the plane)
+    saturates its contributions, and the run reports the overflow on a
+    following step instead of accumulating a wrapped sum."""
+    sim, g, rp = darwin_sim(B0=(0.0, 0.0, 0.4), pump=False)
+    sim.init()
+    st = sim.particles()
+    st["uz"][:5] = np.float32(1e4)
+    sim.set_particles(st)
+    with pytest.raises(L.AwError, match="fixed-point range"):
+        for n in range(8):
+            sim.step(n)
+    sim.close()
+    # an ordinary run never trips it
+    ok, _, _ = darwin_sim(B0=(0.0, 0.0, 0.4), pump=False)
+    ok.init()
+    for n in range(8):
+        ok.step(n)
+    ok.close()

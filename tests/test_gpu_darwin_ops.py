@@ -152,3 +152,29 @@ def test_form_e_closed_form(gpu, with_et):
         base = [EL[0] + ET[0], EL[1] + ET[1], ET[2]]
     for q in range(3):
         assert np.max(np.abs(got[q] - (base[q] + pump[q]))) < 1e-5, q
+
+
+def test_deposit_rho_j_amu_partial_outputs(gpu):
+    """NULL members of the output are skipped (the facade's charge_current
+    and deposit_amu): the grids that are given equal the full deposit's."""
+    rng = np.random.default_rng(12)
+    nx, ny, n = 48, 32, 100_000
+    st = random_particles(rng, nx, ny, n, umax=0.4)
+    g = L.grid(nx, ny, nx * 0.5, ny * 0.5)
+    names = ("rho", "Jx", "Jy", "Jz", "amu0", "amu1", "amu2", "amu3")
+    with Ctx(g) as ctx:
+        p = DevParticles(st, w_uniform=1e-3)
+        full = _grids(n, 8, nx, ny)
+        L.call("aw_deposit_rho_j_amu", ctx.h, C.byref(p.struct()), C.c_double(-1.0),
+               C.byref(_sources(full, names)), stream())
+        rj = _grids(n, 4, nx, ny)
+        L.call("aw_deposit_rho_j_amu", ctx.h, C.byref(p.struct()), C.c_double(-1.0),
+               C.byref(_sources(rj, names[:4])), stream())
+        am = _grids(n, 4, nx, ny)
+        L.call("aw_deposit_rho_j_amu", ctx.h, C.byref(p.struct()), C.c_double(-1.0),
+               C.byref(_sources(am, names[4:])), stream())
+        for a, b in zip(rj + am, full):
+            assert np.array_equal(host(a), host(b))
+        with pytest.raises(L.AwError):
+            L.call("aw_deposit_rho_j_amu", ctx.h, C.byref(p.struct()), C.c_double(-1.0),
+                   C.byref(L.AwSources()), stream())

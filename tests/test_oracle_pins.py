@@ -693,9 +693,11 @@ def test_esirkepov_wrapped_end_point_is_its_minimal_image(oracle):
 
 # ---- discrete adjoint of the ES run ---------------------------------------------------
 
-def test_es_adjoint_matches_central_differences(oracle):
-    """The FP64 adjoint (dL/dA_m through x^0, dL/dv0 through u^0) against
-    central differences of the FP64 forward run, L = EE(30)."""
+@pytest.mark.parametrize("b0z", [0.0, 0.45])
+def test_es_adjoint_matches_central_differences(oracle, b0z):
+    """The FP64 adjoint (dL/dA_m through x^0, dL/dv0 through u^0, dL/duy0)
+    against central differences of the FP64 forward run, L = EE(30),
+    unmagnetised and with B0 along z (the Boris rotation and its transpose)."""
     nx, dx, nsteps, M = 16, 0.5, 30, 4
     g = oracle.grid(nx, nx, dx, dx)
     v0, vth = 0.2, 0.02
@@ -713,12 +715,12 @@ def test_es_adjoint_matches_central_differences(oracle):
         return {"x": np.mod(xb + d, nx), "y": yb, "ux": th + sign * v0,
                 "uy": st["uy"].astype(np.float64)}
 
-    loss, gr = oracle.es_adjoint_f64(g, state(A, v0), w, nsteps)
+    loss, gr = oracle.es_adjoint_f64(g, state(A, v0), w, nsteps, b0z=b0z)
     assert loss > 0
     gA = np.array([np.sum(gr["x"] * np.sin((m + 1) * k1 * xb) / dx) for m in range(M)])
     gv = np.sum(gr["ux"] * sign)
     h = 1e-7
-    f = lambda s: oracle.es_adjoint_f64(g, s, w, nsteps, grad=False)  # noqa: E731
+    f = lambda s: oracle.es_adjoint_f64(g, s, w, nsteps, grad=False, b0z=b0z)  # noqa: E731
     for m in range(M):
         Ap, Am = A.copy(), A.copy()
         Ap[m] += h
@@ -727,6 +729,13 @@ def test_es_adjoint_matches_central_differences(oracle):
         assert abs(fd - gA[m]) <= 1e-4 * abs(gA[m]) + 1e-12, m
     fd = (f(state(A, v0 + h)) - f(state(A, v0 - h))) / (2 * h)
     assert abs(fd - gv) <= 1e-4 * abs(gv)
+    # a uniform uy shift: the rotation couples it into x when B0 is on
+    gy = np.sum(gr["uy"])
+    sp, sm = state(A, v0), state(A, v0)
+    sp["uy"] = sp["uy"] + h
+    sm["uy"] = sm["uy"] - h
+    fd = (f(sp) - f(sm)) / (2 * h)
+    assert abs(fd - gy) <= 1e-4 * abs(gy) + 1e-9 * np.max(np.abs(gA))
 
 
 # ---- module diagnostics: the phase-space histogram ------------------------------
