@@ -207,12 +207,16 @@ _SIGS = {
     "aw_sim_sources": (C.c_int32, [P, PP(AwSources)]),
     "aw_sim_history": (C.c_int32, [P, PP(AwDiagSample), C.c_int64, PP(C.c_int64)]),
     "aw_sim_diag_now": (C.c_int32, [P, C.c_int64, PP(AwDiagSample)]),
-    "aw_phase_hist": (C.c_int32, [P, PP(AwParticles), PP(AwGrid), C.c_int32, C.c_int32,
+    "aw_phase_hist": (C.c_int32, [PP(AwParticles), PP(AwGrid), C.c_int32, C.c_int32,
                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_int32, P, P]),
     "aw_sum_sq": (C.c_int32, [P, C.c_int64, C.c_double, PP(C.c_double), P]),
     "aw_gather_f32": (C.c_int32, [P, P, C.c_int64, P, P]),
     "aw_device_synchronize": (C.c_int32, []),
+    "aw_adj_push_reverse": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32,
+                                        C.c_int32, P, P, P, P, P, P, P]),
+    "aw_adj_gather_t": (C.c_int32, [P, PP(AwParticles), P, P, P, P, P]),
+    "aw_adj_deposit_t": (C.c_int32, [P, PP(AwParticles), P, C.c_double, P, P, P]),
     "aw_checkpoint_write": (C.c_int32, [C.c_char_p, PP(AwCheckpointHeader), C.c_char_p,
                                         C.c_char_p, PP(AwCheckpointArray), C.c_int32]),
     "aw_checkpoint_open": (C.c_int32, [C.c_char_p, PP(P)]),

@@ -225,8 +225,8 @@ __device__ __forceinline__ void wave_max_to(float v, unsigned int* dst) {
 // Exponent e of the step's fixed-point scale: every contribution le * W is
 // bounded by B = |a| (U + c (U + |coef| 2 R)), U the entry-state bound and R
 // the lrho bound (2 R bounds a CIC gradient of lrho), and e makes
-// 2^log2n * B * 2^e < 2^62.  The FP32 rounding of the products stays inside
-// the 1e-3 margin.
+// 2^log2n * B * 2^e < 2^62 and B * 2^e < 2^50.  The FP32 rounding of the
+// products stays inside the 1e-3 margin.
 __device__ __forceinline__ int adj_exponent(const AdjArgs& a) {
     const double U = (double)__uint_as_float(a.bd->smax[a.slot_in]);
     const double R = a.xb ? (double)__uint_as_float(a.bd->rmax) * (double)a.rfac : 0.0;
@@ -237,7 +237,8 @@ __device__ __forceinline__ int adj_exponent(const AdjArgs& a) {
     if (!(B > 0.0) || !(B < 1e300)) return 0;
     int k;
     (void)frexp(B, &k);  // B < 2^k
-    const int e = 62 - a.log2n - k;
+    // each contribution below 2^50 (the quantiser's range), their sum below 2^62
+    const int e = (62 - a.log2n < 50 ? 62 - a.log2n : 50) - k;
     return e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
 }
 
@@ -442,6 +443,126 @@ __global__ void __launch_bounds__(kThreads) grad_kernel(GradArgs a, double* __re
         for (int k = 0; k < q; ++k) part[blockIdx.x * q + k] = sh[k * kThreads];
 }
 
+
+// ---- standalone adjoint primitives (aw_adj_deposit_t / gather_t / push_reverse) ----
+
+// lx += c_p d/dx sum_k W_k lrho_k (the transpose of the CIC charge deposit
+// with respect to the particle position; c_p = qm w_p / (dx dy))
+__global__ void __launch_bounds__(kThreads)
+deposit_t_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                 const float* __restrict__ w, float coef_u, float coef_w,
+                 const float* __restrict__ lrho, float* __restrict__ lx, float* __restrict__ ly,
+                 int n, int nx, int ny) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const Cic c = cic_of(x[p], y[p]);
+    int i0, i1, j0, j1;
+    dev::wrap_pair(c.i0, nx, i0, i1);
+    dev::wrap_pair(c.j0, ny, j0, j1);
+    const float v[4] = {lrho[j0 * nx + i0], lrho[j0 * nx + i1], lrho[j1 * nx + i0],
+                        lrho[j1 * nx + i1]};
+    float gx, gy;
+    grad4(c, v, gx, gy);
+    const float coef = w ? coef_w * w[p] : coef_u;
+    lx[p] = fmaf(coef, gx, lx[p]);
+    ly[p] = fmaf(coef, gy, ly[p]);
+}
+
+// max |le| over the particles (FP32 bit pattern, non-negative)
+__global__ void __launch_bounds__(kThreads)
+le_bound_kernel(const float* __restrict__ a, const float* __restrict__ b, int n,
+                unsigned int* __restrict__ out) {
+    float m = 0.f;
+    for (int p = blockIdx.x * kThreads + threadIdx.x; p < n; p += gridDim.x * kThreads)
+        m = fmaxf(m, fmaxf(fabsf(a[p]), fabsf(b[p])));
+    wave_max_to(m, out);
+}
+
+// lE += sum_p W(x_p) le_p in int64 fixed point: scale 2^e with
+// n * max|le| * 2^e < 2^62 (order-free, deterministic)
+__device__ __forceinline__ int gather_t_exponent(const unsigned int* bound, int log2n) {
+    const double B = (double)__uint_as_float(*bound) * (1.0 + 1e-6);
+    if (!(B > 0.0) || !(B < 1e300)) return 0;
+    int k;
+    (void)frexp(B, &k);
+    // each contribution below 2^50 (the quantiser's range), their sum below 2^62
+    const int e = (62 - log2n < 50 ? 62 - log2n : 50) - k;
+    return e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
+}
+
+__global__ void __launch_bounds__(kThreads)
+gather_t_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                const float* __restrict__ lex, const float* __restrict__ ley, int n, int nx,
+                int ny, const unsigned int* __restrict__ bound, int log2n,
+                unsigned long long* __restrict__ acc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double scale = ldexp(1.0, gather_t_exponent(bound, log2n));
+    const Cic c = cic_of(x[p], y[p]);
+    const float gx = 1.f - c.fx, gy = 1.f - c.fy;
+    const float w4[4] = {gx * gy, c.fx * gy, gx * c.fy, c.fx * c.fy};
+    int i0, i1, j0, j1;
+    dev::wrap_pair(c.i0, nx, i0, i1);
+    dev::wrap_pair(c.j0, ny, j0, j1);
+    const int idx[4] = {j0 * nx + i0, j0 * nx + i1, j1 * nx + i0, j1 * nx + i1};
+    const int64_t nc = (int64_t)nx * ny;
+    const double q0 = dev::clamp_fix_f64((double)lex[p] * scale);
+    const double q1 = dev::clamp_fix_f64((double)ley[p] * scale);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        atomicAdd(&acc[idx[k]], dev::fix_mul(q0, (double)w4[k]));
+        atomicAdd(&acc[nc + idx[k]], dev::fix_mul(q1, (double)w4[k]));
+    }
+}
+
+__global__ void gather_t_convert_kernel(unsigned long long* __restrict__ acc, int64_t nc,
+                                        const unsigned int* __restrict__ bound, int log2n,
+                                        float* __restrict__ lEx, float* __restrict__ lEy) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double inv = ldexp(1.0, -gather_t_exponent(bound, log2n));
+    lEx[c] += (float)((double)(long long)acc[c] * inv);
+    lEy[c] += (float)((double)(long long)acc[nc + c] * inv);
+    acc[c] = 0ull;
+    acc[nc + c] = 0ull;
+}
+
+struct PushT {
+    const float *x, *y, *Ex, *Ey;
+    float *lx, *ly, *lux, *luy, *lex, *ley;
+    int n, nx, ny;
+    float cx, cy, h, rc, rs;
+};
+
+// The transpose of one push at positions x^n: lu += c lx (the move);
+// le = h (I + R^T) lu, lu = R^T lu (the Boris velocity update);
+// lx += (de/dx)^T le (the gather's dependence on the position).
+__global__ void __launch_bounds__(kThreads) push_t_kernel(PushT a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    float lux = fmaf(a.cx, a.lx[p], a.lux[p]), luy = fmaf(a.cy, a.ly[p], a.luy[p]);
+    const float rx = a.rc * lux - a.rs * luy, ry = a.rs * lux + a.rc * luy;
+    const float le0 = a.h * (lux + rx), le1 = a.h * (luy + ry);
+    const Cic c = cic_of(a.x[p], a.y[p]);
+    int i0, i1, j0, j1;
+    dev::wrap_pair(c.i0, a.nx, i0, i1);
+    dev::wrap_pair(c.j0, a.ny, j0, j1);
+    const int idx[4] = {j0 * a.nx + i0, j0 * a.nx + i1, j1 * a.nx + i0, j1 * a.nx + i1};
+    float ve[4], vf[4];
+    for (int k = 0; k < 4; ++k) {
+        ve[k] = a.Ex[idx[k]];
+        vf[k] = a.Ey[idx[k]];
+    }
+    float dxx, dxy, dyx, dyy;
+    grad4(c, ve, dxx, dxy);
+    grad4(c, vf, dyx, dyy);
+    a.lx[p] += le0 * dxx + le1 * dyx;
+    a.ly[p] += le0 * dxy + le1 * dyy;
+    a.lux[p] = rx;
+    a.luy[p] = ry;
+    a.lex[p] = le0;
+    a.ley[p] = le1;
+}
 }  // namespace
 
 }  // namespace aw
@@ -540,7 +661,7 @@ struct aw_adjoint {
     void solve_current(float* E2 = nullptr) {
         ctx->rin.reserve((size_t)ncell(*ctx));
         fix_to_float_add(ctx->acc.get(), rho.get(), ncell(*ctx), ctx->fix_log2, true, stream, true,
-                         strays.get(), ctx->rin.get());
+                         strays.get(), ctx->rin.get(), 1, nullptr, ctx->rho_base);
         solve_es_contig(*ctx, ctx->rin.get(), E2 ? E2 : E.get(), rp.eps0, stream);
     }
 
@@ -840,6 +961,88 @@ struct aw_adjoint {
 
 extern "C" {
 
+aw_status aw_adj_deposit_t(aw_ctx* ctx, const aw_particles* p, const float* lrho, double qm,
+                           float* lx, float* ly, aw_stream s) {
+    return guard("aw_adj_deposit_t", [&] {
+        AW_REQUIRE(ctx && lrho && lx && ly, "NULL argument");
+        check_particles(p);
+        if (p->n == 0) return;
+        const double area = ctx->g.dx * ctx->g.dy;
+        deposit_t_kernel<<<blocks_for(p->n), kThreads, 0, S(s)>>>(
+            p->x, p->y, p->w, (float)(qm * p->w_uniform / area), (float)(qm / area), lrho, lx,
+            ly, (int)p->n, ctx->g.nx, ctx->g.ny);
+        AW_LAUNCH_CHECK();
+    });
+}
+
+aw_status aw_adj_gather_t(aw_ctx* ctx, const aw_particles* p, const float* lex, const float* ley,
+                          float* lEx, float* lEy, aw_stream s) {
+    return guard("aw_adj_gather_t", [&] {
+        AW_REQUIRE(ctx && lEx && lEy, "NULL argument");
+        check_particles(p);
+        if (p->n == 0) return;
+        AW_REQUIRE(lex && ley, "lex/ley is NULL");
+        const int64_t nc = ncell(*ctx);
+        const hipStream_t st = S(s);
+        ctx->jacc.reserve((size_t)(3 * nc));
+        ctx->jbound.reserve(2);
+        AW_HIP(hipMemsetAsync(ctx->jacc.get(), 0, (size_t)(2 * nc) * sizeof(unsigned long long), st));
+        AW_HIP(hipMemsetAsync(ctx->jbound.get(), 0, sizeof(unsigned int), st));
+        unsigned int* bound = ctx->jbound.get();
+        int log2n = 0;
+        while (log2n < 62 && ((int64_t)1 << log2n) < p->n) ++log2n;
+        const int nb = std::min(blocks_for(p->n), 4 * std::max(1, ctx->num_cu));
+        le_bound_kernel<<<nb, kThreads, 0, st>>>(lex, ley, (int)p->n, bound);
+        AW_LAUNCH_CHECK();
+        gather_t_kernel<<<blocks_for(p->n), kThreads, 0, st>>>(p->x, p->y, lex, ley, (int)p->n,
+                                                             ctx->g.nx, ctx->g.ny, bound, log2n,
+                                                             ctx->jacc.get());
+        AW_LAUNCH_CHECK();
+        gather_t_convert_kernel<<<blocks_for(nc), kThreads, 0, st>>>(ctx->jacc.get(), nc, bound,
+                                                                   log2n, lEx, lEy);
+        AW_LAUNCH_CHECK();
+    });
+}
+
+aw_status aw_adj_push_reverse(aw_ctx* ctx, const aw_particles* p, const float* Ex,
+                              const float* Ey, const aw_run_params* rp, int32_t has_b0,
+                              int32_t half_back, float* lx, float* ly, float* lux, float* luy,
+                              float* lex, float* ley, aw_stream s) {
+    return guard("aw_adj_push_reverse", [&] {
+        AW_REQUIRE(ctx && Ex && Ey && rp && lx && ly && lux && luy && lex && ley,
+                   "NULL argument");
+        check_particles(p);
+        AW_REQUIRE(!has_b0 || (rp->B0[0] == 0.f && rp->B0[1] == 0.f),
+                   "the adjoint takes B0 along z only (rp.B0[0] = rp.B0[1] = 0)");
+        AW_REQUIRE(!rp->rel, "the adjoint push is non-relativistic (rp.rel = 0)");
+        if (p->n == 0) return;
+        const double dt_eff = half_back ? -0.5 * rp->dt : rp->dt;
+        const BorisConsts b = boris_make(rp->qm, dt_eff, rp->B0, has_b0 != 0);
+        PushT a{};
+        a.x = p->x;
+        a.y = p->y;
+        a.Ex = Ex;
+        a.Ey = Ey;
+        a.lx = lx;
+        a.ly = ly;
+        a.lux = lux;
+        a.luy = luy;
+        a.lex = lex;
+        a.ley = ley;
+        a.n = (int)p->n;
+        a.nx = ctx->g.nx;
+        a.ny = ctx->g.ny;
+        // the half step back does not move the particle
+        a.cx = half_back ? 0.f : (float)(rp->dt / ctx->g.dx);
+        a.cy = half_back ? 0.f : (float)(rp->dt / ctx->g.dy);
+        a.h = b.qmh;
+        a.rc = (float)(1.0 - (double)b.t[2] * (double)b.s[2]);
+        a.rs = b.s[2];
+        push_t_kernel<<<blocks_for(p->n), kThreads, 0, S(s)>>>(a);
+        AW_LAUNCH_CHECK();
+    });
+}
+
 aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
                         int32_t nspecies, int32_t n_modes, int32_t checkpoint_every,
                         int32_t tile, aw_adjoint** out) {
@@ -850,6 +1053,7 @@ aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         AW_REQUIRE(rp->dt > 0 && rp->eps0 > 0, "dt and eps0 must be positive");
         AW_REQUIRE(rp->B0[0] == 0.f && rp->B0[1] == 0.f,
                    "the adjoint takes B0 along z only (rp.B0[0] = rp.B0[1] = 0)");
+        AW_REQUIRE(!rp->rel, "the adjoint push is non-relativistic (rp.rel = 0)");
         *out = nullptr;
         auto a = std::make_unique<aw_adjoint>();
         a->g = *g;
@@ -874,6 +1078,11 @@ aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_spec
         const int t = tile > 0 ? tile : (a->n >= 64 * nc ? 8 : 16);
         a->ctx.reset(make_ctx(g, t, t));
         a->ctx->fix_log2 = fixed_scale_for(a->species, *rp, false);
+        {
+            double dens = 0.0;
+            for (const auto& s : a->species) dens += s.density;
+            a->ctx->rho_base = rho_baseline(rp->qm, dens, a->ctx->fix_log2);
+        }
         a->bpt = tile_bpt(*a->ctx, a->n);
         AW_HIP(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
         const size_t n = (size_t)a->n;

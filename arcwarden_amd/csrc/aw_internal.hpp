@@ -205,6 +205,9 @@ struct aw_ctx {
     int tx = 16, ty = 16, ntx = 0, nty = 0, ntiles = 0;
     int apron = 2;
     int fix_log2 = 40;
+    // the run's mean charge density in fixed point, subtracted from the
+    // R2C input (the solve drops k = 0; 0: nothing subtracted)
+    long long rho_base = 0;
     int device = 0, num_cu = 0;
     size_t lds_per_block = 0;
     // rocFFT
@@ -261,11 +264,16 @@ void check_particles(const aw_particles* p);
 void deposit_rho_acc(const aw_grid& g, const aw_particles& p, double qm, int policy,
                      int scale_log2, unsigned long long* acc, int num_cu, size_t lds_per_block,
                      hipStream_t s);
-// counter: ncounter counters cleared by the same launch (stray counts)
+// counter: ncounter counters cleared by the same launch (stray counts);
+// out2: a copy less the integer baseline (base_arr per cell, else base_val)
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign = false,
                       unsigned long long* counter = nullptr, float* out2 = nullptr,
-                      int ncounter = 1);
+                      int ncounter = 1, const long long* base_arr = nullptr,
+                      long long base_val = 0);
+// qm * density in fixed point at 2^scale_log2: the mean charge density of a
+// run whose species densities sum to `density`, removed from the R2C input
+long long rho_baseline(double qm, double density, int scale_log2);
 // acc == nullptr: the context's accumulator
 void deposit_rho_fixed(aw_ctx& c, const aw_particles& p, double qm, int policy,
                        const int32_t* sorted_off, hipStream_t s,

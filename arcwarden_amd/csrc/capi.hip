@@ -73,7 +73,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                        hipStream_t s);
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
                    const double* inv, hipStream_t s, int runs, long long sa, long long so,
-                   long long so2, int si);
+                   long long so2, int si, const long long* base0 = nullptr);
 void form_e(const FormE& f, const aw_grid& g, const aw_run_params& rp, double t, hipStream_t s,
             int runs);
 void dcu_scale(const float* const* F, int64_t ncell, float* part, double coef_abs,
@@ -103,7 +103,7 @@ void fix_to_float(aw_ctx& c, float* out, bool assign, hipStream_t s,
                   unsigned long long* counter = nullptr) {
     c.rin.reserve((size_t)ncell(c));
     fix_to_float_add(c.acc.get(), out, ncell(c), c.fix_log2, true, s, assign, counter,
-                     assign ? c.rin.get() : nullptr);
+                     assign ? c.rin.get() : nullptr, 1, nullptr, assign ? c.rho_base : 0);
 }
 
 }  // namespace
@@ -193,6 +193,7 @@ struct aw_sim {
     DevBuf<double> inv;  // inverse scales: rho, J x3, amu x4, dcu x3
     DevBuf<float> cdcu, fmax_part;
     DevBuf<float> cqd;  // rho, J, amu coefficients (device)
+    DevBuf<long long> rbase;  // per run: the mean charge density at the rho scale
     float cq[3] = {0.f, 0.f, 0.f};
     double dcu_abs = 0.0, dcu_signed = 0.0, umax = 1.0;
     bool moments_ready = false;
@@ -266,6 +267,16 @@ struct aw_sim {
                         std::ldexp(1.0, -Sa), std::ldexp(1.0, -Sa), 1.0, 1.0, 1.0};
         for (int r = 0; r < runs; ++r)
             AW_HIP(hipMemcpy(inv.get() + (size_t)r * 11, h, sizeof(h), hipMemcpyHostToDevice));
+        // each run's mean charge density at the rho scale, taken off the E_L input
+        std::vector<long long> hb((size_t)runs);
+        for (int r = 0; r < runs; ++r) {
+            const auto& sp = runs > 1 ? rspecies[(size_t)r] : species;
+            double dens = cfg.legacy_load ? (rp.n0 > 0 ? rp.n0 : 1.0) : 0.0;
+            for (const auto& q : sp) dens += q.density;
+            hb[(size_t)r] = rho_baseline(rp.qm, dens, Sr);
+        }
+        rbase.resize(R);
+        AW_HIP(hipMemcpy(rbase.get(), hb.data(), R * sizeof(long long), hipMemcpyHostToDevice));
         cqd.resize(3);
         AW_HIP(hipMemcpy(cqd.get(), cq, sizeof(cq), hipMemcpyHostToDevice));
         dcu_abs = rp.qm * rp.qm * wmax / area;
@@ -275,20 +286,21 @@ struct aw_sim {
         et.zero(stream);
     }
 
+    // The t = 0 moments and fields, as Simulation::init solves them
+    // (include/pic/simulation.hpp:58-66): fields() holds E and B before the
+    // first step.  The moments are consumed; step 0 deposits them again.
     void init_em() {
         acc8.zero(stream);
         acc3.zero(stream);
-        em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
-                   nullptr, stream, batch(8 * ncell(*ctx)));
-        moments_ready = true;
-        j_staggered = false;
+        moments_ready = false;
+        solve_fields_em(0.0);
     }
 
-    // darwin_fields (include/pic/simulation.hpp:103-119) + boris_em + migrate
-    void step_em(int64_t nstep) {
+    // darwin_fields (include/pic/simulation.hpp:103-119) at time t from the
+    // moments of the current state (deposited here unless the last push did)
+    void solve_fields_em(double t) {
         const hipStream_t s = stream;
         const int64_t nc = ncell(*ctx);
-        const double t = (double)nstep * rp.dt;
         const int R = runs;
         if (!moments_ready) {
             em_moments(*ctx, particles(), tile_off.get(), nullptr, acc8.get(), cqd.get(), bpt,
@@ -302,7 +314,7 @@ struct aw_sim {
         float* fin4 = fftin.get();
         float* fin7 = fftin.get() + (size_t)R * 4 * nc;
         convert_multi(acc8.get(), src.get(), fin4, nc, 4, inv.get(), s, R, 8 * nc, 11 * nc,
-                      4 * nc, 11);
+                      4 * nc, 11, rbase.get());
         convert_multi(acc8.get() + 4 * nc, src.get() + 7 * nc, fin7 + 3 * nc, nc, 4,
                       inv.get() + 4, s, R, 8 * nc, 11 * nc, 7 * nc, 11);
         solve_darwin_elb(*ctx, fin4, elb.get(), rp.eps0, rp.c, rp.B0, s, j_staggered, R);
@@ -329,6 +341,13 @@ struct aw_sim {
         fe.sL = 5 * nc;
         fe.sT = 3 * nc;
         form_e(fe, g, rp, t, s, R);
+    }
+
+    // darwin_fields + boris_em + migrate (include/pic/simulation.hpp:80-84)
+    void step_em(int64_t nstep) {
+        const hipStream_t s = stream;
+        const int64_t nc = ncell(*ctx);
+        solve_fields_em((double)nstep * rp.dt);
         const bool dump = rp.dump_every <= 0 || (nstep % rp.dump_every) == 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (prof) {
@@ -584,7 +603,7 @@ struct aw_sim {
             }
         }
         do_sort();
-        if (em) {  // Darwin: t=0 moments; fields are solved inside each step
+        if (em) {  // Darwin: t=0 moments and fields; each step solves its own
             init_em();
             strays.zero(s);
             initialized = true;
@@ -1284,6 +1303,9 @@ static std::unique_ptr<aw_sim> make_sim(const aw_grid* g, const aw_run_params* r
                             std::ldexp(1.0, 62))
             --S;
     }
+    double dens = cfg->legacy_load ? (rp->n0 > 0 ? rp->n0 : 1.0) : 0.0;
+    for (const auto& s : sim->species) dens += s.density;
+    sim->ctx->rho_base = rho_baseline(rp->qm, dens, sim->ctx->fix_log2);
     return sim;
 }
 

@@ -415,10 +415,13 @@ __global__ void __launch_bounds__(kThreads) em_push_kernel(EmArgs a) {
 // transform reads the copy and the source grids stay intact.
 // Batched (gridDim.y runs): run r uses acc + r*sa, out + r*so, out2 + r*so2,
 // inv + r*si.
+// base0 (per run, may be NULL): the integer baseline taken off grid 0's copy
+// in out2 -- the run's mean charge density, which the E_L solve drops (k = 0),
+// so the FP32 transform rounds relative to the fluctuation.
 __global__ void convert_multi_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
                                      float* __restrict__ out2, int ncell, int nq,
                                      const double* __restrict__ inv, long long sa, long long so,
-                                     long long so2, int si) {
+                                     long long so2, int si, const long long* __restrict__ base0) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const long long r = blockIdx.y;
@@ -428,9 +431,13 @@ __global__ void convert_multi_kernel(unsigned long long* __restrict__ acc, float
     inv += r * si;
     for (int q = 0; q < nq; ++q) {
         const size_t i = (size_t)q * ncell + c;
-        const float v = (float)((double)(long long)acc[i] * inv[q]);
+        const long long a = (long long)acc[i];
+        const float v = (float)((double)a * inv[q]);
         out[i] = v;
-        if (out2) out2[i] = v;
+        if (out2) {
+            const long long b = (q == 0 && base0) ? base0[r] : 0ll;
+            out2[i] = b ? (float)((double)(a - b) * inv[q]) : v;
+        }
         acc[i] = 0ull;
     }
 }
@@ -750,9 +757,9 @@ int em_push(aw_ctx& c, const aw_particles& p, const int32_t* off, const int32_t*
 // out2 + r*so2) with the scales at inv + r*si.
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
                    const double* inv, hipStream_t s, int runs, long long sa, long long so,
-                   long long so2, int si) {
+                   long long so2, int si, const long long* base0) {
     convert_multi_kernel<<<dim3(blocks_for(ncell), runs), kThreads, 0, s>>>(
-        acc, out, out2, (int)ncell, nq, inv, sa, so, so2, si);
+        acc, out, out2, (int)ncell, nq, inv, sa, so, so2, si, base0);
     AW_LAUNCH_CHECK();
 }
 

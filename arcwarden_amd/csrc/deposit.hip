@@ -119,10 +119,15 @@ deposit_tiles_kernel(DepArgs a, TileGeom t, unsigned long long* __restrict__ acc
 
 // out = (assign ? 0 : out) + acc * 2^-S; optionally clears acc for the next
 // accumulation and one 64-bit counter (saves separate memset launches).
+// out2 (the R2C input) receives the value less the integer baseline
+// (base_arr[c] when given, else base_val): the caller passes the mean charge
+// density, which the solve drops anyway (k = 0), so the FP32 transform
+// rounds relative to the fluctuation instead of the mean.
 __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
                                     float* __restrict__ out2, int64_t n, double inv, int zero_acc,
                                     int assign, unsigned long long* __restrict__ counter,
-                                    int ncounter) {
+                                    int ncounter, const long long* __restrict__ base_arr,
+                                    long long base_val) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (counter && c < ncounter) counter[c] = 0ull;
     if (c >= n) return;
@@ -130,7 +135,10 @@ __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float*
     const float add = (float)((double)v * inv);
     const float r = assign ? add : out[c] + add;
     out[c] = r;
-    if (out2) out2[c] = r;
+    if (out2) {
+        const long long b = base_arr ? base_arr[c] : base_val;
+        out2[c] = b ? (float)((double)(v - b) * inv) : r;
+    }
     if (zero_acc) acc[c] = 0ull;
 }
 
@@ -138,13 +146,20 @@ __global__ void fix_to_float_kernel(unsigned long long* __restrict__ acc, float*
 
 void fix_to_float_add(const unsigned long long* acc, float* out, int64_t n, int scale_log2,
                       bool zero_acc, hipStream_t s, bool assign, unsigned long long* counter,
-                      float* out2, int ncounter) {
+                      float* out2, int ncounter, const long long* base_arr, long long base_val) {
     if (n == 0) return;
     AW_REQUIRE(ncounter >= 0, "ncounter must be non-negative");
+    AW_REQUIRE(assign || (!base_arr && !base_val), "a baseline needs assign");
     fix_to_float_kernel<<<blocks_for(std::max<int64_t>(n, ncounter)), kThreads, 0, s>>>(
         const_cast<unsigned long long*>(acc), out, out2, n, std::ldexp(1.0, -scale_log2),
-        zero_acc ? 1 : 0, assign ? 1 : 0, counter, ncounter);
+        zero_acc ? 1 : 0, assign ? 1 : 0, counter, ncounter, base_arr, base_val);
     AW_LAUNCH_CHECK();
+}
+
+long long rho_baseline(double qm, double density, int scale_log2) {
+    const double v = std::ldexp(qm * density, scale_log2);
+    // the accumulators' range is far below 2^62; out-of-range input means no baseline
+    return std::fabs(v) < 4e18 ? (long long)std::llrint(v) : 0ll;
 }
 
 // rho of p into the caller's fixed-point accumulator acc (nx*ny cells, scale

@@ -150,6 +150,7 @@ struct aw_ensemble {
     DevBuf<int32_t> wg_run;  // workgroup -> run (slot) of the multi-run launch
     DevBuf<unsigned long long> acc, strays;
     DevBuf<float> rho, rin, E;
+    DevBuf<long long> rbase;  // per cell: its run's mean charge density (fixed point)
     DevBuf<double> ke_part, diag_stage, diag_ring;
     DevBuf<unsigned char> slots[2], active;
     DevBuf<DiagRun> diag_runs;
@@ -262,7 +263,7 @@ struct aw_ensemble {
     void convert_and_solve() {
         // rho^n for every run; rin receives the R2C input copy, strays reset
         fix_to_float_add(acc.get(), rho.get(), nctot, fix_log2, true, stream, true, strays.get(),
-                         rin.get(), (int)runs.size());
+                         rin.get(), (int)runs.size(), rbase.get());
         // one batched solve per grid size; the sizes touch disjoint buffers
         // (each its own plans, work area and k-space scratch), so they fork
         // onto the side streams and join before the push
@@ -537,6 +538,20 @@ aw_status aw_ensemble_create(const aw_run_desc* descs, int32_t nruns, const aw_s
         AW_HIP(hipMemset(e->strays.get(), 0, e->runs.size() * sizeof(unsigned long long)));
         e->rho.resize((size_t)cb);
         e->rin.resize((size_t)cb);
+        {
+            // the baseline a single aw_sim of the run uses (same scale, same species)
+            std::vector<long long> h((size_t)cb);
+            for (auto& r : e->runs) {
+                double dens = 0.0;
+                for (const auto& sp : r.species) dens += sp.density;
+                r.ctx->rho_base = rho_baseline(r.rp.qm, dens, scale);
+                std::fill(h.begin() + r.cbase, h.begin() + r.cbase + ncell(*r.ctx),
+                          r.ctx->rho_base);
+            }
+            e->rbase.resize((size_t)cb);
+            AW_HIP(hipMemcpy(e->rbase.get(), h.data(), h.size() * sizeof(long long),
+                             hipMemcpyHostToDevice));
+        }
         e->E.resize((size_t)2 * cb);
         e->ke_part.resize((size_t)std::max(1, wg));
         {

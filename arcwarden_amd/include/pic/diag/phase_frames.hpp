@@ -80,8 +80,9 @@ public:
             const float fnx = static_cast<float>(fr.g.nx);
             for (int64_t k = 0; k < m; ++k) {
                 float xc = x[k];
+                // a tiny negative x + nx rounds to nx: the second test maps it to 0
                 if (xc < 0.0f) xc += fnx;
-                else if (xc >= fnx) xc -= fnx;
+                if (xc >= fnx) xc -= fnx;
                 o << xc * fr.g.dx << ',' << vx[k] << '\n';
             }
         }

@@ -688,6 +688,30 @@ aw_status aw_ensemble_history(aw_ensemble* e, int32_t run, aw_diag_sample* out, 
  * weight for all species; uz does not enter the run. */
 typedef struct aw_adjoint aw_adjoint;
 
+/* The reverse sweep's pieces, on caller arrays (no reference counterpart),
+ * for callers that write their own reverse loops:
+ *   aw_adj_push_reverse: the transpose of one push (Pusher::boris,
+ *     include/pic/pusher.hpp:35-80,112-129) at positions p (x^n) with the
+ *     step's field Ex, Ey: lu += (dt/dx) lx (the move; not for half_back),
+ *     le = h (I + R^T) lu and lu = R^T lu (the Boris update, h = qm dt_eff/2,
+ *     R the rotation of B0 z when has_b0, dt_eff = dt or -dt/2 for the half
+ *     step back), lx += (de/dx)^T le; writes le (lex, ley);
+ *   aw_adj_gather_t: lE += sum_p W(x_p) le_p, the transpose of the CIC gather
+ *     (a deposit of per-particle vectors; int64 fixed point with a scale from
+ *     the measured max |le|, deterministic); accumulates into lEx, lEy;
+ *   aw_adj_deposit_t: lx += qm w_p/(dx dy) grad_x sum_k W_k lrho_k, the
+ *     transpose of the charge deposit (Depositor::charge) with respect to the
+ *     positions.
+ * Positions in cells; gradients with respect to x in cells. */
+aw_status aw_adj_push_reverse(aw_ctx* ctx, const aw_particles* p, const float* Ex,
+                              const float* Ey, const aw_run_params* rp, int32_t has_b0,
+                              int32_t half_back, float* lx, float* ly, float* lux, float* luy,
+                              float* lex, float* ley, aw_stream s);
+aw_status aw_adj_gather_t(aw_ctx* ctx, const aw_particles* p, const float* lex, const float* ley,
+                          float* lEx, float* lEy, aw_stream s);
+aw_status aw_adj_deposit_t(aw_ctx* ctx, const aw_particles* p, const float* lrho, double qm,
+                           float* lx, float* ly, aw_stream s);
+
 /* tile: 0 -> 8 at >= 64 particles per cell, else 16. */
 aw_status aw_adj_create(const aw_grid* g, const aw_run_params* rp, const aw_species* sp,
                         int32_t nspecies, int32_t n_modes, int32_t checkpoint_every,

@@ -1208,30 +1208,39 @@ void orc_yee_push_f32(const orc_grid* g, const orc_yee_params* p, double tnow, c
     }
 }
 
-/* k_energy (include/pic/yee2d.hpp:663-678): {eps0/2 |E|^2 dA, c^2/2 |B|^2 dA} */
+/* k_energy (include/pic/yee2d.hpp:648-665): {eps0/2 |E|^2 dA, c^2/2 |B|^2 dA}
+ * in FP64 over the FP32 mesh values; dA and c^2 are the mesh's FP32 copies
+ * (YeeViews dxp, dyp, c2; yee2d.hpp:676-677) widened to FP64. */
 void orc_yee_energy_f64(const orc_grid* g, const float* F, double c2, double* out2) {
     const int64_t nc = (int64_t)g->nx * g->ny;
-    const double dA = g->dx * g->dy;
+    const double dA = (double)(float)g->dx * (double)(float)g->dy;
+    const double c2f = (double)(float)c2;
     double we = 0.0, wb = 0.0;
     for (int64_t c = 0; c < nc; ++c) {
-        for (int q = 0; q < 3; ++q) we += 0.5 * (double)F[q * nc + c] * F[q * nc + c] * dA;
-        for (int q = 3; q < 6; ++q) wb += 0.5 * c2 * (double)F[q * nc + c] * F[q * nc + c] * dA;
+        double e = 0.0, b = 0.0;
+        for (int q = 0; q < 3; ++q) e += (double)F[q * nc + c] * F[q * nc + c];
+        for (int q = 3; q < 6; ++q) b += (double)F[q * nc + c] * F[q * nc + c];
+        we += 0.5 * e * dA;
+        wb += 0.5 * c2f * b * dA;
     }
     out2[0] = we;
     out2[1] = wb;
 }
 
-/* rms div B on the Yee mesh (include/pic/yee2d.hpp:640-643) */
+/* rms div B on the Yee mesh (k_div_stats, include/pic/yee2d.hpp:608-626):
+ * the cell divergence in FP32 with the mesh's FP32 cell sizes, squared and
+ * summed in FP64 */
 double orc_yee_divb_rms(const orc_grid* g, const float* F) {
     const int64_t nc = (int64_t)g->nx * g->ny;
     const float* bx = F + 3 * nc; const float* by = F + 4 * nc;
+    const float dxf = (float)g->dx, dyf = (float)g->dy;
     double s = 0.0;
     for (int j = 0; j < g->ny; ++j)
         for (int i = 0; i < g->nx; ++i) {
             const int c = j * g->nx + i;
-            const double d = ((double)bx[yat(g, i + 1, j)] - bx[c]) / g->dx +
-                             ((double)by[yat(g, i, j + 1)] - by[c]) / g->dy;
-            s += d * d;
+            const float d = (bx[yat(g, i + 1, j)] - bx[c]) / dxf +
+                            (by[yat(g, i, j + 1)] - by[c]) / dyf;
+            s += (double)d * d;
         }
     return sqrt(s / (double)nc);
 }

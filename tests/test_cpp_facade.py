@@ -9,6 +9,7 @@ import csv
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -149,17 +150,19 @@ def test_arcsim_every_diagnostic(gpu, tmp_path, model):
     assert [m["file"] for m in man] == [f"frame_{k:04d}.csv" for k in range(5)]
     frame = list(csv.DictReader(open(out / man[-1]["file"])))
     assert len(frame) == min(n, 100000) and list(frame[0]) == ["x", "vx"]
-    assert all(0.0 <= float(f["x"]) < nx * 0.5 for f in frame)
+    # written with the stream's default 6 significant digits, as the reference
+    # writes them: x just below Lx prints as Lx
+    assert all(0.0 <= float(f["x"]) <= nx * 0.5 for f in frame)
     # k-t dump: rows of nx floats every kt_stride = 4 steps, steps 0..40
     km = _meta(out / "run_kt.meta")
-    assert int(km["nx"]) == nx and int(km["nsamp"]) == 11
+    assert int(km["nx"]) == nx
     assert float(km["dt_sample"]) == pytest.approx(4 * dt)
     assert int(km["M"]) == 3 and int(km["band_lo"]) == 2 and int(km["band_hi"]) == 6
     kt = np.fromfile(out / "run_kt.bin", np.float32)
     if model == "electrostatic":  # no E_L grid on the electrostatic path
         assert kt.size == 0 and int(km["nsamp"]) == 0
     else:
-        assert kt.size == 11 * nx and np.isfinite(kt).all()
+        assert kt.size == 11 * nx and np.isfinite(kt).all() and int(km["nsamp"]) == 11
     # phase video: n_frames = 5 -> every 8 steps, 6 frames
     vm = _meta(out / "run_vid.meta")
     assert (int(vm["nframe"]), int(vm["nxb"]), int(vm["nvb"])) == (6, 256, 200)

@@ -91,8 +91,15 @@ def test_config5_whistler_step_matches_fp64_oracle(gpu, oracle):
     e64 = (el64[0] + et64[0] + p[0], el64[1] + et64[1] + p[1], et64[2] + p[2])
     assert rel_l2(f["ELx"], el64[0]) < 1e-5 and rel_l2(f["ELy"], el64[1]) < 1e-5
     for q, name in enumerate(("Bx", "By", "Bz")):
-        # the self field alone (B0 would hide its error)
-        assert rel_l2(f[name] - np.float32(rp.B0[q]), bself[q]) < 1e-5, name
+        # the self field alone (B0 would hide its error).  The engine holds
+        # the total B0 + B_self in FP32, which resolves the self field to
+        # about one ulp of |B0| (3e-8 at 0.43, 1.4e-3 of this run's 2e-5
+        # self field): the bound is 1e-5 of the self field plus two ulps
+        err = f[name].astype(np.float64) - (rp.B0[q] + bself[q])
+        ulp = float(np.spacing(np.float32(abs(rp.B0[q]) + np.abs(bself[q]).max())))
+        rms = lambda a: float(np.sqrt(np.mean(np.square(a))))  # noqa: E731
+        print(name, "rms error", rms(err), "self field rms", rms(bself[q]), "ulp", ulp)
+        assert rms(err) < 1e-5 * rms(bself[q]) + 2 * ulp, name
     for name, ref in zip(("ETx", "ETy", "ETz"), et64):
         assert rel_l2(f[name], ref) < 1e-4, name
     for name, ref in zip(("Ex", "Ey", "Ez"), e64):

@@ -115,7 +115,10 @@ def test_yee_step_matches_oracle(gpu, oracle, case):
     ref, x0, y0, vz, wdep = oracle.yee_push(
         og, yp, t0, F1, st, w=w, wd=wd0, wc=np.zeros_like(wd0) if wd0 is not None else None,
         wdd=wd0.astype(np.float64) if wd0 is not None else None)
-    exact = not rp.pump  # the pump's double cos/sin may differ in the last FP32 place
+    # the pump's double cos/sin, the damping layers' expf and the loss-cone
+    # expf are the device's libm, which may differ from the host's in the
+    # last place: those cases compare within FP32 rounding, the rest bit for bit
+    exact = not (rp.pump or rp.bnd_x == 2 or (rp.deltaf and rp.df_dist == 1))
     for k in ("ux", "uy", "uz"):
         if exact:
             assert np.array_equal(got[k], ref[k]), (case, k)
@@ -127,7 +130,10 @@ def test_yee_step_matches_oracle(gpu, oracle, case):
     else:
         assert rel_l2(got["x"], rx) < 1e-7 and rel_l2(got["y"], ry) < 1e-7, case
     if rp.deltaf:
-        assert np.array_equal(got["wd"], ref["wd"]), case
+        if exact:
+            assert np.array_equal(got["wd"], ref["wd"]), case
+        else:
+            assert rel_l2(got["wd"], ref["wd"]) < 1e-6, case
     # the current of the move, then smoothing, against the FP64 restatement
     wdep_or_w = wdep if rp.deltaf else w
     # Jz takes uz / gamma after the kick, before the walls damp u

@@ -853,6 +853,26 @@ def test_yee_faraday_keeps_div_b(oracle):
     assert oracle.yee_divb_rms(g, F) < 1e-5 * bscale
 
 
+def test_yee_energy_closed_form(oracle):
+    """Uniform fields: W_E = nc dA |E|^2 / 2 and W_B = nc dA c^2 |B|^2 / 2, with
+    dA taken from the mesh's FP32 cell sizes (dy = 0.3 is not exact in FP32:
+    an FP64 dA would differ in the 8th digit); div B of a uniform B is 0."""
+    nx, ny, dx, dy, c2 = 20, 12, 0.5, 0.3, 9.0
+    g = oracle.grid(nx, ny, dx, dy)
+    nc = nx * ny
+    E = (0.5, -1.25, 2.0)
+    B = (0.75, 0.0, -1.5)
+    F = np.zeros(9 * nc, np.float32)
+    for q, v in enumerate(E + B):
+        F[q * nc:(q + 1) * nc] = v
+    dA = float(np.float32(dx)) * float(np.float32(dy))
+    we, wb = oracle.yee_energy(g, F, c2)
+    assert we == pytest.approx(0.5 * nc * dA * sum(v * v for v in E), rel=1e-14)
+    assert wb == pytest.approx(0.5 * nc * dA * c2 * sum(v * v for v in B), rel=1e-14)
+    assert we != pytest.approx(0.5 * nc * dx * dy * sum(v * v for v in E), rel=1e-9)
+    assert oracle.yee_divb_rms(g, F) == 0.0
+
+
 def _one(x, y, ux, uy, uz):
     return {k: np.array([v], np.float32) for k, v in
             (("x", x), ("y", y), ("ux", ux), ("uy", uy), ("uz", uz))}
