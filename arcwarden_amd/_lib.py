@@ -176,6 +176,8 @@ _SIGS = {
                                     PP(C.c_double), P]),
     "aw_load_legacy": (C.c_int32, [P, PP(AwRunParams), PP(AwParticles), P]),
     "aw_deposit_rho": (C.c_int32, [P, PP(AwParticles), C.c_double, C.c_int32, P, P]),
+    "aw_deposit_rho_noalloc": (C.c_int32, [PP(AwGrid), PP(AwParticles), C.c_double, C.c_int32,
+                                           C.c_int32, P, P]),
     "aw_deposit_esirkepov": (C.c_int32, [P, P, P, PP(AwParticles), C.c_double, C.c_double,
                                          C.c_int32, P, P, P, P]),
     "aw_push_es": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32, P]),

@@ -276,6 +276,19 @@ aw_status aw_load_legacy(aw_ctx* ctx, const aw_run_params* rp, aw_particles* p,
 aw_status aw_deposit_rho(aw_ctx* ctx, const aw_particles* p, double qm, int32_t policy,
                          float* rho, aw_stream s);
 
+/* No counterpart in the reference.  The charge deposit of aw_deposit_rho for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): it needs no context
+ * and allocates nothing, accumulating into a static per-GPU device array of
+ * at most 2^20 cells (8 MiB; larger grids are an AW_ERR_INVALID), then adding
+ * the FP32 result into rho like aw_deposit_rho.  The same integer sums as
+ * aw_deposit_rho at fixed-point scale 2^scale_log2 give the same bits.
+ * policy: AW_DEPOSIT_GLOBAL, AW_DEPOSIT_LDS_GRID or AW_DEPOSIT_AUTO.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (several streams or host threads) race; calls on one stream are ordered. */
+aw_status aw_deposit_rho_noalloc(const aw_grid* g, const aw_particles* p, double qm,
+                                 int32_t policy, int32_t scale_log2, float* rho, aw_stream s);
+
 /* Esirkepov charge-conserving current, the deposit half of k_push_esirkepov /
  * k_push_esirkepov_tiled with esirkepov_scatter + GlobalJSink
  * (include/pic/yee2d.hpp:124-132,310-389,401-469): for the moves
