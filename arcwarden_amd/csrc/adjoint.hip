@@ -66,6 +66,26 @@ struct KG {
 
 // S^T in k-space: lrho_k = i G (kx lEx_k + ky lEy_k), G = 1/(eps0 k^2);
 // DC and Nyquist zero (the transpose of es_kspace_kernel).
+// Order-free maxima (FP32 bit patterns of non-negative values, unsigned order
+// = value order) are kept in kBoundSlots slots, one 128-byte line each: a
+// wave takes the max with atomicMax on its block's slot, and readers take the
+// max over the slots.  One shared address would serialise the thousands of
+// wave atomics of a 1 M-particle launch in L2 (about 0.2 ms).
+constexpr int kBoundSlots = 64, kSlotStride = 32;
+constexpr int kBoundWords = kBoundSlots * kSlotStride;
+
+__device__ __forceinline__ void wave_max_to(float v, unsigned int* slots) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.f)
+        atomicMax(slots + (blockIdx.x % kBoundSlots) * kSlotStride, __float_as_uint(v));
+}
+
+__device__ __forceinline__ float slots_max(const unsigned int* slots) {
+    unsigned int m = 0u;
+    for (int k = 0; k < kBoundSlots; ++k) m = max(m, slots[k * kSlotStride]);
+    return __uint_as_float(m);
+}
+
 __global__ void es_adj_kspace_kernel(const float2* __restrict__ ak, float2* __restrict__ rk,
                                      KG k, unsigned int* __restrict__ rmax) {
     const int64_t nk = (int64_t)k.nkx * k.ny;
@@ -85,18 +105,16 @@ __global__ void es_adj_kspace_kernel(const float2* __restrict__ ak, float2* __re
         r.y = (float)(g * (kx * (double)ax.x + ky * (double)ay.x));
     }
     if (live) rk[t] = r;
-    float mx = live ? fabsf(r.x) + fabsf(r.y) : 0.f;
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(rmax, __float_as_uint(mx));
+    wave_max_to(live ? fabsf(r.x) + fabsf(r.y) : 0.f, rmax);
 }
 
-// Order-free bounds of one reverse step (FP32 bit patterns of non-negative
-// maxima, unsigned order = value order): smax[2] alternate between steps (the
-// state entering step st is bounded by smax[(st+1)&1], the step's outputs go
-// to smax[st&1]); rmax bounds the lrho spectrum; e is the step's exponent.
+// Order-free bounds of one reverse step (slotted maxima, as above): smax[2]
+// alternate between steps (the state entering step st is bounded by
+// smax[(st+1)&1], the step's outputs go to smax[st&1]); rmax bounds the lrho
+// spectrum; e is the step's exponent.
 struct AdjBound {
-    unsigned int smax[2];
-    unsigned int rmax;
+    unsigned int smax[2][kBoundWords];
+    unsigned int rmax[kBoundWords];
     int e;
 };
 
@@ -106,9 +124,9 @@ struct AdjBound {
 __global__ void lE_convert_kernel(unsigned long long* __restrict__ acc, float* __restrict__ out,
                                   int64_t n, AdjBound* __restrict__ bd, int free_slot) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0) {
-        bd->smax[free_slot] = 0u;
-        bd->rmax = 0u;
+    if (c < kBoundSlots) {  // the grid has at least one block of kThreads >= kBoundSlots
+        bd->smax[free_slot][c * kSlotStride] = 0u;
+        bd->rmax[c * kSlotStride] = 0u;
     }
     if (c >= n) return;
     const double inv = ldexp(1.0, -bd->e);
@@ -217,10 +235,6 @@ __device__ __forceinline__ float max4abs(float a, float b, float c, float d) {
 }
 
 // block-wide max of non-negative v, one atomicMax per wavefront
-__device__ __forceinline__ void wave_max_to(float v, unsigned int* dst) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
-}
 
 // Exponent e of the step's fixed-point scale: every contribution le * W is
 // bounded by B = |a| (U + c (U + |coef| 2 R)), U the entry-state bound and R
@@ -228,8 +242,8 @@ __device__ __forceinline__ void wave_max_to(float v, unsigned int* dst) {
 // 2^log2n * B * 2^e < 2^62 and B * 2^e < 2^50.  The FP32 rounding of the
 // products stays inside the 1e-3 margin.
 __device__ __forceinline__ int adj_exponent(const AdjArgs& a) {
-    const double U = (double)__uint_as_float(a.bd->smax[a.slot_in]);
-    const double R = a.xb ? (double)__uint_as_float(a.bd->rmax) * (double)a.rfac : 0.0;
+    const double U = (double)slots_max(a.bd->smax[a.slot_in]);
+    const double R = a.xb ? (double)slots_max(a.bd->rmax) * (double)a.rfac : 0.0;
     const double c = fmax(fabs((double)a.cx), fabs((double)a.cy));
     // |le| <= (|h| + |h0|) (1 + sqrt 2) sqrt 2 |lu'| (rotations keep the 2-norm)
     const double B = 4.0 * (fabs((double)a.h) + fabs((double)a.h0)) *
@@ -337,7 +351,7 @@ __global__ void __launch_bounds__(kThreads) adj_step_kernel(AdjArgs a) {
             }
         }
     }
-    wave_max_to(smax, &a.bd->smax[a.slot_in ^ 1]);
+    wave_max_to(smax, a.bd->smax[a.slot_in ^ 1]);
     __syncthreads();
     for (int c = threadIdx.x; c < R; c += kThreads) {
         const int gc = dev::wrapfar(orgy + c / W, a.ny) * a.nx + dev::wrapfar(orgx + c % W, a.nx);
@@ -481,7 +495,7 @@ le_bound_kernel(const float* __restrict__ a, const float* __restrict__ b, int n,
 // lE += sum_p W(x_p) le_p in int64 fixed point: scale 2^e with
 // n * max|le| * 2^e < 2^62 (order-free, deterministic)
 __device__ __forceinline__ int gather_t_exponent(const unsigned int* bound, int log2n) {
-    const double B = (double)__uint_as_float(*bound) * (1.0 + 1e-6);
+    const double B = (double)slots_max(bound) * (1.0 + 1e-6);
     if (!(B > 0.0) || !(B < 1e300)) return 0;
     int k;
     (void)frexp(B, &k);
@@ -789,7 +803,7 @@ struct aw_adjoint {
         fft_exec(*ctx, ctx->r2c2, lEf.get(), ak, s);  // [lEx | lEy] -> [ak_x | ak_y]
         KG k{g.nx, g.ny, g.nx / 2 + 1, 6.28318530717958647692 / g.Lx,
              6.28318530717958647692 / g.Ly, rp.eps0};
-        es_adj_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(ak, rk, k, &bd.get()->rmax);
+        es_adj_kspace_kernel<<<blocks_for(nk), kThreads, 0, s>>>(ak, rk, k, bd.get()->rmax);
         AW_LAUNCH_CHECK();
         fft_exec(*ctx, ctx->c2r, rk, lrho.get(), s);
     }
@@ -851,7 +865,7 @@ struct aw_adjoint {
                 const float* xs = segx.get() + (size_t)(s1 - s0) * 2 * n;
                 adj_drho_kernel<<<blocks_for(n), kThreads, 0, s>>>(
                     xs, xs + n, lrho.get(), lx.get(), ly.get(), (int)n, g.nx, g.ny, coef,
-                    &bd.get()->smax[s1 & 1]);
+                    bd.get()->smax[s1 & 1]);
                 AW_LAUNCH_CHECK();
                 pending_drho = false;
             }
@@ -905,7 +919,7 @@ struct aw_adjoint {
             const float* x0 = segx.get();
             adj_drho_kernel<<<blocks_for(n), kThreads, 0, s>>>(x0, x0 + n, lrho.get(), lx.get(),
                                                               ly.get(), (int)n, g.nx, g.ny, coef,
-                                                              &bd.get()->smax[s0 & 1]);
+                                                              bd.get()->smax[s0 & 1]);
             AW_LAUNCH_CHECK();
             pending_drho = false;
             // undo the sort that produced this segment's order
@@ -985,9 +999,9 @@ aw_status aw_adj_gather_t(aw_ctx* ctx, const aw_particles* p, const float* lex, 
         const int64_t nc = ncell(*ctx);
         const hipStream_t st = S(s);
         ctx->jacc.reserve((size_t)(3 * nc));
-        ctx->jbound.reserve(2);
+        ctx->jbound.reserve(kBoundWords);
         AW_HIP(hipMemsetAsync(ctx->jacc.get(), 0, (size_t)(2 * nc) * sizeof(unsigned long long), st));
-        AW_HIP(hipMemsetAsync(ctx->jbound.get(), 0, sizeof(unsigned int), st));
+        AW_HIP(hipMemsetAsync(ctx->jbound.get(), 0, kBoundWords * sizeof(unsigned int), st));
         unsigned int* bound = ctx->jbound.get();
         int log2n = 0;
         while (log2n < 62 && ((int64_t)1 << log2n) < p->n) ++log2n;
