@@ -178,6 +178,8 @@ _SIGS = {
     "aw_deposit_rho": (C.c_int32, [P, PP(AwParticles), C.c_double, C.c_int32, P, P]),
     "aw_deposit_rho_noalloc": (C.c_int32, [PP(AwGrid), PP(AwParticles), C.c_double, C.c_int32,
                                            C.c_int32, P, P]),
+    "aw_rho_fixed_scale": (C.c_int32, [PP(AwGrid), C.c_double, C.c_double, C.c_double,
+                                       C.c_int64, PP(C.c_int32)]),
     "aw_deposit_esirkepov": (C.c_int32, [P, P, P, PP(AwParticles), C.c_double, C.c_double,
                                          C.c_int32, P, P, P, P]),
     "aw_push_es": (C.c_int32, [P, PP(AwParticles), P, P, PP(AwRunParams), C.c_int32, P]),

@@ -306,6 +306,7 @@ void push_deposit_multi(const void* slots, const int32_t* wg_run, int total_wg, 
 // Contexts (capi.hip)
 aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y);
 int fixed_scale_for(const std::vector<aw_species>& sp, const aw_run_params& rp, bool legacy);
+int rho_scale_guarded(const aw_grid& g, double qm, double wmax, int64_t n, int S);
 // Added to a step's stray count when a Darwin moment left the fixed-point
 // range its static scale assumes (the count itself stays below 2^40).
 constexpr unsigned long long kRangeFlag = 1ull << 40;

@@ -827,6 +827,22 @@ aw_ctx* make_ctx(const aw_grid* g, int tile_x, int tile_y) {
     return c.release();
 }
 
+// Lowers S until no int64 cell sum of rho can wrap: even every particle's
+// largest contribution, |qm| wmax / (dx dy) 2^S, summed in one cell stays
+// below 2^62.  A configuration that needs S < 8 is an error, not a wrap.
+int rho_scale_guarded(const aw_grid& g, double qm, double wmax, int64_t n, int S) {
+    const double cmax = std::fabs(qm) * wmax / (g.dx * g.dy);
+    AW_REQUIRE(std::isfinite(cmax), "particle weight or charge is not finite");
+    const double total = (double)std::max<int64_t>(1, n) * cmax;
+    while (S > 8 && total * std::ldexp(1.0, S) >= std::ldexp(1.0, 62)) --S;
+    if (total * std::ldexp(1.0, S) >= std::ldexp(1.0, 62))
+        fail(AW_ERR_INVALID,
+             "rho fixed point: n |qm| w / (dx dy) = " + std::to_string(total) +
+                 " leaves no scale >= 2^8 whose int64 cell sums cannot wrap",
+             __FILE__, __LINE__);
+    return S;
+}
+
 int fixed_scale_for(const std::vector<aw_species>& sp, const aw_run_params& rp, bool legacy) {
     // rho magnitude ~ |qm| * total density; keep 2^23 headroom over it
     double dens = 0.0;
@@ -848,6 +864,22 @@ void aw_last_error(char* buf, size_t len) {
 }
 
 const char* aw_version(void) { return "arcwarden-amd 0.1 gfx950"; }
+
+aw_status aw_rho_fixed_scale(const aw_grid* g, double qm, double density, double wmax,
+                             int64_t n, int32_t* scale_log2) {
+    return guard("aw_rho_fixed_scale", [&] {
+        check_grid(g);
+        AW_REQUIRE(scale_log2, "scale_log2 is NULL");
+        AW_REQUIRE(n >= 0 && wmax >= 0.0 && density >= 0.0, "n, wmax and density must be >= 0");
+        aw_species sp{};
+        sp.density = density;
+        sp.ppc = 1;
+        aw_run_params rp{};
+        rp.qm = qm;
+        const int S = fixed_scale_for(std::vector<aw_species>{sp}, rp, false);
+        *scale_log2 = rho_scale_guarded(*g, qm, wmax, n, S);
+    });
+}
 
 aw_status aw_device_alloc(size_t bytes, void** out) {
     return guard("aw_device_alloc", [&] {
@@ -1292,16 +1324,10 @@ static std::unique_ptr<aw_sim> make_sim(const aw_grid* g, const aw_run_params* r
                              ? cfg->fixed_scale
                              : fixed_scale_for(sim->species, *rp, cfg->legacy_load != 0);
     if (cfg->fixed_scale <= 0) {
-        // No int64 cell sum of rho can wrap: even every particle's largest
-        // contribution, |qm| w / (dx dy) 2^S, summed in one cell stays below 2^62.
         double wmax = cfg->legacy_load ? std::fabs(rp->weight) : 0.0;
         for (const auto& s : sim->species)
             wmax = std::max(wmax, std::fabs(s.density) * g->dx * g->dy / s.ppc);
-        const double cmax = std::fabs(rp->qm) * wmax / (g->dx * g->dy);
-        int& S = sim->ctx->fix_log2;
-        while (S > 8 && (double)std::max<int64_t>(1, sim->n) * cmax * std::ldexp(1.0, S) >=
-                            std::ldexp(1.0, 62))
-            --S;
+        sim->ctx->fix_log2 = rho_scale_guarded(*g, rp->qm, wmax, sim->n, sim->ctx->fix_log2);
     }
     double dens = cfg->legacy_load ? (rp->n0 > 0 ? rp->n0 : 1.0) : 0.0;
     for (const auto& s : sim->species) dens += s.density;

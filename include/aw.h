@@ -276,6 +276,16 @@ aw_status aw_load_legacy(aw_ctx* ctx, const aw_run_params* rp, aw_particles* p,
 aw_status aw_deposit_rho(aw_ctx* ctx, const aw_particles* p, double qm, int32_t policy,
                          float* rho, aw_stream s);
 
+/* No counterpart in the reference (its deposits add FP32 with atomics).  The
+ * int64 fixed-point scale 2^S the engine uses for rho: S puts |qm| density at
+ * 2^40 (clamped to [8, 60]), then is lowered until n particles of weight
+ * wmax summed into one cell stay below 2^62, so no cell sum can wrap; a
+ * configuration that would need S < 8 is AW_ERR_INVALID.  Host only.  For
+ * callers that choose the scale of aw_deposit_rho_noalloc or
+ * aw_ctx_set_fixed_scale. */
+aw_status aw_rho_fixed_scale(const aw_grid* g, double qm, double density, double wmax,
+                             int64_t n, int32_t* scale_log2);
+
 /* No counterpart in the reference.  The charge deposit of aw_deposit_rho for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc): it needs no context
