@@ -1000,8 +1000,14 @@ aw_status aw_adj_gather_t(aw_ctx* ctx, const aw_particles* p, const float* lex, 
         const hipStream_t st = S(s);
         ctx->jacc.reserve((size_t)(3 * nc));
         ctx->jbound.reserve(kBoundWords);
-        AW_HIP(hipMemsetAsync(ctx->jacc.get(), 0, (size_t)(2 * nc) * sizeof(unsigned long long), st));
-        AW_HIP(hipMemsetAsync(ctx->jbound.get(), 0, kBoundWords * sizeof(unsigned int), st));
+        // cleared by kernels, not hipMemsetAsync, so the call can be graph-captured
+        const int64_t nw = 4 * nc;  // the 2 nc int64 accumulators as 32-bit words
+        zero_words_kernel<<<(int)((nw + kThreads - 1) / kThreads), kThreads, 0, st>>>(
+            reinterpret_cast<uint32_t*>(ctx->jacc.get()), nw);
+        AW_LAUNCH_CHECK();
+        zero_words_kernel<<<(kBoundWords + kThreads - 1) / kThreads, kThreads, 0, st>>>(
+            ctx->jbound.get(), kBoundWords);
+        AW_LAUNCH_CHECK();
         unsigned int* bound = ctx->jbound.get();
         int log2n = 0;
         while (log2n < 62 && ((int64_t)1 << log2n) < p->n) ++log2n;
