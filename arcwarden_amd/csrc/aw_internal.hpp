@@ -227,6 +227,7 @@ struct aw_ctx {
     aw::DevBuf<double> red;                  // reduction partials
     aw::DevBuf<unsigned long long> jacc;     // Esirkepov Jx|Jy|Jz fixed-point accumulators
     aw::DevBuf<unsigned int> jbound;         // Esirkepov scale bounds (max|w|, max|w uz|)
+    aw::DevBuf<float> jpart;                 // their per-workgroup partial maxima
     aw::SortScratch sort;
     aw::EmScratch em;
     ~aw_ctx();
@@ -255,6 +256,22 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
 // in tile order, off[ntiles + 1] (one-pass counting sort when the per-tile
 // counters fit LDS, else the radix sort of tile keys).
 void bin_tiles(aw_ctx& c, const aw_particles& p, int32_t* perm, int32_t* off, hipStream_t s);
+
+// Esirkepov current (esirkepov.hip).  Optional inputs: the per-workgroup
+// maxima pairs (max |w|, max |w uz|, or max |uz| for a uniform weight) of the
+// deposit's particles, already formed by the caller's push; the tile offsets
+// of particles the caller keeps in tile order (ntiles + 1, from an earlier
+// stable tile sort of the same n particles); outputs for the wrapped new
+// positions (may alias p.x, p.y).
+struct EskExtra {
+    const float* bound_part = nullptr;
+    int npart = 0;
+    const int32_t* slice_off = nullptr;
+    float *xw = nullptr, *yw = nullptr;
+};
+void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
+                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
+                       hipStream_t s, const EskExtra& ex = EskExtra{});
 
 // Argument checks of the C ABI (capi.hip)
 void check_grid(const aw_grid* g);
@@ -331,6 +348,10 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 }
 
 __device__ __forceinline__ int wrapfar(int i, int n) {
+    // within one period of the box: no integer division
+    if (i >= 0 && i < n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i - n < n) return i - n;
     i %= n;
     if (i < 0) i += n;
     return i;
@@ -341,6 +362,28 @@ __device__ __forceinline__ int wrapfar(int i, int n) {
 __device__ __forceinline__ void wrap_pair(int i, int n, int& a, int& b) {
     a = wrapfar(i, n);
     b = (a + 1 == n) ? 0 : a + 1;
+}
+
+// Max over the workgroup of (a, b), for workgroups of up to 1024 threads in
+// which every thread calls it; thread 0 receives the result.
+__device__ __forceinline__ void block_max2(float& a, float& b) {
+    __shared__ float sa[16], sb[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, off, 64));
+        b = fmaxf(b, __shfl_xor(b, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sa[wave] = a;
+        sb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < nw; ++k) {
+            a = fmaxf(a, sa[k]);
+            b = fmaxf(b, sb[k]);
+        }
+    }
 }
 
 // Tile-local LDS coordinate test: l in [0, lim] as an unsigned compare, so a
@@ -542,15 +585,25 @@ __device__ __forceinline__ unsigned long long fixq(double coef, double w, double
 // For a move (X0, Y0) -> (X1, Y1) of less than one cell (cell units; X1, Y1
 // taken as the minimal image of the stored, possibly wrapped, new position)
 // the 4x4 union stencil {ib-1..ib+2} x {jb-1..jb+2}, ib = floor(min(X0, X1)),
-// carries the CIC node shapes S0 = S(X0) and the differences DS = S(X1) - S(X0).
+// carries the CIC node shapes S0 = S(X0), S1 = S(X1), S the node shape
+// max(0, 1 - |x - node|).  Both end points lie at or beyond node ib (jb), so
+// every shape at node ib - 1 (jb - 1) is zero; nodes ib..ib+2 keep the mean
+// shape M = (S0 + S1)/2 and the difference D = S1 - S0.  Every one of these
+// is exact in FP64: differences of float positions and small integers.
 struct EskShape {
     int ib, jb;
-    double Sx0[4], Sy0[4], DSx[4], DSy[4];
+    double mx[3], my[3], dx[3], dy[3];  // nodes ib..ib+2 (jb..jb+2)
 };
 
-__device__ __forceinline__ void esk_shape4(double x, int base, double S[4]) {
+__device__ __forceinline__ void esk_axis(double a, double b, int base, double m[3], double d[3]) {
+    const double fa = a - (double)base, fb = b - (double)base;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) S[k] = fmax(0.0, 1.0 - fabs(x - (double)(base - 1 + k)));
+    for (int k = 0; k < 3; ++k) {
+        const double sa = fmax(0.0, 1.0 - fabs(fa - (double)k));
+        const double sb = fmax(0.0, 1.0 - fabs(fb - (double)k));
+        m[k] = 0.5 * (sa + sb);
+        d[k] = sb - sa;
+    }
 }
 
 __device__ __forceinline__ void esk_shape(double X0, double Y0, double X1, double Y1, int nx,
@@ -561,73 +614,56 @@ __device__ __forceinline__ void esk_shape(double X0, double Y0, double X1, doubl
     else if (Y0 - Y1 > 0.5 * ny) Y1 += ny;
     m.ib = (int)floor(fmin(X0, X1));
     m.jb = (int)floor(fmin(Y0, Y1));
-    double Sx1[4], Sy1[4];
-    esk_shape4(X0, m.ib, m.Sx0);
-    esk_shape4(X1, m.ib, Sx1);
-    esk_shape4(Y0, m.jb, m.Sy0);
-    esk_shape4(Y1, m.jb, Sy1);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        m.DSx[k] = Sx1[k] - m.Sx0[k];
-        m.DSy[k] = Sy1[k] - m.Sy0[k];
-    }
-}
-
-__device__ __forceinline__ unsigned long long esk_fix(double v) {
-    return (unsigned long long)fix64_raw(v);
+    esk_axis(X0, X1, m.ib, m.mx, m.dx);
+    esk_axis(Y0, Y1, m.jb, m.my, m.dy);
 }
 
 // Calls add(q, k, b, v) for every non-zero fixed-point contribution v of
-// component q (0 Jx, 1 Jy, 2 Jz) at stencil node (ib-1+k, jb-1+b):
-//   Jx = cJx * sum_{a<=k} DSx_a (Sy0_b + DSy_b/2)
-//   Jy = cJy * sum_{c<=b} DSy_c (Sx0_k + DSx_k/2)
-//   Jz = cJz * (Sx0 Sy0 + (DSx Sy0 + Sx0 DSy)/2 + DSx DSy/3)
+// component q (0 Jx, 1 Jy, 2 Jz) at stencil node (ib-1+k, jb-1+b), k, b in
+// 1..3 (nodes ib..ib+2, jb..jb+2):
+//   Jx = cJx * P_k My_b,   P_k = sum_{a<=k} Dx_a
+//   Jy = cJy * Q_b Mx_k,   Q_b = sum_{c<=b} Dy_c
+//   Jz = cJz * (Mx My + Dx Dy / 12)
 // (cJx = -q w/(dt dy), cJy = -q w/(dt dx), cJz = q w uz/(dx dy), each times
-// the fixed-point scale).  Both end points lie at or beyond node ib (jb), so
-// every shape and difference at k = 0 (b = 0) is exactly zero and those
-// terms are skipped; the full prefix sum at k = 3 (b = 3) is the particle's
-// net charge change, zero, and is skipped too.  What remains is the 3x3
-// block of nodes ib..ib+2 x jb..jb+2 with 6 + 6 + 9 contributions.
+// the fixed-point scale).  These are the Esirkepov weights: S0 + D/2 = M and
+// S0 Sy0 + (Dx Sy0 + Sx0 Dy)/2 + Dx Dy/3 = Mx My + Dx Dy/12.  The full prefix
+// sum at k = 3 (b = 3) is the particle's net charge change, zero, and is
+// skipped: 6 + 6 + 9 contributions.  P My, Q Mx and (12 Mx My + Dx Dy)/16
+// lie in [-1, 1] and are exact in FP64 wherever the float positions are
+// spaced 2^-23 or wider (|x| >= 1); each contribution is then one rounding,
+// the fma of fix_mul, of its exact value.
 template <class Add>
 __device__ __forceinline__ void esk_emit(const EskShape& m, double cJx, double cJy, double cJz,
                                          Add&& add) {
-    // every prefix sum and every Jz weight lies in [-1, 1]: clamping the
-    // coefficients keeps each product inside the range of fix64_raw
+    // clamping the coefficients keeps each product inside the range of fix_mul
     cJx = clamp_fix_f64(cJx);
     cJy = clamp_fix_f64(cJy);
-    cJz = clamp_fix_f64(cJz);
+    const double cz = clamp_fix_f64(cJz * (16.0 / 12.0));
+    const double P[2] = {m.dx[0], m.dx[0] + m.dx[1]};
+    const double Q[2] = {m.dy[0], m.dy[0] + m.dy[1]};
 #pragma unroll
-    for (int b = 1; b < 4; ++b) {
-        const double sy = m.Sy0[b] + 0.5 * m.DSy[b];
-        double acc = 0.0;
+    for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int k = 1; k < 3; ++k) {
-            acc += m.DSx[k] * sy;
-            const unsigned long long v = esk_fix(cJx * acc);
-            if (v) add(0, k, b, v);
+        for (int k = 0; k < 2; ++k) {
+            const unsigned long long v = fix_mul(cJx, P[k] * m.my[b]);
+            if (v) add(0, k + 1, b + 1, v);
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const unsigned long long v = fix_mul(cJy, Q[b] * m.mx[k]);
+            if (v) add(1, k + 1, b + 1, v);
+        }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const double my12 = (12.0 / 16.0) * m.my[b], dy16 = (1.0 / 16.0) * m.dy[b];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long v = fix_mul(cz, fma(m.dx[k], dy16, m.mx[k] * my12));
+            if (v) add(2, k + 1, b + 1, v);
         }
     }
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-        const double sx = m.Sx0[k] + 0.5 * m.DSx[k];
-        double acc = 0.0;
-#pragma unroll
-        for (int b = 1; b < 3; ++b) {
-            acc += m.DSy[b] * sx;
-            const unsigned long long v = esk_fix(cJy * acc);
-            if (v) add(1, k, b, v);
-        }
-    }
-#pragma unroll
-    for (int b = 1; b < 4; ++b)
-#pragma unroll
-        for (int k = 1; k < 4; ++k) {
-            const double wz = m.Sx0[k] * m.Sy0[b] +
-                              0.5 * (m.DSx[k] * m.Sy0[b] + m.Sx0[k] * m.DSy[b]) +
-                              m.DSx[k] * m.DSy[b] * (1.0 / 3.0);
-            const unsigned long long v = esk_fix(cJz * wz);
-            if (v) add(2, k, b, v);
-        }
 }
 
 // The dispatcher hands consecutive workgroup ids to the 8 XCDs in turn.  This

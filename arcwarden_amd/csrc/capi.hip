@@ -68,9 +68,6 @@ void deposit_dcu(aw_ctx& c, const aw_particles& p, double qm, const float* const
                  float* const out[3], hipStream_t s);
 void push_boris_em(aw_ctx& c, const aw_particles& p, double qm, double dt, const float* const* F,
                    hipStream_t s);
-void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
-                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
-                       hipStream_t s);
 void convert_multi(unsigned long long* acc, float* out, float* out2, int64_t ncell, int nq,
                    const double* inv, hipStream_t s, int runs, long long sa, long long so,
                    long long so2, int si, const long long* base0 = nullptr);

@@ -12,11 +12,17 @@
 //
 // MI355X design: shapes and products in FP64, contributions summed as int64
 // fixed point (order-free, so every policy returns the same bits).  The
-// per-component scale comes from a bound pass over the particles: max |w| and
+// per-component scale comes from a bound over the particles: max |w| and
 // max |w uz| give each component's largest single contribution, which maps to
-// at most 2^44 (2^19 such contributions of headroom per node).  Policies:
-// global atomics, whole grid in LDS, or tiles binned by the old position with
-// a 2-node apron in LDS (moves that leave it fall back to global atomics).
+// at most 2^44 (2^19 such contributions of headroom per node).  The bound is
+// a max over per-workgroup partial maxima (no atomics), taken by a bound pass
+// here or handed in by a push that already holds the values.  Policies:
+// global atomics, whole grid in LDS, tiles binned by the old position with a
+// 2-node apron in LDS, or -- for particles the caller keeps in tile order --
+// the tile slices of the last sort with a 4-node apron (moves that leave the
+// apron fall back to global atomics).  Each kernel can write the wrapped new
+// position back, which folds the periodic wrap of the caller's move into the
+// pass that already reads it.
 #include "aw_internal.hpp"
 
 namespace aw {
@@ -25,12 +31,14 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kFixBits = 44;
-constexpr int kApron = 2;
+constexpr int kApron = 2;        // particles binned by this step's old position
+constexpr int kSortedApron = 4;  // particles in the tile order of an earlier sort
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
 struct EskArgs {
     const float *x0, *y0, *x1, *y1, *uz, *w;
+    float *xw, *yw;  // optional: x1, y1 wrapped into the box, written back
     int n, nx, ny;
     double qm, w_uniform, dt, dx, dy;
     const unsigned int* bound;  // [0] max|w| (or 0), [1] max|w uz| (|uz| for uniform w)
@@ -77,13 +85,9 @@ __device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
     for (int q = 0; q < 3; ++q) scale[q] = ldexp(1.0, S[q]);
 }
 
-__global__ void __launch_bounds__(kThreads) esk_zero_bound_kernel(unsigned int* bound) {
-    if (threadIdx.x < 2) bound[threadIdx.x] = 0u;
-}
-
-// max |w| and max |w uz| (max |uz| for a uniform weight) as positive-float
-// bit patterns, whose unsigned order is the value order.
-__global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, unsigned int* bound) {
+// per-workgroup max |w| and max |w uz| (max |uz| for a uniform weight) into
+// part[2 b], part[2 b + 1]
+__global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, float* __restrict__ part) {
     float mw = 0.f, mwu = 0.f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += blockDim.x * gridDim.x) {
         const float uz = fabsf(a.uz[i]);
@@ -95,13 +99,34 @@ __global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, unsigned
             mwu = fmaxf(mwu, uz);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        mw = fmaxf(mw, __shfl_xor(mw, off, 64));
-        mwu = fmaxf(mwu, __shfl_xor(mwu, off, 64));
+    dev::block_max2(mw, mwu);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mw;
+        part[2 * blockIdx.x + 1] = mwu;
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(&bound[0], __float_as_uint(mw));
-        atomicMax(&bound[1], __float_as_uint(mwu));
+}
+
+// bound[0..1] = the maxima over npart partial pairs, as positive-float bit
+// patterns
+__global__ void __launch_bounds__(kThreads) esk_bound_final_kernel(const float* __restrict__ part,
+                                                                   int npart,
+                                                                   unsigned int* __restrict__ bound) {
+    float mw = 0.f, mwu = 0.f;
+    for (int b = threadIdx.x; b < npart; b += blockDim.x) {
+        mw = fmaxf(mw, part[2 * b]);
+        mwu = fmaxf(mwu, part[2 * b + 1]);
+    }
+    dev::block_max2(mw, mwu);
+    if (threadIdx.x == 0) {
+        bound[0] = __float_as_uint(mw);
+        bound[1] = __float_as_uint(mwu);
+    }
+}
+
+__device__ __forceinline__ void write_wrapped(const EskArgs& a, int i) {
+    if (a.xw) {
+        a.xw[i] = dev::wrap_coord(a.x1[i], (float)a.nx);
+        a.yw[i] = dev::wrap_coord(a.y1[i], (float)a.ny);
     }
 }
 
@@ -122,6 +147,7 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
     esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
         global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
     });
+    write_wrapped(a, i);
 }
 
 // whole grid (3 components) privatised in LDS per resident workgroup
@@ -141,6 +167,7 @@ __global__ void __launch_bounds__(kThreads) esk_ldsgrid_kernel(EskArgs a,
             const int gj = dev::wrapfar(m.sh.jb - 1 + b, a.ny);
             atomicAdd(&s_j[q * nc + gj * a.nx + gi], v);
         });
+        write_wrapped(a, i);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) {
@@ -150,9 +177,9 @@ __global__ void __launch_bounds__(kThreads) esk_ldsgrid_kernel(EskArgs a,
 }
 
 struct EskTiles {
-    const int32_t* off;  // ntiles+1, tile bins of the old positions
-    const int32_t* idx;  // bin permutation
-    int ntx, tx, ty, W, H, bpt;
+    const int32_t* off;  // ntiles+1, tile bins of the old positions (or of the last sort)
+    const int32_t* idx;  // bin permutation (NULL: the particles are in tile order)
+    int ntx, tx, ty, A, W, H, bpt;  // A: apron nodes on each side, W = tx + 2A, H = ty + 2A
 };
 
 __device__ __forceinline__ int wrap_local(int l, int n) {
@@ -161,13 +188,13 @@ __device__ __forceinline__ int wrap_local(int l, int n) {
     return l;
 }
 
-// workgroup per (tile, part): the tile's J plus a 2-node apron in LDS
+// workgroup per (tile, part): the tile's J plus an A-node apron in LDS
 __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles t,
                                                              unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_j[];
     const int R = t.W * t.H;
     const int tile = blockIdx.x / t.bpt, part = blockIdx.x % t.bpt;
-    const int orgx = (tile % t.ntx) * t.tx - kApron, orgy = (tile / t.ntx) * t.ty - kApron;
+    const int orgx = (tile % t.ntx) * t.tx - t.A, orgy = (tile / t.ntx) * t.ty - t.A;
     for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) s_j[c] = 0ull;
     __syncthreads();
     double scale[3];
@@ -177,7 +204,7 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
     const int chunk = (end - beg + t.bpt - 1) / t.bpt;
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
     for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
-        const int i = t.idx[r];
+        const int i = t.idx ? t.idx[r] : r;
         EskMove m;
         esk_prepare(a, i, scale, m);
         const int li = wrap_local(m.sh.ib - 1 - orgx, a.nx), lj = wrap_local(m.sh.jb - 1 - orgy, a.ny);
@@ -191,6 +218,7 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
                 global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
             });
         }
+        write_wrapped(a, i);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) {
@@ -226,9 +254,10 @@ int tile_bpt(const aw_ctx& c, int64_t n);
 
 void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,
-                       hipStream_t s) {
+                       hipStream_t s, const EskExtra& ex) {
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
     AW_REQUIRE(dt > 0, "dt must be positive");
+    AW_REQUIRE(!ex.xw == !ex.yw, "wrapped x and y outputs go together");
     const int64_t nc = ncell(c);
     c.jacc.reserve((size_t)(3 * nc));
     AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
@@ -240,6 +269,8 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
     a.y1 = p.y;
     a.uz = p.uz;
     a.w = p.w;
+    a.xw = ex.xw;
+    a.yw = ex.yw;
     a.n = (int)p.n;
     a.nx = c.g.nx;
     a.ny = c.g.ny;
@@ -250,19 +281,46 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
     a.dy = c.g.dy;
     a.bound = c.jbound.get();
     unsigned long long* acc = c.jacc.get();
-    esk_zero_bound_kernel<<<1, 64, 0, s>>>(c.jbound.get());
+    const float* part = ex.bound_part;
+    int npart = ex.npart;
+    if (!part) {
+        npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
+        c.jpart.reserve((size_t)std::max(2, 2 * npart));
+        if (npart > 0) {
+            esk_bound_kernel<<<npart, kThreads, 0, s>>>(a, c.jpart.get());
+            AW_LAUNCH_CHECK();
+        }
+        part = c.jpart.get();
+    }
+    esk_bound_final_kernel<<<1, kThreads, 0, s>>>(part, npart, c.jbound.get());
     AW_LAUNCH_CHECK();
     if (p.n > 0) {
-        esk_bound_kernel<<<std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)), kThreads, 0, s>>>(
-            a, c.jbound.get());
-        AW_LAUNCH_CHECK();
-        if (policy == AW_DEPOSIT_AUTO) {
+        if (ex.slice_off) {
+            AW_REQUIRE(policy == AW_DEPOSIT_AUTO || policy == AW_DEPOSIT_BINNED,
+                       "tile-ordered particles take the binned deposit");
+            EskTiles t{};
+            t.off = ex.slice_off;
+            t.idx = nullptr;
+            t.ntx = c.ntx;
+            t.tx = c.tx;
+            t.ty = c.ty;
+            t.A = kSortedApron;
+            t.W = c.tx + 2 * t.A;
+            t.H = c.ty + 2 * t.A;
+            t.bpt = tile_bpt(c, p.n);
+            const size_t sh = (size_t)3 * t.W * t.H * 8;
+            AW_REQUIRE(sh <= c.lds_per_block, "tile apron does not fit LDS");
+            esk_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, acc);
+            AW_LAUNCH_CHECK();
+            policy = -1;  // done
+        } else if (policy == AW_DEPOSIT_AUTO) {
             if ((size_t)(3 * nc) * 8 <= c.lds_per_block && p.n >= 16 * nc)
                 policy = AW_DEPOSIT_LDS_GRID;
             else
                 policy = (double)p.n / (double)nc >= 8.0 ? AW_DEPOSIT_BINNED : AW_DEPOSIT_GLOBAL;
         }
-        if (policy == AW_DEPOSIT_GLOBAL) {
+        if (policy == -1) {
+        } else if (policy == AW_DEPOSIT_GLOBAL) {
             esk_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, acc);
             AW_LAUNCH_CHECK();
         } else if (policy == AW_DEPOSIT_LDS_GRID) {
@@ -288,6 +346,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
             t.ntx = c.ntx;
             t.tx = c.tx;
             t.ty = c.ty;
+            t.A = kApron;
             t.W = c.tx + 2 * kApron;
             t.H = c.ty + 2 * kApron;
             t.bpt = tile_bpt(c, p.n);

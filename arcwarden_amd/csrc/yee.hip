@@ -34,14 +34,11 @@
 
 namespace aw {
 
-void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
-                       double qm, double dt, int policy, float* jx, float* jy, float* jz,
-                       hipStream_t s);
-
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 256;
+constexpr int kPushBlocksPerCu = 8;  // grid-stride push: 8 workgroups of 256 per CU
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
@@ -162,6 +159,7 @@ struct PushArgs {
     float *x0, *y0;      // out: positions before the move
     float* vz;           // out: deposit velocity uz / gamma (rel), else NULL
     float* wdep;         // out: deposit weight w * wd (delta-f), else NULL
+    float* bpart;        // out: per-workgroup (max |w|, max |w uz|) of the deposit, or NULL
     int n;
     double tnow;
 };
@@ -172,9 +170,10 @@ struct PushArgs {
 // weight equation at the time-centred u, the move, and the reflecting x
 // walls of bnd_x.  Positions are stored unwrapped (the wrap runs after the
 // current deposit).
-__global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= a.n) return;
+// One particle; (bw, bwu) take the max of |w| and |w uz| of what the current
+// deposit reads (max |uz| for a uniform weight).
+__device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const aw_run_params& rp,
+                                         int t, float& bw, float& bwu) {
     const float x0 = a.x[t], y0 = a.y[t];
     float Ex = gather_at(m.ex, m, x0, y0, 0.5f, 0.f);
     float Ey = gather_at(m.ey, m, x0, y0, 0.f, 0.5f);
@@ -211,6 +210,7 @@ __global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, 
     uy = fmaf(qmh, Ey, uy);
     uz = fmaf(qmh, Ez, uz);
     const float gnew = rp.rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
+    float wdv = 0.f;
     if (rp.deltaf) {
         // dwd/dt = -(1 - wd) (q/m) F . dln f0/du at u^n = (u^{n-1/2} + u^{n+1/2})/2,
         // F = E + v x dB (wave fields and pump), f0 the bi-Maxwellian with B0
@@ -259,7 +259,8 @@ __global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, 
             wn = fminf(fmaxf((wd + (float)(rp.dt * rp.qm) * (1.f - wd) * S) * relax, lo), hi);
         }
         a.wd[t] = wn;
-        if (a.wdep) a.wdep[t] = (a.w ? a.w[t] : a.w_uniform) * wn;
+        wdv = (a.w ? a.w[t] : a.w_uniform) * wn;
+        if (a.wdep) a.wdep[t] = wdv;
     }
     float x1 = x0 + (float)((double)(ux * gnew) * rp.dt / (double)m.dx);
     const float y1 = y0 + (float)((double)(uy * gnew) * rp.dt / (double)m.dy);
@@ -289,18 +290,43 @@ __global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, 
     a.ux[t] = uxs;
     a.uy[t] = uys;
     a.uz[t] = uzs;
-    if (a.vz) a.vz[t] = uz * gnew;
+    const float uzd = a.vz ? uz * gnew : uzs;
+    if (a.vz) a.vz[t] = uzd;
     a.x0[t] = x0;
     a.y0[t] = y0;
     a.x[t] = x1;
     a.y[t] = y1;
+    // the same float operations as the deposit's own bound pass
+    const float uzb = fabsf(uzd);
+    if (rp.deltaf || a.w) {
+        const float wb = fabsf(rp.deltaf ? wdv : a.w[t]);
+        bw = fmaxf(bw, wb);
+        bwu = fmaxf(bwu, wb * uzb);
+    } else {
+        bwu = fmaxf(bwu, uzb);
+    }
 }
 
-__global__ void wrap_kernel(float* __restrict__ x, float* __restrict__ y, int n, int nx, int ny) {
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= n) return;
-    x[t] = dev::wrap_coord(x[t], (float)nx);
-    y[t] = dev::wrap_coord(y[t], (float)ny);
+// MaxwellSimulation's particle push (include/pic/yee2d.hpp:140-300), grid-
+// stride, with the per-workgroup bound of the current deposit
+__global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
+    float bw = 0.f, bwu = 0.f;
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < a.n;
+         t += (int64_t)kThreads * gridDim.x)
+        push_one(a, m, rp, (int)t, bw, bwu);
+    if (a.bpart) {
+        dev::block_max2(bw, bwu);
+        if (threadIdx.x == 0) {
+            a.bpart[2 * blockIdx.x] = bw;
+            a.bpart[2 * blockIdx.x + 1] = bwu;
+        }
+    }
+}
+
+__global__ void gather_kernel(const float* __restrict__ src, const int32_t* __restrict__ perm,
+                              float* __restrict__ dst, int n) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < n) dst[k] = src[perm[k]];
 }
 
 __global__ void zero_kernel(float* __restrict__ p, int64_t n) {
@@ -308,11 +334,8 @@ __global__ void zero_kernel(float* __restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.f;
 }
 
-// one pass of [1,2,1] x [1,2,1] / 16 smoothing (include/pic/yee2d.hpp:494-508)
-__global__ void __launch_bounds__(kThreads)
-binomial_kernel(const float* __restrict__ src, float* __restrict__ dst, Mesh m) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
+// [1,2,1] x [1,2,1] / 16 at cell c of one periodic grid (include/pic/yee2d.hpp:494-508)
+__device__ __forceinline__ float binomial_at(const float* src, const Mesh& m, int c) {
     const int i = c % m.nx, j = c / m.nx;
     const float w[3] = {0.25f, 0.5f, 0.25f};
     float acc = 0.f;
@@ -321,7 +344,37 @@ binomial_kernel(const float* __restrict__ src, float* __restrict__ dst, Mesh m) 
 #pragma unroll
         for (int di = -1; di <= 1; ++di)
             acc = fmaf(w[di + 1] * w[dj + 1], src[at(m, i + di, j + dj)], acc);
-    dst[c] = acc;
+    return acc;
+}
+
+// one smoothing pass over grid blockIdx.y of consecutive nx*ny grids
+__global__ void __launch_bounds__(kThreads)
+binomial_kernel(const float* __restrict__ src, float* __restrict__ dst, Mesh m) {
+    const int64_t nc = (int64_t)m.nx * m.ny;
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= nc) return;
+    dst[blockIdx.y * nc + c] = binomial_at(src + blockIdx.y * nc, m, c);
+}
+
+// every pass of grid blockIdx.x in LDS, for grids of up to kSmoothLdsCells
+constexpr int kSmoothThreads = 1024;
+constexpr int kSmoothLdsCells = 8192;
+__global__ void __launch_bounds__(kSmoothThreads)
+binomial_lds_kernel(float* __restrict__ a, Mesh m, int passes) {
+    extern __shared__ float s_b[];
+    const int nc = m.nx * m.ny;
+    float* g = a + (int64_t)blockIdx.x * nc;
+    float *cur = s_b, *nxt = s_b + nc;
+    for (int c = threadIdx.x; c < nc; c += kSmoothThreads) cur[c] = g[c];
+    __syncthreads();
+    for (int k = 0; k < passes; ++k) {
+        for (int c = threadIdx.x; c < nc; c += kSmoothThreads) nxt[c] = binomial_at(cur, m, c);
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int c = threadIdx.x; c < nc; c += kSmoothThreads) g[c] = cur[c];
 }
 
 // the rotating transverse antenna current (include/pic/yee2d.hpp:582-607)
@@ -615,10 +668,17 @@ struct aw_maxwell {
     DevBuf<float> mask;           // [node | half-node] x damping masks
     DevBuf<float> vc;             // [vcy | vcz] cold fluid
     DevBuf<float> x0, y0, vz, wdep;
+    DevBuf<float> bpart;          // the push's per-workgroup deposit bounds
     DevBuf<float> rho, rres, r0, dfields;
     DevBuf<double> part;
     bool have_ref = false;
     int64_t nstep = 0;
+    // rp.tile_sort > 0: particles kept in 16x16-tile order, re-sorted every
+    // tile_sort steps; the current deposit walks the tile slices of the last sort
+    DevBuf<float> sbuf;           // sort staging: x y ux uy uz [w] [wd]
+    DevBuf<int32_t> soff, sperm;  // tile offsets of the last sort; its permutation
+    int64_t next_sort = 0, sorted_n = -1;
+    const float* sorted_x = nullptr;
 
     int64_t nc() const { return (int64_t)g.nx * g.ny; }
 
@@ -672,18 +732,81 @@ struct aw_maxwell {
         AW_HIP(hipMemcpy(mask.get(), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
 
-    void smooth(float* a) {  // rp.jfilter binomial passes in place
-        if (rp.jfilter <= 0) return;
+    // rp.jfilter binomial passes, in place, over ncomp consecutive grids
+    void smooth(float* a, int ncomp) {
+        const int J = rp.jfilter;
+        if (J <= 0) return;
         const Mesh m = mesh();
-        float *src = a, *dst = jtmp.get();
-        for (int k = 0; k < rp.jfilter; ++k) {
-            binomial_kernel<<<blocks_for(nc()), kThreads, 0, stream>>>(src, dst, m);
+        const int64_t n = nc();
+        const size_t lds = (size_t)(2 * n) * sizeof(float);
+        if (n <= kSmoothLdsCells && lds <= ctx->lds_per_block) {
+            binomial_lds_kernel<<<ncomp, kSmoothThreads, lds, stream>>>(a, m, J);
             AW_LAUNCH_CHECK();
-            std::swap(src, dst);
+            return;
         }
-        if (src != a)
-            AW_HIP(hipMemcpyAsync(a, src, (size_t)nc() * sizeof(float), hipMemcpyDeviceToDevice,
+        jtmp.reserve((size_t)(2 * ncomp * n));
+        float* t[2] = {jtmp.get(), jtmp.get() + ncomp * n};
+        const dim3 grid(blocks_for(n), ncomp);
+        if (J == 1) {
+            binomial_kernel<<<grid, kThreads, 0, stream>>>(a, t[0], m);
+            AW_LAUNCH_CHECK();
+            AW_HIP(hipMemcpyAsync(a, t[0], (size_t)(ncomp * n) * sizeof(float),
+                                  hipMemcpyDeviceToDevice, stream));
+            return;
+        }
+        // the first pass reads a, the last writes it, the others ping-pong
+        for (int k = 0; k < J; ++k) {
+            const float* src = k == 0 ? a : t[(k - 1) & 1];
+            float* dst = k == J - 1 ? a : t[k & 1];
+            binomial_kernel<<<grid, kThreads, 0, stream>>>(src, dst, m);
+            AW_LAUNCH_CHECK();
+        }
+    }
+
+    // Stable physical sort of the particles into tile order, in place in the
+    // caller's arrays (Particles::sort_by_tile, include/pic/particles.hpp:677-726)
+    void tile_sort(aw_particles& p, const aw_deltaf_state* df) {
+        AW_REQUIRE(!rp.deltaf || rp.df_wprec == 0,
+                   "tile_sort needs df_wprec = 0 (the weight-precision accumulators are not "
+                   "re-sorted); set tile_sort = 0");
+        const size_t n = (size_t)p.n;
+        const bool has_w = p.w != nullptr;
+        const int nf = 5 + (has_w ? 1 : 0) + (rp.deltaf ? 1 : 0);
+        sbuf.reserve((size_t)nf * n);
+        float* b = sbuf.get();
+        float* const from[7] = {p.x, p.y, p.ux, p.uy, p.uz, has_w ? p.w : nullptr,
+                                rp.deltaf ? df->wd : nullptr};
+        aw_particles src = p;
+        float** to[6] = {&src.x, &src.y, &src.ux, &src.uy, &src.uz, &src.w};
+        int k = 0;
+        for (int f = 0; f < 7; ++f) {
+            if (!from[f]) continue;
+            float* d = b + (size_t)k * n;
+            AW_HIP(hipMemcpyAsync(d, from[f], n * sizeof(float), hipMemcpyDeviceToDevice,
                                   stream));
+            if (f < 6) *to[f] = d;
+            ++k;
+        }
+        const float* wd_src = rp.deltaf ? b + (size_t)(nf - 1) * n : nullptr;
+        soff.reserve((size_t)ctx->ntiles + 1);
+        int32_t* perm = nullptr;
+        if (rp.deltaf) {
+            sperm.reserve(n);
+            perm = sperm.get();
+        }
+        if (!tile_sort_onepass(*ctx, src, p, soff.get(), perm, stream)) {
+            sperm.reserve(n);
+            perm = sperm.get();
+            bin_tiles(*ctx, src, perm, soff.get(), stream);
+            permute_particles(src, p, perm, stream);
+        }
+        if (rp.deltaf) {
+            gather_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(wd_src, perm, df->wd,
+                                                                  (int)p.n);
+            AW_LAUNCH_CHECK();
+        }
+        sorted_n = p.n;
+        sorted_x = p.x;
     }
 
     // MaxwellSimulation::step_at (include/pic/simulation_maxwell.hpp:52-109)
@@ -724,19 +847,29 @@ struct aw_maxwell {
             a.wdep = rp.deltaf ? wdep.get() : nullptr;
             a.n = (int)p.n;
             a.tnow = tnow;
-            yee_push_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(a, m, rp);
+            EskExtra ex;
+            if (rp.tile_sort > 0) {
+                if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
+                    tile_sort(p, df);
+                    next_sort = nstep + rp.tile_sort;
+                }
+                ex.slice_off = soff.get();
+            }
+            const int pb = std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
+            bpart.reserve((size_t)(2 * pb));
+            a.bpart = bpart.get();
+            yee_push_kernel<<<pb, kThreads, 0, stream>>>(a, m, rp);
             AW_LAUNCH_CHECK();
             aw_particles dep = p;
             if (own_vz) dep.uz = vz.get();
             if (rp.deltaf) dep.w = wdep.get();
+            ex.bound_part = bpart.get();
+            ex.npart = pb;
+            ex.xw = p.x;  // the periodic wrap of the move
+            ex.yw = p.y;
             deposit_esirkepov(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, AW_DEPOSIT_AUTO,
-                              m.jx, m.jy, m.jz, stream);
-            wrap_kernel<<<blocks_for(p.n), kThreads, 0, stream>>>(p.x, p.y, (int)p.n, g.nx,
-                                                                 g.ny);
-            AW_LAUNCH_CHECK();
-            smooth(m.jx);
-            smooth(m.jy);
-            smooth(m.jz);
+                              m.jx, m.jy, m.jz, stream, ex);
+            smooth(m.jx, 3);
         }
         if (rp.ant_amp != 0.0) {
             if (p.n == 0) zero(m.jx, 3 * nc());
@@ -785,7 +918,6 @@ struct aw_maxwell {
         rho.reserve((size_t)n);
         rres.reserve((size_t)n);
         r0.reserve((size_t)n);
-        jtmp.reserve((size_t)n);
         if (p.n > 0) {
             aw_particles q = p;
             if (wd) {  // delta-f: the perturbation charge, weights w wd
@@ -799,7 +931,7 @@ struct aw_maxwell {
         } else {
             zero(rho.get(), n);
         }
-        smooth(rho.get());
+        smooth(rho.get(), 1);
         gauss_kernel<<<blocks_for(n), kThreads, 0, stream>>>(mesh(), rho.get(), rres.get());
         AW_LAUNCH_CHECK();
         if (!have_ref) {
@@ -857,7 +989,7 @@ aw_status aw_maxwell_create(const aw_grid* g, const aw_run_params* rp, aw_maxwel
         const int64_t nc = m->nc();
         m->F.resize((size_t)(9 * nc));
         m->zero(m->F.get(), 9 * nc);
-        if (rp->jfilter > 0) m->jtmp.resize((size_t)nc);
+        if (rp->jfilter > 0 && nc > kSmoothLdsCells) m->jtmp.resize((size_t)(6 * nc));
         if (rp->bnd_x) m->build_masks();
         if (rp->cold_nc > 0.0) {
             m->vc.resize((size_t)(2 * nc));

@@ -319,6 +319,136 @@ def adjoint_bench(nsteps: int = 100, cpu_steps: int = 0):
             "dL_dv0": float(gU[0, 0] - gU[1, 0]), "dL_dA1": float(gA[0])}
 
 
+def yee_bench(steps: int = 300, warmup: int = 5):
+    """The full-Maxwell (Yee) branch on the reference's chen2026_case1_df deck
+    (tests/golden/decks): a 1D field-aligned line of 5000 cells, dipole
+    background fitted from the deck, loss-cone delta-f hot electrons at 800
+    ppc from the mirror loader, FP32 weights, cold-fluid background,
+    reflecting walls with damping layers, 3-pass binomial current smoothing.
+    Set up as tools/chirp2d.cu:44-99 does: the delta-f f0 from the species
+    (temperatures, loss-cone shape, drift time), wd(0) uniform with rms
+    wdnoise, and band-limited random-phase By, Bz of rms bnoise*wce over
+    k in [0.2, 1.5] (numpy's generator here, so the noise samples differ from
+    the reference's rand()).  Whole-step throughput over `steps` steps."""
+    import numpy as np
+    from arcwarden_amd.deck import load_deck
+    from arcwarden_amd.maxwell import MaxwellSimulation
+    d = load_deck(ROOT / "tests" / "golden" / "decks" / "chen2026_case1_df.ini")
+    rp, sp = d.rp, d.species[0]
+    rp.deltaf = 1
+    rp.df_tpar = sp.uth[0] ** 2
+    rp.df_tperp = sp.uth[1] ** 2
+    rp.df_taud, rp.df_dist = sp.taud, sp.dist
+    rp.df_rho, rp.df_kappa = sp.lc_rho, sp.lc_kappa
+    g = d.grid()
+    sim = MaxwellSimulation(g, rp)
+    sim.load_mirror(sp)
+    n = sim.n
+    rng = np.random.default_rng(int(rp.rng_seed) + 7)
+    a = sp.wdnoise * np.sqrt(3.0)
+    sim.enable_deltaf(rp.df_wprec, wd0=rng.uniform(-a, a, n).astype(np.float32))
+    lx = g.nx * g.dx
+    m1 = max(1, int(np.ceil(0.2 * lx / (2 * np.pi))))
+    m2 = min(g.nx // 2 - 1, int(1.5 * lx / (2 * np.pi)))
+    amp = d.bnoise * rp.wce / np.sqrt(m2 - m1 + 1)
+    i = np.arange(g.nx)
+    nb, nz = np.zeros(g.nx), np.zeros(g.nx)
+    for m in range(m1, m2 + 1):
+        k = 2 * np.pi * m / lx
+        p1, p2 = rng.uniform(0, 2 * np.pi, 2)
+        nb += amp * np.cos(k * (i + 0.5) * g.dx + p1)
+        nz += amp * np.cos(k * i * g.dx + p2)
+    sim.set_mesh(by=nb, bz=nz)
+    for _ in range(warmup):
+        sim.step()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    we, wb = sim.field_energy()
+    wsum, wrms, wmax = sim.wd_stats()
+    sim.close()
+    # per particle-step: the push reads x, y, ux, uy, uz, w, wd (28 B) and
+    # writes x, y, ux, uy, uz, wd, the old x, y and the deposit weight (36 B);
+    # the current reads the old and new x, y, uz and that weight (24 B)
+    bpp = 88
+    out = {"workload": "Yee delta-f whistler line, chen2026_case1_df deck: 5000x1 cells, "
+                       "dipole background, loss-cone hot electrons 800 ppc, walls, "
+                       "jfilter 3, wd and By/Bz noise seeds", "particles": n, "steps": steps,
+           "value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3,
+           "step_bytes_per_particle": bpp,
+           "step_GBs": n * bpp * steps / el / 1e9,
+           "field_energy": [we, wb], "wd_rms": wrms}
+    print(f"# yee: {n} particles, {out['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
+def yee_eaw_bench(steps: int = 40, warmup: int = 5):
+    """The reference's published Yee benchmark, the eaw_case7_tiled deck
+    (tests/golden/decks): 625x625 periodic cells, 400 ppc (156.25 M
+    electrons, T_perp/T_par = 5), B0 = 0.25 along x, c = 1, dt 0.0145,
+    3-pass binomial current smoothing, tile re-sort every 20 steps, no
+    diagnostics inside the timed steps.  The reference quotes 1.51e10
+    particle-steps/s for this run without diagnostics on an RTX 5090
+    (docs/PROFILE_BASELINE.md:39-50)."""
+    from arcwarden_amd.deck import load_deck
+    from arcwarden_amd.maxwell import MaxwellSimulation
+    d = load_deck(ROOT / "tests" / "golden" / "decks" / "eaw_case7_tiled.ini")
+    sim = MaxwellSimulation(d.grid(), d.rp)
+    sim.load_species(d.species)
+    n = sim.n
+    for _ in range(warmup):
+        sim.step()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    we, wb = sim.field_energy()
+    sim.close()
+    ref = 1.51e10
+    out = {"workload": "Yee eaw_case7_tiled deck: 625x625 cells, 400 ppc, B0 0.25 x, "
+                       "jfilter 3, tile_sort 20", "particles": n, "steps": steps,
+           "value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3,
+           "reference_rtx5090": ref, "vs_reference": n * steps / el / ref,
+           "field_energy": [we, wb]}
+    print(f"# yee eaw_case7: {n} particles, {out['ms_per_step']:.4f} ms/step", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def checkpoint_bench(sim_factory, sim, step: int):
+    """Write the running simulation's restart file (ARCW v1: particles, the
+    sources of the next step, the last fields) to a temporary directory, then
+    restore it into a fresh simulation of the same configuration; bytes and
+    wall time of each."""
+    import tempfile
+    d = tempfile.mkdtemp(prefix="aw_ck_")
+    path = os.path.join(d, "run.arcw")
+    try:
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.save_checkpoint(path, step)
+        t1 = time.perf_counter()
+        size = os.path.getsize(path)
+        other = sim_factory()
+        t2 = time.perf_counter()
+        other.restore_checkpoint(path)
+        other.synchronize()
+        t3 = time.perf_counter()
+        other.close()
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+        os.rmdir(d)
+    return {"bytes": size, "write_s": t1 - t0, "read_s": t3 - t2,
+            "write_GBs": size / (t1 - t0) / 1e9, "read_GBs": size / (t3 - t2) / 1e9,
+            "where": "the box's local temporary directory"}
+
+
 def darwin_bench(steps: int, current: int, autotune: bool = True, batch: bool = True):
     """The Darwin whistler-pump scan (v_r/v_th in {3.2, 2.1, 1.0}, 512x128,
     256 ppc = 16,777,216 electrons per run) stepped run after run by aw_sim
@@ -529,6 +659,12 @@ def main():
                     help="fixed square tile size (implies --no-autotune; 0: default)")
     ap.add_argument("--bpt", type=int, default=0,
                     help="fixed workgroups per tile (implies --no-autotune; 0: default)")
+    ap.add_argument("--yee-steps", type=int, default=300,
+                    help="steps of the Yee delta-f deck (0: skip)")
+    ap.add_argument("--eaw-steps", type=int, default=40,
+                    help="steps of the eaw_case7_tiled Yee deck (0: skip)")
+    ap.add_argument("--checkpoint", type=int, default=1,
+                    help="1: time a restart file write and restore of the headline run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device copy (torch clone) so that a "
                          "rocprofv3 --pmc run can scale FETCH_SIZE/WRITE_SIZE by a "
@@ -560,11 +696,17 @@ def main():
     rp = L.run_params(dt=0.1, qm=-1.0, eps0=1.0, noisy_load=1, rng_seed=20260628,
                       B0=(0.0, 0.0, 0.5), dump_every=10)
     sp = [L.species("electrons", 1.0, args.ppc, (0.05, 0.05, 0.05), (0.0, 0.0, 0.0))]
-    def timed_run(sort_every, keep_state):
-        sim = Simulation(g, rp, sp, SimConfig(has_b0=True, sort_every=sort_every,
-                                              tile_x=args.tile, tile_y=args.tile,
-                                              blocks_per_tile=args.bpt,
-                                              autotune=not args.no_autotune))
+
+    def make_sim(sort_every):
+        return Simulation(g, rp, sp, SimConfig(has_b0=True, sort_every=sort_every,
+                                               tile_x=args.tile, tile_y=args.tile,
+                                               blocks_per_tile=args.bpt,
+                                               autotune=not args.no_autotune))
+
+    ck_out = {}
+
+    def timed_run(sort_every, keep_state, checkpoint=False):
+        sim = make_sim(sort_every)
         sim.init()
         n = sim.particle_views().n
         exec_cfg = sim.exec_config()
@@ -585,11 +727,14 @@ def main():
         t1 = time.perf_counter()
         kms, launches, sorts = sim.kernel_time()
         hist = sim.history()
+        if checkpoint:  # outside the timed region
+            ck_out.update(checkpoint_bench(lambda: make_sim(sort_every), sim,
+                                           args.warmup + args.steps))
         sim.close()
         return dict(n=n, exec_cfg=exec_cfg, elapsed=t1 - t0, kms=kms, launches=launches,
                     sorts=sorts, hist=hist, state=state, rho0=rho0)
 
-    main_run = timed_run(args.sort_every, not args.no_cpu)
+    main_run = timed_run(args.sort_every, not args.no_cpu, checkpoint=args.checkpoint)
     n, exec_cfg, elapsed = main_run["n"], main_run["exec_cfg"], main_run["elapsed"]
     kms, launches, sorts, hist = (main_run[k] for k in ("kms", "launches", "sorts", "hist"))
     state, rho0 = main_run["state"], main_run["rho0"]
@@ -619,6 +764,8 @@ def main():
         kw = dict(autotune=bool(args.darwin_autotune), batch=not args.no_darwin_batch)
         dar = darwin_bench(args.darwin_steps, L.AW_CURRENT_ESIRKEPOV, **kw)
         dar_cic = darwin_bench(args.darwin_steps, L.AW_CURRENT_CIC, **kw)
+    yee = yee_bench(args.yee_steps) if args.yee_steps > 0 else None
+    yee_eaw = yee_eaw_bench(args.eaw_steps) if args.eaw_steps > 0 else None
     pol = None if args.no_policies else policy_bench(nx, args.ppc)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -658,6 +805,9 @@ def main():
         "two_stream": ts2,
         "darwin": dar,
         "darwin_cic_current": dar_cic,
+        "yee": yee,
+        "yee_eaw_case7": yee_eaw,
+        "checkpoint": ck_out or None,
         "policies": pol,
         "checks": {"charge_drift": charge_drift, "sorts": sorts,
                    "fixed_sort_cadence": fixed},

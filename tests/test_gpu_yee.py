@@ -4,7 +4,11 @@
   (FP32, same operation sequence), the current of the move within 1e-6 of
   the FP64 Esirkepov restatement, across B0 orientations, the relativistic
   push, delta-f weights (three precisions), the mirror background on a
-  one-row grid, reflecting walls and the pump;
+  one-row grid, reflecting walls and the pump, and the current smoothing in
+  LDS and in global passes;
+- the tile_sort path (particles kept in tile order, the current deposited
+  from the tile slices of the last sort) bit for bit against the unsorted
+  run, full-f and delta-f;
 - the conservation diagnostics against the oracle, and the reference's
   physics gates: vacuum dispersion (tests/test_yee_vacuum.cu, 3e-4), cold
   Langmuir on the Maxwell path (tests/test_yee_langmuir.cu, 2%), the
@@ -57,6 +61,9 @@ CASES = {
                                              b0_xc=16.0)),
     "walls_smoothing": (48, 8, 0.5, 0.5, dict(B0=(0.2, 0.0, 0.3), bnd_x=2, bnd_nd=6,
                                               bnd_numax=0.5, jfilter=2)),
+    # grids above the in-LDS smoothing size: the multi-launch passes
+    "smoothing_3_passes_large": (128, 80, 0.5, 0.5, dict(B0=(0.0, 0.3, 0.2), jfilter=3)),
+    "smoothing_1_pass_large": (128, 80, 0.5, 0.5, dict(B0=(0.0, 0.3, 0.2), jfilter=1)),
     "pump": (48, 12, 0.5, 0.5, dict(B0=(0.4, 0.0, 0.2), pump=1, pump_ex=1e-3, pump_ey=2e-3,
                                     pump_ez=-2e-3, pump_k0=TWO_PI * 3 / 24.0, pump_w0=0.15,
                                     pump_trmp=0.5, pump_toff=100.0)),
@@ -383,3 +390,75 @@ def test_mirror_load_counts_and_moments(gpu, oracle):
     up2 = st["uy"].astype(np.float64) ** 2 + st["uz"].astype(np.float64) ** 2
     per_cell = np.bincount(cells, weights=up2, minlength=nx) / np.diff(off)
     assert np.allclose(per_cell, 2 * tp, rtol=0.15)
+
+
+def _tile_sort_run(tile_sort, deltaf, steps=15, seed=11):
+    nx, ny, dx = 64, 48, 0.5
+    g = L.grid(nx, ny, nx * dx, ny * dx)
+    rp = L.run_params(dt=0.06, qm=-1.0, eps0=1.0, c=5.0)
+    rp.B0[0], rp.B0[2] = 0.4, 0.1
+    rp.jfilter, rp.tile_sort = 3, tile_sort
+    if deltaf:
+        rp.B0[2] = 0.0
+        rp.deltaf, rp.df_tpar, rp.df_tperp = 1, 1.5, 2.0
+    rng = np.random.default_rng(seed)
+    n = 200_000
+    st = {"x": rng.uniform(0, nx, n).astype(np.float32),
+          "y": rng.uniform(0, ny, n).astype(np.float32)}
+    for k in ("ux", "uy", "uz"):  # up to ~0.8 cells a step, ~2 cells between sorts
+        st[k] = rng.normal(0, 1.5, n).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32) * (dx * dx / 64)
+    sim = MaxwellSimulation(g, rp)
+    sim.set_particles(st, w=w)
+    if deltaf:
+        sim.enable_deltaf(0, wd0=rng.uniform(-0.05, 0.05, n).astype(np.float32))
+    nc = nx * ny
+    F0 = rng.normal(0, 1e-2, 6 * nc).astype(np.float32)
+    names = ("ex", "ey", "ez", "bx", "by", "bz")
+    sim.set_mesh(**{k: F0[q * nc:(q + 1) * nc] for q, k in enumerate(names)})
+    for _ in range(steps):
+        sim.step()
+    M = sim.mesh_flat()
+    P = sim.particles()
+    sim.close()
+    return M, P
+
+
+def _as_set(P, keys):
+    cols = np.stack([P[k] for k in keys])
+    order = np.lexsort(cols[::-1])
+    return cols[:, order]
+
+
+@pytest.mark.parametrize("deltaf", [False, True])
+def test_tile_sort_matches_unsorted(gpu, deltaf):
+    """MaxwellSimulation::step_at with rp.tile_sort (include/pic/simulation_maxwell.hpp:
+    60-72): re-sorting the particles every 5 steps only reorders them -- the
+    fields after 15 steps are the same bits, and so is the particle set."""
+    M0, P0 = _tile_sort_run(0, deltaf)
+    M1, P1 = _tile_sort_run(5, deltaf)
+    print(f"tile_sort deltaf={deltaf}: max |J| {np.abs(M0[-3 * 3072:]).max():.3e}", flush=True)
+    assert np.array_equal(M0, M1)
+    keys = ("x", "y", "ux", "uy", "uz", "w") + (("wd",) if deltaf else ())
+    assert np.array_equal(_as_set(P0, keys), _as_set(P1, keys))
+    # the sorted run keeps the particles near tile order (the unsorted one
+    # steps back to a lower tile at about every other particle)
+    t = (P1["y"] // 16).astype(np.int64) * 4 + (P1["x"] // 16).astype(np.int64)
+    assert np.mean(np.diff(t) < 0) < 0.25
+
+
+def test_tile_sort_refuses_weight_precision_accumulators(gpu):
+    g = L.grid(32, 32, 16.0, 16.0)
+    rp = L.run_params(dt=0.04, qm=-1.0, eps0=1.0, c=5.0)
+    rp.B0[0] = 0.5
+    rp.deltaf, rp.df_tpar, rp.df_tperp, rp.df_wprec, rp.tile_sort = 1, 0.01, 0.02, 1, 4
+    rng = np.random.default_rng(2)
+    n = 1000
+    st = {"x": rng.uniform(0, 32, n), "y": rng.uniform(0, 32, n),
+          "ux": rng.normal(0, 0.1, n), "uy": rng.normal(0, 0.1, n), "uz": rng.normal(0, 0.1, n)}
+    sim = MaxwellSimulation(g, rp)
+    sim.set_particles(st, w=np.full(n, 0.25, np.float32))
+    sim.enable_deltaf(1)
+    with pytest.raises(Exception, match="tile_sort"):
+        sim.step()
+    sim.close()
