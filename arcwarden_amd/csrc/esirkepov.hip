@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, float* _
 
 // bound[0..1] = the maxima over npart partial pairs, as positive-float bit
 // patterns
-__global__ void __launch_bounds__(kThreads) esk_bound_final_kernel(const float* __restrict__ part,
+constexpr int kFinalThreads = 1024;
+__global__ void __launch_bounds__(kFinalThreads) esk_bound_final_kernel(const float* __restrict__ part,
                                                                    int npart,
                                                                    unsigned int* __restrict__ bound) {
     float mw = 0.f, mwu = 0.f;
@@ -193,7 +194,9 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
                                                              unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_j[];
     const int R = t.W * t.H;
-    const int tile = blockIdx.x / t.bpt, part = blockIdx.x % t.bpt;
+    // a tile's workgroups on one XCD: their flushes meet in one L2
+    const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+    const int tile = lb / t.bpt, part = lb % t.bpt;
     const int orgx = (tile % t.ntx) * t.tx - t.A, orgy = (tile / t.ntx) * t.ty - t.A;
     for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) s_j[c] = 0ull;
     __syncthreads();
@@ -292,7 +295,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
         }
         part = c.jpart.get();
     }
-    esk_bound_final_kernel<<<1, kThreads, 0, s>>>(part, npart, c.jbound.get());
+    esk_bound_final_kernel<<<1, kFinalThreads, 0, s>>>(part, npart, c.jbound.get());
     AW_LAUNCH_CHECK();
     if (p.n > 0) {
         if (ex.slice_off) {

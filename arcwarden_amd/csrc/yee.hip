@@ -34,11 +34,14 @@
 
 namespace aw {
 
+int tile_bpt(const aw_ctx& c, int64_t n);
+
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRedBlocks = 256;
 constexpr int kPushBlocksPerCu = 8;  // grid-stride push: 8 workgroups of 256 per CU
+constexpr int kPushApron = 2;        // tile push: cells of drift since the sort in LDS
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
@@ -85,20 +88,6 @@ __global__ void __launch_bounds__(kThreads) ampere_kernel(Mesh m) {
     m.ex[c] = m.ex[c] + m.dt * (m.c2 * dbz_y - m.jx[c]);
     m.ey[c] = m.ey[c] + m.dt * (-m.c2 * dbz_x - m.jy[c]);
     m.ez[c] = m.ez[c] + m.dt * (m.c2 * (dby_x - dbx_y) - m.jz[c]);
-}
-
-// CIC interpolation of a grid whose samples sit at (i + ox, j + oy).
-__device__ __forceinline__ float gather_at(const float* a, const Mesh& m, float x, float y,
-                                           float ox, float oy) {
-    const float xs = x - ox, ys = y - oy;
-    const float fi = floorf(xs), fj = floorf(ys);
-    const int i0 = dev::wrapfar((int)fi, m.nx), j0 = dev::wrapfar((int)fj, m.ny);
-    const int i1 = i0 + 1 == m.nx ? 0 : i0 + 1, j1 = j0 + 1 == m.ny ? 0 : j0 + 1;
-    const float fx = xs - fi, fy = ys - fj;
-    const float gx = 1.f - fx, gy = 1.f - fy;
-    const float v00 = a[j0 * m.nx + i0], v10 = a[j0 * m.nx + i1];
-    const float v01 = a[j1 * m.nx + i0], v11 = a[j1 * m.nx + i1];
-    return fmaf(fx * fy, v11, fmaf(gx * fy, v01, fmaf(fx * gy, v10, (gx * gy) * v00)));
 }
 
 // u' = u + u x t, u += u' x s with t = h B, s = 2t/(1 + t^2)
@@ -170,14 +159,98 @@ struct PushArgs {
 // weight equation at the time-centred u, the move, and the reflecting x
 // walls of bnd_x.  Positions are stored unwrapped (the wrap runs after the
 // current deposit).
+// The six fields sit on four staggerings: (ex, by) at (i + 1/2, j), (ey, bx)
+// at (i, j + 1/2), ez at (i, j), bz at (i + 1/2, j + 1/2).  A gather is a
+// staggering's CIC stencil, set up once and applied to its fields.
+struct Cic {
+    float fx, fy, gx, gy;
+    int i0, j0;  // lower corner (cell index before wrapping)
+    __device__ __forceinline__ Cic(float x, float y, float ox, float oy) {
+        const float xs = x - ox, ys = y - oy;
+        const float fi = floorf(xs), fj = floorf(ys);
+        i0 = (int)fi;
+        j0 = (int)fj;
+        fx = xs - fi;
+        fy = ys - fj;
+        gx = 1.f - fx;
+        gy = 1.f - fy;
+    }
+    __device__ __forceinline__ float mix(float v00, float v10, float v01, float v11) const {
+        return fmaf(fx * fy, v11, fmaf(gx * fy, v01, fmaf(fx * gy, v10, (gx * gy) * v00)));
+    }
+};
+
+__device__ __forceinline__ float cic_mesh(const float* a, const Mesh& m, const Cic& c) {
+    const int i0 = dev::wrapfar(c.i0, m.nx), j0 = dev::wrapfar(c.j0, m.ny);
+    const int i1 = i0 + 1 == m.nx ? 0 : i0 + 1, j1 = j0 + 1 == m.ny ? 0 : j0 + 1;
+    return c.mix(a[j0 * m.nx + i0], a[j0 * m.nx + i1], a[j1 * m.nx + i0], a[j1 * m.nx + i1]);
+}
+
+constexpr float kOx[4] = {0.5f, 0.f, 0.f, 0.5f}, kOy[4] = {0.f, 0.5f, 0.f, 0.5f};
+
+// gathers straight from the mesh; staggering s: 0 (ex, by), 1 (ey, bx), 2 ez, 3 bz
+struct MeshGather {
+    const Mesh& m;
+    __device__ __forceinline__ void two(int s, float x, float y, float& a, float& b) const {
+        const Cic c(x, y, kOx[s], kOy[s]);
+        a = cic_mesh(s == 0 ? m.ex : m.ey, m, c);
+        b = cic_mesh(s == 0 ? m.by : m.bx, m, c);
+    }
+    __device__ __forceinline__ float one(int s, float x, float y) const {
+        const Cic c(x, y, kOx[s], kOy[s]);
+        return cic_mesh(s == 2 ? m.ez : m.bz, m, c);
+    }
+};
+
+// gathers from a workgroup's LDS copy of the fields over the cells
+// [orgx, orgx + W) x [orgy, orgy + H) (periodically wrapped): planes of
+// float2 (ex, by), float2 (ey, bx), float ez, float bz.  A stencil that
+// leaves the window reads the mesh; the window holds the mesh's own values,
+// so both return the same bits.
+struct WindowGather {
+    const float* s;
+    const Mesh& m;
+    int orgx, orgy, W, H;
+    __device__ __forceinline__ bool local(const Cic& c, int& l) const {
+        int li = c.i0 - orgx, lj = c.j0 - orgy;
+        if ((unsigned)li > (unsigned)(W - 2)) li += li < 0 ? m.nx : -m.nx;
+        if ((unsigned)lj > (unsigned)(H - 2)) lj += lj < 0 ? m.ny : -m.ny;
+        l = lj * W + li;
+        return (unsigned)li <= (unsigned)(W - 2) && (unsigned)lj <= (unsigned)(H - 2);
+    }
+    __device__ __forceinline__ void two(int st, float x, float y, float& a, float& b) const {
+        const Cic c(x, y, kOx[st], kOy[st]);
+        int l;
+        if (!local(c, l)) {
+            a = cic_mesh(st == 0 ? m.ex : m.ey, m, c);
+            b = cic_mesh(st == 0 ? m.by : m.bx, m, c);
+            return;
+        }
+        const float2* p = reinterpret_cast<const float2*>(s) + st * W * H + l;
+        const float2 v00 = p[0], v10 = p[1], v01 = p[W], v11 = p[W + 1];
+        a = c.mix(v00.x, v10.x, v01.x, v11.x);
+        b = c.mix(v00.y, v10.y, v01.y, v11.y);
+    }
+    __device__ __forceinline__ float one(int st, float x, float y) const {
+        const Cic c(x, y, kOx[st], kOy[st]);
+        int l;
+        if (!local(c, l)) return cic_mesh(st == 2 ? m.ez : m.bz, m, c);
+        const float* p = s + (st + 2) * W * H + l;
+        return c.mix(p[0], p[1], p[W], p[W + 1]);
+    }
+};
+
 // One particle; (bw, bwu) take the max of |w| and |w uz| of what the current
 // deposit reads (max |uz| for a uniform weight).
+template <class Gather>
 __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const aw_run_params& rp,
-                                         int t, float& bw, float& bwu) {
+                                         int t, float& bw, float& bwu, const Gather& gat) {
     const float x0 = a.x[t], y0 = a.y[t];
-    float Ex = gather_at(m.ex, m, x0, y0, 0.5f, 0.f);
-    float Ey = gather_at(m.ey, m, x0, y0, 0.f, 0.5f);
-    float Ez = gather_at(m.ez, m, x0, y0, 0.f, 0.f);
+    float Ex, Ey, dBx, dBy;
+    gat.two(0, x0, y0, Ex, dBy);
+    gat.two(1, x0, y0, Ey, dBx);
+    float Ez = gat.one(2, x0, y0);
+    const float dBz = gat.one(3, x0, y0);
     if (rp.pump && a.tnow < rp.pump_toff) {
         const double th = rp.pump_k0 * (double)x0 * (double)m.dx - rp.pump_w0 * a.tnow;
         const double r = ramp_of(a.tnow, rp.pump_trmp, rp.pump_toff);
@@ -185,9 +258,6 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
         Ey += (float)(-rp.pump_ey * sin(th) * r);
         Ez += (float)(rp.pump_ez * cos(th) * r);
     }
-    const float dBx = gather_at(m.bx, m, x0, y0, 0.f, 0.5f);
-    const float dBy = gather_at(m.by, m, x0, y0, 0.5f, 0.f);
-    const float dBz = gather_at(m.bz, m, x0, y0, 0.5f, 0.5f);
     float ux = a.ux[t], uy = a.uy[t], uz = a.uz[t];
     const float uxo = ux, uyo = uy, uzo = uz;  // u^{n-1/2}
     const float qmh = (float)(rp.qm * 0.5 * rp.dt);
@@ -311,15 +381,54 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
 // stride, with the per-workgroup bound of the current deposit
 __global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
     float bw = 0.f, bwu = 0.f;
+    const MeshGather gat{m};
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < a.n;
          t += (int64_t)kThreads * gridDim.x)
-        push_one(a, m, rp, (int)t, bw, bwu);
+        push_one(a, m, rp, (int)t, bw, bwu, gat);
     if (a.bpart) {
         dev::block_max2(bw, bwu);
         if (threadIdx.x == 0) {
             a.bpart[2 * blockIdx.x] = bw;
             a.bpart[2 * blockIdx.x + 1] = bwu;
         }
+    }
+}
+
+// Particles in the tile order of an earlier sort: a workgroup per (tile,
+// part) pushes its part of the tile's slice, gathering from an LDS copy of
+// the tile's fields with a D-cell apron for the drift since the sort.
+struct PushTiles {
+    const int32_t* off;  // ntiles + 1
+    int ntx, tx, ty, D, W, H, bpt;  // W = tx + 2 D + 2, H = ty + 2 D + 2
+};
+
+__global__ void __launch_bounds__(kThreads)
+yee_push_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t) {
+    extern __shared__ float s_f[];
+    const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
+    const int tile = lb / t.bpt, part = lb % t.bpt;
+    const int orgx = (tile % t.ntx) * t.tx - t.D - 1, orgy = (tile / t.ntx) * t.ty - t.D - 1;
+    const int R = t.W * t.H;
+    float2* s2 = reinterpret_cast<float2*>(s_f);
+    for (int l = threadIdx.x; l < R; l += kThreads) {
+        const int gx = dev::wrapfar(orgx + l % t.W, m.nx), gy = dev::wrapfar(orgy + l / t.W, m.ny);
+        const int g = gy * m.nx + gx;
+        s2[l] = make_float2(m.ex[g], m.by[g]);
+        s2[R + l] = make_float2(m.ey[g], m.bx[g]);
+        s_f[4 * R + l] = m.ez[g];
+        s_f[5 * R + l] = m.bz[g];
+    }
+    __syncthreads();
+    const WindowGather gat{s_f, m, orgx, orgy, t.W, t.H};
+    const int beg = t.off[tile], end = t.off[tile + 1];
+    const int chunk = (end - beg + t.bpt - 1) / t.bpt;
+    const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
+    float bw = 0.f, bwu = 0.f;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += kThreads) push_one(a, m, rp, r, bw, bwu, gat);
+    dev::block_max2(bw, bwu);
+    if (threadIdx.x == 0) {
+        a.bpart[2 * blockIdx.x] = bw;
+        a.bpart[2 * blockIdx.x + 1] = bwu;
     }
 }
 
@@ -855,10 +964,29 @@ struct aw_maxwell {
                 }
                 ex.slice_off = soff.get();
             }
-            const int pb = std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
-            bpart.reserve((size_t)(2 * pb));
-            a.bpart = bpart.get();
-            yee_push_kernel<<<pb, kThreads, 0, stream>>>(a, m, rp);
+            int pb = 0;
+            if (rp.tile_sort > 0) {
+                PushTiles t{};
+                t.off = soff.get();
+                t.ntx = ctx->ntx;
+                t.tx = ctx->tx;
+                t.ty = ctx->ty;
+                t.D = kPushApron;
+                t.W = t.tx + 2 * t.D + 2;
+                t.H = t.ty + 2 * t.D + 2;
+                t.bpt = tile_bpt(*ctx, p.n);
+                pb = ctx->ntiles * t.bpt;
+                bpart.reserve((size_t)(2 * pb));
+                a.bpart = bpart.get();
+                const size_t sh = (size_t)6 * t.W * t.H * sizeof(float);
+                AW_REQUIRE(sh <= ctx->lds_per_block, "push field window does not fit LDS");
+                yee_push_tiles_kernel<<<pb, kThreads, sh, stream>>>(a, m, rp, t);
+            } else {
+                pb = std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
+                bpart.reserve((size_t)(2 * pb));
+                a.bpart = bpart.get();
+                yee_push_kernel<<<pb, kThreads, 0, stream>>>(a, m, rp);
+            }
             AW_LAUNCH_CHECK();
             aw_particles dep = p;
             if (own_vz) dep.uz = vz.get();
