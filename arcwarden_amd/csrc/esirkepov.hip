@@ -151,8 +151,11 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
     write_wrapped(a, i);
 }
 
-// whole grid (3 components) privatised in LDS per resident workgroup
-__global__ void __launch_bounds__(kThreads) esk_ldsgrid_kernel(EskArgs a,
+// whole grid (3 components) privatised in LDS per resident workgroup; the
+// grid takes most of a CU's LDS, so a workgroup is 16 waves to keep the CU
+// busy
+constexpr int kGridThreads = 1024;
+__global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_kernel(EskArgs a,
                                                                unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_j[];
     const int nc = a.nx * a.ny;
@@ -331,9 +334,10 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
             AW_REQUIRE(sh <= c.lds_per_block, "grid does not fit LDS for AW_DEPOSIT_LDS_GRID");
             int per_cu = 0;
             AW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel), kThreads, sh));
-            const int nb = std::min(std::max(1, per_cu) * std::max(1, c.num_cu), blocks_for(p.n));
-            esk_ldsgrid_kernel<<<nb, kThreads, sh, s>>>(a, acc);
+                &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel), kGridThreads, sh));
+            const int nb = (int)std::min<int64_t>(std::max(1, per_cu) * std::max(1, c.num_cu),
+                                                  (p.n + kGridThreads - 1) / kGridThreads);
+            esk_ldsgrid_kernel<<<nb, kGridThreads, sh, s>>>(a, acc);
             AW_LAUNCH_CHECK();
         } else if (policy == AW_DEPOSIT_BINNED) {
             // bin by the old position (stable), as deposit_rho_fixed does
