@@ -21,7 +21,9 @@ on this host's cores on the same particle state.  ``ensemble`` compares a
 times the field-energy gradient of a 128^2 two-stream run against its plain
 forward run; ``darwin`` times the three-run Darwin whistler-pump scan with
 the Esirkepov current deposit (``darwin_cic_current``: with the reference's
-CIC current).
+CIC current); ``yee_eaw_case7`` runs the reference's own Yee benchmark deck
+and ``yee`` its delta-f whistler line; ``checkpoint`` times a restart file
+write and restore of the headline run, outside the timed region.
 """
 from __future__ import annotations
 
