@@ -74,9 +74,24 @@ __device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const doubl
     m.cJz = qw * (double)a.uz[i] / (a.dx * a.dy) * scale[2];
 }
 
-template <class Add>
+// The contributions of one move.  On a one-row grid (ny = 1) the three
+// stencil rows are the same row: each (component, column) sum of the
+// fixed-point contributions is formed in registers (exact, int64) and added
+// once, at row 1, instead of once per row.
+template <bool ONE_ROW, class Add>
 __device__ __forceinline__ void esk_emit(const EskMove& m, Add&& add) {
-    dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz, add);
+    if constexpr (!ONE_ROW) {
+        dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz, add);
+        return;
+    }
+    unsigned long long col[3][4] = {};
+    dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz,
+                  [&](int q, int k, int, unsigned long long v) { col[q][k] += v; });
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (col[q][k]) add(q, k, 1, col[q][k]);
 }
 
 __device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
@@ -136,6 +151,7 @@ __device__ __forceinline__ void global_add(unsigned long long* acc, int nx, int 
     atomicAdd(&acc[q * nc + (int64_t)dev::wrapfar(j, ny) * nx + dev::wrapfar(i, nx)], v);
 }
 
+template <bool ONE_ROW>
 __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
                                                               unsigned long long* __restrict__ acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,7 +161,7 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
     const int64_t nc = (int64_t)a.nx * a.ny;
     EskMove m;
     esk_prepare(a, i, scale, m);
-    esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+    esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
         global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
     });
     write_wrapped(a, i);
@@ -155,6 +171,7 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
 // grid takes most of a CU's LDS, so a workgroup is 16 waves to keep the CU
 // busy
 constexpr int kGridThreads = 1024;
+template <bool ONE_ROW>
 __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_kernel(EskArgs a,
                                                                unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_j[];
@@ -166,7 +183,7 @@ __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_kernel(EskArgs a,
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += blockDim.x * gridDim.x) {
         EskMove m;
         esk_prepare(a, i, scale, m);
-        esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+        esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
             const int gi = dev::wrapfar(m.sh.ib - 1 + k, a.nx);
             const int gj = dev::wrapfar(m.sh.jb - 1 + b, a.ny);
             atomicAdd(&s_j[q * nc + gj * a.nx + gi], v);
@@ -193,6 +210,7 @@ __device__ __forceinline__ int wrap_local(int l, int n) {
 }
 
 // workgroup per (tile, part): the tile's J plus an A-node apron in LDS
+template <bool ONE_ROW>
 __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles t,
                                                              unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_j[];
@@ -216,11 +234,11 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
         const int li = wrap_local(m.sh.ib - 1 - orgx, a.nx), lj = wrap_local(m.sh.jb - 1 - orgy, a.ny);
         if (dev::in_tile(li, t.W - 4) && dev::in_tile(lj, t.H - 4)) {
             const int base = lj * t.W + li;
-            esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+            esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
                 atomicAdd(&s_j[q * R + base + b * t.W + k], v);
             });
         } else {
-            esk_emit(m, [&](int q, int k, int b, unsigned long long v) {
+            esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
                 global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
             });
         }
@@ -258,6 +276,15 @@ __global__ void __launch_bounds__(kThreads) esk_convert_kernel(EskArgs a,
 
 int tile_bpt(const aw_ctx& c, int64_t n);
 
+namespace {
+// the one-row (ny = 1) instance of a kernel, or the general one
+template <class K, class... Args>
+void launch_one_row(K one, K general, bool one_row, int grid, int threads, size_t sh,
+                    hipStream_t s, Args... args) {
+    hipLaunchKernelGGL(one_row ? one : general, dim3(grid), dim3(threads), sh, s, args...);
+}
+}  // namespace
+
 void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,
                        hipStream_t s, const EskExtra& ex) {
@@ -287,6 +314,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
     a.dy = c.g.dy;
     a.bound = c.jbound.get();
     unsigned long long* acc = c.jacc.get();
+    const bool one_row = c.g.ny == 1;
     const float* part = ex.bound_part;
     int npart = ex.npart;
     if (!part) {
@@ -316,7 +344,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
             t.bpt = tile_bpt(c, p.n);
             const size_t sh = (size_t)3 * t.W * t.H * 8;
             AW_REQUIRE(sh <= c.lds_per_block, "tile apron does not fit LDS");
-            esk_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, acc);
+            launch_one_row(esk_tiles_kernel<true>, esk_tiles_kernel<false>, one_row, c.ntiles * t.bpt, kThreads, sh, s, a, t, acc);
             AW_LAUNCH_CHECK();
             policy = -1;  // done
         } else if (policy == AW_DEPOSIT_AUTO) {
@@ -327,17 +355,17 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
         }
         if (policy == -1) {
         } else if (policy == AW_DEPOSIT_GLOBAL) {
-            esk_global_kernel<<<blocks_for(p.n), kThreads, 0, s>>>(a, acc);
+            launch_one_row(esk_global_kernel<true>, esk_global_kernel<false>, one_row, blocks_for(p.n), kThreads, 0, s, a, acc);
             AW_LAUNCH_CHECK();
         } else if (policy == AW_DEPOSIT_LDS_GRID) {
             const size_t sh = (size_t)(3 * nc) * 8;
             AW_REQUIRE(sh <= c.lds_per_block, "grid does not fit LDS for AW_DEPOSIT_LDS_GRID");
             int per_cu = 0;
             AW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel), kGridThreads, sh));
+                &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel<false>), kGridThreads, sh));
             const int nb = (int)std::min<int64_t>(std::max(1, per_cu) * std::max(1, c.num_cu),
                                                   (p.n + kGridThreads - 1) / kGridThreads);
-            esk_ldsgrid_kernel<<<nb, kGridThreads, sh, s>>>(a, acc);
+            launch_one_row(esk_ldsgrid_kernel<true>, esk_ldsgrid_kernel<false>, one_row, nb, kGridThreads, sh, s, a, acc);
             AW_LAUNCH_CHECK();
         } else if (policy == AW_DEPOSIT_BINNED) {
             // bin by the old position (stable), as deposit_rho_fixed does
@@ -358,7 +386,7 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
             t.H = c.ty + 2 * kApron;
             t.bpt = tile_bpt(c, p.n);
             const size_t sh = (size_t)3 * t.W * t.H * 8;
-            esk_tiles_kernel<<<c.ntiles * t.bpt, kThreads, sh, s>>>(a, t, acc);
+            launch_one_row(esk_tiles_kernel<true>, esk_tiles_kernel<false>, one_row, c.ntiles * t.bpt, kThreads, sh, s, a, t, acc);
             AW_LAUNCH_CHECK();
         } else {
             fail(AW_ERR_INVALID, "unknown deposit policy " + std::to_string(policy), __FILE__,

@@ -60,10 +60,8 @@ __device__ __forceinline__ int at(const Mesh& m, int i, int j) {
     return wrap1(j, m.ny) * m.nx + wrap1(i, m.nx);
 }
 
-// B += h * (-curl E)  (Faraday, half step h = dt/2)
-__global__ void __launch_bounds__(kThreads) faraday_kernel(Mesh m, float h) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
+// B += h * (-curl E)  (Faraday, half step h = dt/2), one cell
+__device__ __forceinline__ void faraday_cell(const Mesh& m, float h, int c) {
     const int i = c % m.nx, j = c / m.nx;
     const float ez = m.ez[c];
     const float dez_y = (m.ez[at(m, i, j + 1)] - ez) / m.dy;
@@ -75,10 +73,8 @@ __global__ void __launch_bounds__(kThreads) faraday_kernel(Mesh m, float h) {
     m.bz[c] = m.bz[c] - h * (dey_x - dex_y);
 }
 
-// E += dt * (c^2 curl B - J)  (Ampere, eps0 = 1)
-__global__ void __launch_bounds__(kThreads) ampere_kernel(Mesh m) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
+// E += dt * (c^2 curl B - J)  (Ampere, eps0 = 1), one cell
+__device__ __forceinline__ void ampere_cell(const Mesh& m, int c) {
     const int i = c % m.nx, j = c / m.nx;
     const float bz = m.bz[c];
     const float dbz_y = (bz - m.bz[at(m, i, j - 1)]) / m.dy;
@@ -88,6 +84,22 @@ __global__ void __launch_bounds__(kThreads) ampere_kernel(Mesh m) {
     m.ex[c] = m.ex[c] + m.dt * (m.c2 * dbz_y - m.jx[c]);
     m.ey[c] = m.ey[c] + m.dt * (-m.c2 * dbz_x - m.jy[c]);
     m.ez[c] = m.ez[c] + m.dt * (m.c2 * (dby_x - dbx_y) - m.jz[c]);
+}
+
+// Faraday half step (include/pic/yee2d.hpp:70-81); zero_j also clears the
+// current for the step's deposit
+__global__ void __launch_bounds__(kThreads) faraday_kernel(Mesh m, float h, int zero_j) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    faraday_cell(m, h, c);
+    if (zero_j) m.jx[c] = m.jy[c] = m.jz[c] = 0.f;
+}
+
+// Ampere step (include/pic/yee2d.hpp:84-97)
+__global__ void __launch_bounds__(kThreads) ampere_kernel(Mesh m) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    ampere_cell(m, c);
 }
 
 // u' = u + u x t, u += u' x s with t = h B, s = 2t/(1 + t^2)
@@ -486,10 +498,9 @@ binomial_lds_kernel(float* __restrict__ a, Mesh m, int passes) {
     for (int c = threadIdx.x; c < nc; c += kSmoothThreads) g[c] = cur[c];
 }
 
-// the rotating transverse antenna current (include/pic/yee2d.hpp:582-607)
-__global__ void antenna_kernel(Mesh m, aw_run_params rp, double t) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
+// the rotating transverse antenna current (include/pic/yee2d.hpp:582-607), one cell
+__device__ __forceinline__ void antenna_cell(const Mesh& m, const aw_run_params& rp, double t,
+                                             int c) {
     const int i = c % m.nx;
     const double d = i - rp.ant_x0;
     if (fabs(d) > 4.0 * rp.ant_sigma) return;
@@ -506,11 +517,16 @@ __global__ void antenna_kernel(Mesh m, aw_run_params rp, double t) {
     m.jz[c] += (float)(g * sin(ph));
 }
 
-// linearised cold electron fluid on the nodes, ny = 1 (include/pic/yee2d.hpp:554-578)
-__global__ void cold_fluid_kernel(Mesh m, aw_run_params rp, float* __restrict__ vcy,
-                                  float* __restrict__ vcz) {
+__global__ void antenna_kernel(Mesh m, aw_run_params rp, double t) {
     const int c = blockIdx.x * kThreads + threadIdx.x;
     if (c >= m.nx * m.ny) return;
+    antenna_cell(m, rp, t, c);
+}
+
+// linearised cold electron fluid on the nodes, ny = 1 (include/pic/yee2d.hpp:554-578), one cell
+__device__ __forceinline__ void cold_cell(const Mesh& m, const aw_run_params& rp,
+                                          float* __restrict__ vcy, float* __restrict__ vcz,
+                                          int c) {
     const int i = c % m.nx;
     const float Ey = m.ey[c], Ez = m.ez[c];
     const float h = (float)(0.5 * rp.dt * rp.qm);
@@ -530,12 +546,18 @@ __global__ void cold_fluid_kernel(Mesh m, aw_run_params rp, float* __restrict__ 
     m.jz[c] += jc * uz;
 }
 
-// x damping layers: node mask on Ey, Ez, Bx (and the cold fluid), half-node
-// mask on Ex, By, Bz (include/pic/yee2d.hpp:531-545)
-__global__ void damp_kernel(Mesh m, const float* __restrict__ mn, const float* __restrict__ mh,
-                            float* vcy, float* vcz) {
+__global__ void cold_fluid_kernel(Mesh m, aw_run_params rp, float* __restrict__ vcy,
+                                  float* __restrict__ vcz) {
     const int c = blockIdx.x * kThreads + threadIdx.x;
     if (c >= m.nx * m.ny) return;
+    cold_cell(m, rp, vcy, vcz, c);
+}
+
+// x damping layers: node mask on Ey, Ez, Bx (and the cold fluid), half-node
+// mask on Ex, By, Bz (include/pic/yee2d.hpp:531-545), one cell
+__device__ __forceinline__ void damp_cell(const Mesh& m, const float* __restrict__ mn,
+                                          const float* __restrict__ mh, float* vcy, float* vcz,
+                                          int c) {
     const int i = c % m.nx;
     const float a = mn[i], b = mh[i];
     if (a == 1.f && b == 1.f) return;
@@ -549,6 +571,13 @@ __global__ void damp_kernel(Mesh m, const float* __restrict__ mn, const float* _
         vcy[c] *= a;
         vcz[c] *= a;
     }
+}
+
+__global__ void damp_kernel(Mesh m, const float* __restrict__ mn, const float* __restrict__ mh,
+                            float* vcy, float* vcz) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    damp_cell(m, mn, mh, vcy, vcz, c);
 }
 
 // Gauss residual at the nodes: div E - rho (include/pic/yee2d.hpp:614-623)
@@ -923,7 +952,8 @@ struct aw_maxwell {
         const Mesh m = mesh();
         const int nb = blocks_for(nc());
         const float h = 0.5f * (float)rp.dt;
-        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h);
+        // the first half step also clears J for the particles' current
+        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h, p.n > 0 ? 1 : 0);
         AW_LAUNCH_CHECK();
         if (p.n > 0) {
             AW_REQUIRE(!rp.deltaf || (df && df->wd), "delta-f run without a weight array");
@@ -938,7 +968,6 @@ struct aw_maxwell {
             const bool own_vz = rp.rel || rp.bnd_x;
             if (own_vz) vz.reserve(n);
             if (rp.deltaf) wdep.reserve(n);
-            zero(m.jx, 3 * nc());
             PushArgs a{};
             a.x = p.x;
             a.y = p.y;
@@ -999,24 +1028,26 @@ struct aw_maxwell {
                               m.jx, m.jy, m.jz, stream, ex);
             smooth(m.jx, 3);
         }
-        if (rp.ant_amp != 0.0) {
-            if (p.n == 0) zero(m.jx, 3 * nc());
+        // without particles the antenna and the cold fluid start from J = 0
+        const bool ant = rp.ant_amp != 0.0, cold = rp.cold_nc > 0.0;
+        const bool clear = p.n == 0 && (ant || cold);
+        float* vcy = cold ? vc.get() : nullptr;
+        float* vcz = cold ? vc.get() + nc() : nullptr;
+        if (clear) zero(m.jx, 3 * nc());
+        if (ant) {
             antenna_kernel<<<nb, kThreads, 0, stream>>>(m, rp, tnow);
             AW_LAUNCH_CHECK();
         }
-        if (rp.cold_nc > 0.0) {
-            if (p.n == 0 && rp.ant_amp == 0.0) zero(m.jx, 3 * nc());
-            cold_fluid_kernel<<<nb, kThreads, 0, stream>>>(m, rp, vc.get(), vc.get() + nc());
+        if (cold) {
+            cold_fluid_kernel<<<nb, kThreads, 0, stream>>>(m, rp, vcy, vcz);
             AW_LAUNCH_CHECK();
         }
-        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h);
+        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h, 0);
         AW_LAUNCH_CHECK();
         ampere_kernel<<<nb, kThreads, 0, stream>>>(m);
         AW_LAUNCH_CHECK();
         if (rp.bnd_x) {
-            damp_kernel<<<nb, kThreads, 0, stream>>>(m, mask.get(), mask.get() + g.nx,
-                                                     rp.cold_nc > 0.0 ? vc.get() : nullptr,
-                                                     rp.cold_nc > 0.0 ? vc.get() + nc() : nullptr);
+            damp_kernel<<<nb, kThreads, 0, stream>>>(m, mask.get(), mask.get() + g.nx, vcy, vcz);
             AW_LAUNCH_CHECK();
         }
         ++nstep;

@@ -1016,6 +1016,94 @@ void orc_yee_ampere_f32(const orc_grid* g, float* F, float c2, float dt) {
         }
 }
 
+/* k_antenna (include/pic/yee2d.hpp:582-607): the rotating transverse current
+ * column g(i) (cos, sin)(w0 t), g Gaussian in x around x0 (cut at 4 sigma)
+ * with a trapezoidal envelope, added to Jy, Jz.  Double arithmetic as the
+ * kernel; exp, cos and sin are the host's. */
+void orc_yee_antenna(const orc_grid* g, double amp, double x0, double sigma, double w0,
+                     double trmp, double toff, double t, float* F) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    float* jy = F + 7 * nc; float* jz = F + 8 * nc;
+    if (toff > 0 && t >= toff) return;
+    double ramp = 1.0;
+    if (trmp > 0) {
+        if (t < trmp) ramp = t / trmp;
+        else if (toff > 0 && t > toff - trmp) ramp = (toff - t) / trmp;
+    }
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const double d = i - x0;
+            if (fabs(d) > 4.0 * sigma) continue;
+            const double a = amp * ramp * exp(-d * d / (2.0 * sigma * sigma));
+            const int c = j * g->nx + i;
+            jy[c] += (float)(a * cos(w0 * t));
+            jz[c] += (float)(a * sin(w0 * t));
+        }
+}
+
+/* k_cold_fluid (include/pic/yee2d.hpp:536-560): the linearised cold electron
+ * fluid on the x nodes (ny = 1): half E kick, rotation by -qm B0(x) dt, half
+ * E kick, then J += qm nc v.  FP32 as the kernel; sinf and cosf are the
+ * host's (the kernel's sincosf is the device's). */
+void orc_yee_cold_fluid_f32(const orc_grid* g, const orc_yee_params* p, double cold_nc,
+                            float* F, float* vcy, float* vcz) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    const float* ey = F + nc; const float* ez = F + 2 * nc;
+    float* jy = F + 7 * nc; float* jz = F + 8 * nc;
+    const float dx = (float)g->dx;
+    const float h = (float)(0.5 * p->dt * p->qm);
+    const float jc = (float)(p->qm * cold_nc);
+    for (int64_t c = 0; c < nc; ++c) {
+        const int i = (int)(c % g->nx);
+        const float Ey = ey[c], Ez = ez[c];
+        float uy = vcy[c] + h * Ey;
+        float uz = vcz[c] + h * Ez;
+        const float b0 = p->b0_prof ? orc_b0_profile(p, i * dx) : p->B0[0];
+        const float th = (float)(-p->qm * p->dt) * b0;
+        const float sn = sinf(th), cs = cosf(th);
+        const float uy2 = uy * cs - uz * sn;
+        const float uz2 = uy * sn + uz * cs;
+        uy = uy2 + h * Ey;
+        uz = uz2 + h * Ez;
+        vcy[c] = uy;
+        vcz[c] = uz;
+        jy[c] += jc * uy;
+        jz[c] += jc * uz;
+    }
+}
+
+/* The x damping layers: masks exp(-numax d^2 dt), d the depth into the
+ * nd-cell layer over nd, at integer x (Ey, Ez, Bx, the cold fluid) and
+ * half-integer x (Ex, By, Bz) (include/pic/simulation_maxwell.hpp:189-203),
+ * applied as k_damp_x (include/pic/yee2d.hpp:514-526).  vcy may be NULL. */
+static float damp_mask(const orc_grid* g, int nd, double numax, double dt, double xi) {
+    double d = 0.0;
+    if (xi < nd) d = (nd - xi) / (double)nd;
+    else if (xi > g->nx - nd) d = (xi - (g->nx - nd)) / (double)nd;
+    if (d <= 0.0) return 1.0f;
+    return (float)exp(-numax * d * d * dt);
+}
+
+void orc_yee_damp_f32(const orc_grid* g, int32_t nd, double numax, double dt, float* F,
+                      float* vcy, float* vcz) {
+    const int64_t nc = (int64_t)g->nx * g->ny;
+    for (int64_t c = 0; c < nc; ++c) {
+        const int i = (int)(c % g->nx);
+        const float a = damp_mask(g, nd, numax, dt, i), b = damp_mask(g, nd, numax, dt, i + 0.5);
+        if (a == 1.f && b == 1.f) continue;
+        F[nc + c] *= a;      /* ey */
+        F[2 * nc + c] *= a;  /* ez */
+        F[3 * nc + c] *= a;  /* bx */
+        F[c] *= b;           /* ex */
+        F[4 * nc + c] *= b;  /* by */
+        F[5 * nc + c] *= b;  /* bz */
+        if (vcy) {
+            vcy[c] *= a;
+            vcz[c] *= a;
+        }
+    }
+}
+
 /* one [1,2,1]x[1,2,1]/16 pass (include/pic/yee2d.hpp:494-508) */
 void orc_yee_binomial_f32(const orc_grid* g, const float* src, float* dst) {
     const float w[3] = {0.25f, 0.5f, 0.25f};

@@ -147,6 +147,12 @@ void orc_yee_faraday_f32(const orc_grid* g, float* F, float h);
 void orc_yee_ampere_f32(const orc_grid* g, float* F, float c2, float dt);
 void orc_yee_binomial_f32(const orc_grid* g, const float* src, float* dst);
 float orc_b0_profile(const orc_yee_params* p, float xph);
+void orc_yee_antenna(const orc_grid* g, double amp, double x0, double sigma, double w0,
+                     double trmp, double toff, double t, float* F);
+void orc_yee_cold_fluid_f32(const orc_grid* g, const orc_yee_params* p, double cold_nc,
+                            float* F, float* vcy, float* vcz);
+void orc_yee_damp_f32(const orc_grid* g, int32_t nd, double numax, double dt, float* F,
+                      float* vcy, float* vcz);
 void orc_yee_push_f32(const orc_grid* g, const orc_yee_params* p, double tnow, const float* F,
                       float* x, float* y, float* ux, float* uy, float* uz, const float* w,
                       float w_uniform, float* wd, float* wc, double* wdd, float* x0o,

@@ -429,6 +429,34 @@ def yee_binomial(g, a):
     return out
 
 
+def yee_antenna(g, F, amp, x0, sigma, w0, trmp, toff, t):
+    """orc_yee_antenna on a copy of the nine mesh grids F."""
+    F = _f(F).copy()
+    lib().orc_yee_antenna(C.byref(g), C.c_double(amp), C.c_double(x0), C.c_double(sigma),
+                          C.c_double(w0), C.c_double(trmp), C.c_double(toff), C.c_double(t),
+                          _p(F))
+    return F
+
+
+def yee_cold_fluid(g, yp, cold_nc, F, vcy, vcz):
+    """orc_yee_cold_fluid_f32 on copies; returns (F, vcy, vcz)."""
+    F, vcy, vcz = _f(F).copy(), _f(vcy).copy(), _f(vcz).copy()
+    lib().orc_yee_cold_fluid_f32(C.byref(g), C.byref(yp), C.c_double(cold_nc), _p(F), _p(vcy),
+                                 _p(vcz))
+    return F, vcy, vcz
+
+
+def yee_damp(g, nd, numax, dt, F, vcy=None, vcz=None):
+    """orc_yee_damp_f32 on copies; returns (F, vcy, vcz) (None without the fluid)."""
+    F = _f(F).copy()
+    if vcy is not None:
+        vcy, vcz = _f(vcy).copy(), _f(vcz).copy()
+    lib().orc_yee_damp_f32(C.byref(g), int(nd), C.c_double(numax), C.c_double(dt), _p(F),
+                           _p(vcy) if vcy is not None else None,
+                           _p(vcz) if vcz is not None else None)
+    return F, vcy, vcz
+
+
 def yee_push(g, yp, tnow, F, state, w=None, w_uniform=0.0, wd=None, wc=None, wdd=None):
     """One Yee push (orc_yee_push_f32) of copies of the state; returns
     (new state, x0, y0, vz, wdep)."""

@@ -92,11 +92,13 @@ def test_esirkepov_continuity_reference_setup(gpu, oracle):
 
 
 @pytest.mark.parametrize("nx,ny,tile,n", [(128, 96, 16, 1_000_000), (40, 24, 8, 300_000),
-                                          (16, 4, 16, 5_000)])
+                                          (16, 4, 16, 5_000), (600, 1, 16, 200_000),
+                                          (3000, 1, 16, 60_000)])
 def test_esirkepov_wrapped_positions_and_policies(gpu, oracle, nx, ny, tile, n):
     """Uniform weight; new positions wrapped with the migrate semantics, so the
     kernel must take the minimal image.  Policies agree bit for bit and with
-    the FP64 oracle on the unwrapped moves."""
+    the FP64 oracle on the same moves; the one-row grids (whose stencil
+    rows coincide) take the LDS grid (600 x 1) and the binned tiles."""
     dx, dy, dt, qm = 0.5, 0.5, 0.2, -1.0
     rng = np.random.default_rng(nx * 7 + n)
     x0, y0, x1, y1, uz, _ = _moves(rng, nx, ny, n, dx, dy, dt, uniform_w=True)
@@ -107,7 +109,16 @@ def test_esirkepov_wrapped_positions_and_policies(gpu, oracle, nx, ny, tile, n):
     yw[yw >= ny] = 0.0
     assert np.any(xw != x1)
     og = oracle.grid(nx, ny, dx, dy)
-    ref = oracle.esirkepov_f64(og, x0, y0, x1, y1, uz, qm, dt, w_uniform=wu)
+    # the oracle gets the move the kernel reconstructs from the wrapped end
+    # point (its minimal image), shifted by a period where needed so that
+    # both ends stay exact floats
+    x0o, y0o, x1o, y1o = x0.copy(), y0.copy(), xw.copy(), yw.copy()
+    for a0, a1, n_ in ((x0o, x1o, nx), (y0o, y1o, ny)):
+        up = a0 - a1 > 0.5 * n_  # the kernel takes a1 + n
+        a0[up] -= np.float32(n_)
+        dn = a1 - a0 > 0.5 * n_  # the kernel takes a1 - n
+        a1[dn] -= np.float32(n_)
+    ref = oracle.esirkepov_f64(og, x0o, y0o, x1o, y1o, uz, qm, dt, w_uniform=wu)
     g = L.grid(nx, ny, nx * dx, ny * dy)
     with Ctx(g, tile, tile) as ctx:
         outs = [_gpu_j(ctx, nx, ny, x0, y0, xw, yw, uz, None, wu, qm, dt, pol)

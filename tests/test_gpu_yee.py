@@ -462,3 +462,50 @@ def test_tile_sort_refuses_weight_precision_accumulators(gpu):
     with pytest.raises(Exception, match="tile_sort"):
         sim.step()
     sim.close()
+
+
+@pytest.mark.parametrize("nx", [256, 20000])
+def test_antenna_cold_fluid_walls_match_oracle(gpu, oracle, nx):
+    """The field side of MaxwellSimulation::step_at without particles
+    (include/pic/simulation_maxwell.hpp:52-109): B half step, the antenna
+    column (yee2d.hpp:582-607), the cold fluid with the profile's B0(x)
+    (:536-560), B half step, E step, the damping layers (:514-526), three
+    steps against the oracle.  The antenna's exp/cos/sin, the fluid's sincosf
+    and the masks' exp are libm calls, so the comparison is within FP32
+    rounding."""
+    dx = 0.25
+    g = L.grid(nx, 1, nx * dx, dx)
+    rp = L.run_params(dt=0.2, qm=-1.0, eps0=1.0, c=1.0)
+    rp.B0[0] = 0.3
+    rp.b0_prof, rp.b0_a, rp.b0_xc = 1, 1e-4, nx * dx / 2
+    rp.cold_nc, rp.bnd_x, rp.bnd_nd, rp.bnd_numax = 0.8, 2, 20, 0.4
+    rp.ant_amp, rp.ant_x0, rp.ant_sigma, rp.ant_w0 = 1e-3, nx / 3, 4.0, 0.25
+    rp.ant_trmp, rp.ant_toff = 0.3, 100.0
+    rng = np.random.default_rng(7)
+    F = rng.normal(0, 1e-3, 9 * nx).astype(np.float32)
+    F[6 * nx:] = 0.0
+    sim = MaxwellSimulation(g, rp)
+    names = ("ex", "ey", "ez", "bx", "by", "bz")
+    sim.set_mesh(**{k: F[q * nx:(q + 1) * nx] for q, k in enumerate(names)})
+    og = oracle.grid(nx, 1, dx, dx)
+    yp = _oracle_params(oracle, rp)
+    vcy = np.zeros(nx, np.float32)
+    vcz = np.zeros(nx, np.float32)
+    h = np.float32(0.5 * rp.dt)
+    for k in range(3):
+        t = k * rp.dt
+        F = oracle.yee_faraday(og, F, h)
+        F[6 * nx:] = 0.0
+        F = oracle.yee_antenna(og, F, rp.ant_amp, rp.ant_x0, rp.ant_sigma, rp.ant_w0,
+                               rp.ant_trmp, rp.ant_toff, t)
+        F, vcy, vcz = oracle.yee_cold_fluid(og, yp, rp.cold_nc, F, vcy, vcz)
+        F = oracle.yee_faraday(og, F, h)
+        F = oracle.yee_ampere(og, F, np.float32(rp.c * rp.c), np.float32(rp.dt))
+        F, vcy, vcz = oracle.yee_damp(og, rp.bnd_nd, rp.bnd_numax, rp.dt, F, vcy, vcz)
+        sim.step()
+        M = sim.mesh_flat()
+        for q in range(9):
+            a, b = M[q * nx:(q + 1) * nx], F[q * nx:(q + 1) * nx]
+            print(f"nx={nx} step {k} grid {q}: rel {rel_l2(a, b):.2e}", flush=True)
+            assert rel_l2(a, b) < 2e-6, (nx, k, q)
+    sim.close()

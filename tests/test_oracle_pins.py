@@ -987,3 +987,83 @@ def test_mirror_offsets(oracle):
     tp = 1 / ((1 - 1 / b) / 0.01 + (1 / b) / 0.04)
     assert (cnt == np.round(32 * tp / 0.04)).all()
     assert (cnt == cnt[::-1]).all() and cnt[nx // 2] == 32
+
+
+# ---- Yee antenna, cold fluid and damping layers ----------------------------------------
+
+def test_yee_antenna_closed_form(oracle):
+    """k_antenna (include/pic/yee2d.hpp:582-607): at the column centre the
+    current is amp (cos, sin)(w0 t) on the flat top, half of it halfway up and
+    down the ramps, exp(-1/2) of it one sigma away, nothing past 4 sigma or
+    after toff."""
+    g = oracle.grid(40, 3, 0.5, 0.5)
+    nc = 40 * 3
+    amp, x0, sig, w0, trmp, toff = 0.3, 10.0, 2.0, 0.7, 5.0, 50.0
+
+    def cur(t):
+        F = oracle.yee_antenna(g, np.zeros(9 * nc, np.float32), amp, x0, sig, w0, trmp, toff, t)
+        return F[7 * nc:8 * nc].reshape(3, 40), F[8 * nc:].reshape(3, 40)
+
+    for t, r in ((20.0, 1.0), (2.5, 0.5), (47.5, 0.5)):
+        jy, jz = cur(t)
+        for j in range(3):
+            assert jy[j, 10] == pytest.approx(r * amp * math.cos(w0 * t), rel=1e-6, abs=1e-9)
+            assert jz[j, 10] == pytest.approx(r * amp * math.sin(w0 * t), rel=1e-6, abs=1e-9)
+            assert jy[j, 12] == pytest.approx(r * amp * math.exp(-0.5) * math.cos(w0 * t),
+                                              rel=1e-6, abs=1e-9)
+            assert jy[j, 18] != 0.0 and jy[j, 19] == 0.0 and jy[j, 1] == 0.0
+    jy, jz = cur(50.0)
+    assert not jy.any() and not jz.any()
+
+
+def test_yee_cold_fluid_rotation_and_kick(oracle):
+    """k_cold_fluid (include/pic/yee2d.hpp:536-560): without E the fluid
+    velocity turns by -qm B0 dt a step at constant speed (B0 from the profile
+    at each node); without B0 a uniform Ey adds qm dt Ey a step; the current
+    is qm nc v."""
+    nx, dt, qm, ncold = 16, 0.1, -1.0, 0.9
+    g = oracle.grid(nx, 1, 0.5, 0.5)
+    yp = oracle.yee_params(qm=qm, dt=dt, B0=(0.4, 0.0, 0.0), b0_prof=1, b0_a=2e-3, b0_xc=4.0)
+    F = np.zeros(9 * nx, np.float32)
+    vcy = np.full(nx, 0.01, np.float32)
+    vcz = np.zeros(nx, np.float32)
+    N = 50
+    for _ in range(N):
+        F[7 * nx:] = 0.0
+        F, vcy, vcz = oracle.yee_cold_fluid(g, yp, ncold, F, vcy, vcz)
+    for i in (0, 5, 15):
+        th = -qm * dt * 0.4 * (1.0 + 2e-3 * (i * 0.5 - 4.0) ** 2)
+        assert vcy[i] == pytest.approx(0.01 * math.cos(N * th), abs=2e-8)
+        assert vcz[i] == pytest.approx(0.01 * math.sin(N * th), abs=2e-8)
+        assert F[7 * nx + i] == pytest.approx(qm * ncold * vcy[i], rel=1e-6)
+        assert F[8 * nx + i] == pytest.approx(qm * ncold * vcz[i], rel=1e-6)
+    yp0 = oracle.yee_params(qm=qm, dt=dt, B0=(0.0, 0.0, 0.0))
+    F = np.zeros(9 * nx, np.float32)
+    F[nx:2 * nx] = 0.2  # Ey
+    vcy = np.zeros(nx, np.float32)
+    vcz = np.zeros(nx, np.float32)
+    for _ in range(N):
+        F, vcy, vcz = oracle.yee_cold_fluid(g, yp0, ncold, F, vcy, vcz)
+    assert np.allclose(vcy, N * qm * dt * 0.2, rtol=1e-5) and not vcz.any()
+
+
+def test_yee_damping_masks_closed_form(oracle):
+    """The damping masks exp(-numax d^2 dt) (include/pic/simulation_maxwell.hpp:
+    189-203): d = 1 at the wall node, (nd - 1/2)/nd at the first half node,
+    zero from nd cells in; node mask on Ey, Ez, Bx and the fluid, half-node
+    mask on Ex, By, Bz (include/pic/yee2d.hpp:514-526)."""
+    nx, nd, numax, dt = 48, 6, 0.5, 0.1
+    g = oracle.grid(nx, 2, 0.5, 0.5)
+    nc = 2 * nx
+    F, vcy, vcz = oracle.yee_damp(g, nd, numax, dt, np.ones(9 * nc, np.float32),
+                                  np.ones(nc, np.float32), np.ones(nc, np.float32))
+    m = lambda d: np.float32(math.exp(-numax * d * d * dt))
+    grid = lambda q: F[q * nc:(q + 1) * nc].reshape(2, nx)
+    for q in (1, 2, 3):  # nodes
+        assert grid(q)[1, 0] == m(1.0) and grid(q)[0, 47] == m(5 / 6)
+        assert grid(q)[0, 6] == 1.0 and grid(q)[0, 42] == 1.0
+    for q in (0, 4, 5):  # half nodes
+        assert grid(q)[1, 0] == m(5.5 / 6) and grid(q)[0, 47] == m(5.5 / 6)
+        assert grid(q)[0, 5] == m(0.5 / 6) and grid(q)[0, 6] == 1.0
+    assert not (F[6 * nc:] != 1.0).any()  # J untouched
+    assert vcy[0] == m(1.0) and vcz[47] == m(5 / 6) and vcy[20] == 1.0
