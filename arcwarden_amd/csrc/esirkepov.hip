@@ -23,82 +23,18 @@
 // apron fall back to global atomics).  Each kernel can write the wrapped new
 // position back, which folds the periodic wrap of the caller's move into the
 // pass that already reads it.
-#include "aw_internal.hpp"
+#include "esk_device.hpp"
 
 namespace aw {
 
 namespace {
 
+using namespace esk;
+
 constexpr int kThreads = 256;
-constexpr int kFixBits = 44;
 constexpr int kApron = 2;        // particles binned by this step's old position
-constexpr int kSortedApron = 4;  // particles in the tile order of an earlier sort
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
-
-struct EskArgs {
-    const float *x0, *y0, *x1, *y1, *uz, *w;
-    float *xw, *yw;  // optional: x1, y1 wrapped into the box, written back
-    int n, nx, ny;
-    double qm, w_uniform, dt, dx, dy;
-    const unsigned int* bound;  // [0] max|w| (or 0), [1] max|w uz| (|uz| for uniform w)
-};
-
-// Scale exponents of the three components (identical in deposit and convert).
-__device__ __forceinline__ void esk_exponents(const EskArgs& a, int S[3]) {
-    const double wmax = a.w ? (double)__uint_as_float(a.bound[0]) : fabs(a.w_uniform);
-    const double wuz = a.w ? (double)__uint_as_float(a.bound[1])
-                           : fabs(a.w_uniform) * (double)__uint_as_float(a.bound[1]);
-    const double b[3] = {fabs(a.qm) * wmax / (a.dt * a.dy), fabs(a.qm) * wmax / (a.dt * a.dx),
-                         fabs(a.qm) * wuz / (a.dx * a.dy)};
-    for (int q = 0; q < 3; ++q) {
-        int e = 0;
-        (void)frexp(b[q], &e);  // b < 2^e
-        S[q] = kFixBits - e;
-    }
-}
-
-// One particle's move: the shared stencil geometry plus its fixed-point
-// scaled coefficients.
-struct EskMove {
-    dev::EskShape sh;
-    double cJx, cJy, cJz;
-};
-
-__device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double scale[3],
-                                            EskMove& m) {
-    dev::esk_shape(a.x0[i], a.y0[i], a.x1[i], a.y1[i], a.nx, a.ny, m.sh);
-    const double qw = a.qm * (a.w ? (double)a.w[i] : a.w_uniform);
-    m.cJx = -qw / (a.dt * a.dy) * scale[0];
-    m.cJy = -qw / (a.dt * a.dx) * scale[1];
-    m.cJz = qw * (double)a.uz[i] / (a.dx * a.dy) * scale[2];
-}
-
-// The contributions of one move.  On a one-row grid (ny = 1) the three
-// stencil rows are the same row: each (component, column) sum of the
-// fixed-point contributions is formed in registers (exact, int64) and added
-// once, at row 1, instead of once per row.
-template <bool ONE_ROW, class Add>
-__device__ __forceinline__ void esk_emit(const EskMove& m, Add&& add) {
-    if constexpr (!ONE_ROW) {
-        dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz, add);
-        return;
-    }
-    unsigned long long col[3][4] = {};
-    dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz,
-                  [&](int q, int k, int, unsigned long long v) { col[q][k] += v; });
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int k = 1; k < 4; ++k)
-            if (col[q][k]) add(q, k, 1, col[q][k]);
-}
-
-__device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
-    int S[3];
-    esk_exponents(a, S);
-    for (int q = 0; q < 3; ++q) scale[q] = ldexp(1.0, S[q]);
-}
 
 // per-workgroup max |w| and max |w uz| (max |uz| for a uniform weight) into
 // part[2 b], part[2 b + 1]
@@ -121,13 +57,13 @@ __global__ void __launch_bounds__(kThreads) esk_bound_kernel(EskArgs a, float* _
     }
 }
 
-// bound[0..1] = the maxima over npart partial pairs, as positive-float bit
-// patterns
+// bound[0..1] = the maxima of (f0, f1) and the npart partial pairs, as
+// positive-float bit patterns
 constexpr int kFinalThreads = 1024;
 __global__ void __launch_bounds__(kFinalThreads) esk_bound_final_kernel(const float* __restrict__ part,
-                                                                   int npart,
+                                                                   int npart, float f0, float f1,
                                                                    unsigned int* __restrict__ bound) {
-    float mw = 0.f, mwu = 0.f;
+    float mw = f0, mwu = f1;
     for (int b = threadIdx.x; b < npart; b += blockDim.x) {
         mw = fmaxf(mw, part[2 * b]);
         mwu = fmaxf(mwu, part[2 * b + 1]);
@@ -202,12 +138,6 @@ struct EskTiles {
     const int32_t* idx;  // bin permutation (NULL: the particles are in tile order)
     int ntx, tx, ty, A, W, H, bpt;  // A: apron nodes on each side, W = tx + 2A, H = ty + 2A
 };
-
-__device__ __forceinline__ int wrap_local(int l, int n) {
-    if (l < 0) l += n;
-    else if (l >= n) l -= n;
-    return l;
-}
 
 // workgroup per (tile, part): the tile's J plus an A-node apron in LDS
 template <bool ONE_ROW>
@@ -289,45 +219,10 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,
                        hipStream_t s, const EskExtra& ex) {
     AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
-    AW_REQUIRE(dt > 0, "dt must be positive");
-    AW_REQUIRE(!ex.xw == !ex.yw, "wrapped x and y outputs go together");
+    EskArgs a = esk::setup(c, x0, y0, p, qm, dt, s, ex);
     const int64_t nc = ncell(c);
-    c.jacc.reserve((size_t)(3 * nc));
-    AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
-    c.jbound.reserve(2);
-    EskArgs a{};
-    a.x0 = x0;
-    a.y0 = y0;
-    a.x1 = p.x;
-    a.y1 = p.y;
-    a.uz = p.uz;
-    a.w = p.w;
-    a.xw = ex.xw;
-    a.yw = ex.yw;
-    a.n = (int)p.n;
-    a.nx = c.g.nx;
-    a.ny = c.g.ny;
-    a.qm = qm;
-    a.w_uniform = p.w_uniform;
-    a.dt = dt;
-    a.dx = c.g.dx;
-    a.dy = c.g.dy;
-    a.bound = c.jbound.get();
     unsigned long long* acc = c.jacc.get();
     const bool one_row = c.g.ny == 1;
-    const float* part = ex.bound_part;
-    int npart = ex.npart;
-    if (!part) {
-        npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
-        c.jpart.reserve((size_t)std::max(2, 2 * npart));
-        if (npart > 0) {
-            esk_bound_kernel<<<npart, kThreads, 0, s>>>(a, c.jpart.get());
-            AW_LAUNCH_CHECK();
-        }
-        part = c.jpart.get();
-    }
-    esk_bound_final_kernel<<<1, kFinalThreads, 0, s>>>(part, npart, c.jbound.get());
-    AW_LAUNCH_CHECK();
     if (p.n > 0) {
         if (ex.slice_off) {
             AW_REQUIRE(policy == AW_DEPOSIT_AUTO || policy == AW_DEPOSIT_BINNED,
@@ -393,8 +288,67 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                  __LINE__);
         }
     }
-    esk_convert_kernel<<<blocks_for(nc), kThreads, 0, s>>>(a, acc, jx, jy, jz);
+    esk::convert(c, a, jx, jy, jz, s);
+}
+
+namespace esk {
+
+EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
+              double dt, hipStream_t s, const EskExtra& ex) {
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    AW_REQUIRE(dt > 0, "dt must be positive");
+    AW_REQUIRE(!ex.xw == !ex.yw, "wrapped x and y outputs go together");
+    const int64_t nc = ncell(c);
+    c.jacc.reserve((size_t)(3 * nc));
+    AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
+    c.jbound.reserve(2);
+    EskArgs a{};
+    a.x0 = x0;
+    a.y0 = y0;
+    a.x1 = p.x;
+    a.y1 = p.y;
+    a.uz = p.uz;
+    a.w = p.w;
+    a.per_w = ex.fixed ? ex.fixed_per_w : (p.w != nullptr);
+    a.xw = ex.xw;
+    a.yw = ex.yw;
+    a.n = (int)p.n;
+    a.nx = c.g.nx;
+    a.ny = c.g.ny;
+    a.qm = qm;
+    a.w_uniform = p.w_uniform;
+    a.dt = dt;
+    a.dx = c.g.dx;
+    a.dy = c.g.dy;
+    a.bound = c.jbound.get();
+    const float* part = ex.bound_part;
+    int npart = ex.npart;
+    if (ex.fixed) {
+        AW_REQUIRE(ex.fixed_bound[0] >= 0.f && ex.fixed_bound[1] >= 0.f &&
+                       std::isfinite(ex.fixed_bound[0]) && std::isfinite(ex.fixed_bound[1]),
+                   "the current's fixed scale bounds must be finite and non-negative");
+        part = nullptr;
+        npart = 0;
+    } else if (!part) {
+        npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
+        c.jpart.reserve((size_t)std::max(2, 2 * npart));
+        if (npart > 0) {
+            esk_bound_kernel<<<npart, kThreads, 0, s>>>(a, c.jpart.get());
+            AW_LAUNCH_CHECK();
+        }
+        part = c.jpart.get();
+    }
+    esk_bound_final_kernel<<<1, kFinalThreads, 0, s>>>(part, npart, ex.fixed_bound[0],
+                                                       ex.fixed_bound[1], c.jbound.get());
+    AW_LAUNCH_CHECK();
+    return a;
+}
+
+void convert(aw_ctx& c, const EskArgs& a, float* jx, float* jy, float* jz, hipStream_t s) {
+    esk_convert_kernel<<<blocks_for(ncell(c)), kThreads, 0, s>>>(a, c.jacc.get(), jx, jy, jz);
     AW_LAUNCH_CHECK();
 }
+
+}  // namespace esk
 
 }  // namespace aw

@@ -1,0 +1,103 @@
+// esk_device.hpp -- the Esirkepov current's per-particle device code and
+// its host set-up, shared by the deposit kernels (esirkepov.hip) and by
+// kernels that deposit the current of the moves they make (yee.hip).
+//
+// Reference: s1_shape4, esirkepov_scatter (include/pic/yee2d.hpp:124-132,
+// 310-389).  Contributions are int64 fixed point with one scale exponent per
+// component, taken from a bound on the largest single contribution.
+#pragma once
+#include "aw_internal.hpp"
+
+namespace aw::esk {
+
+constexpr int kFixBits = 44;
+constexpr int kSortedApron = 4;  // particles in the tile order of an earlier sort
+
+struct EskArgs {
+    const float *x0, *y0, *x1, *y1, *uz, *w;
+    float *xw, *yw;  // optional: x1, y1 wrapped into the box, written back
+    int n, nx, ny;
+    int per_w;       // per-particle deposit weights (w, or the caller's own)
+    double qm, w_uniform, dt, dx, dy;
+    const unsigned int* bound;  // [0] max|w| (or 0), [1] max|w uz| (|uz| for uniform w)
+};
+
+// Scale exponents of the three components (identical in deposit and convert).
+__device__ __forceinline__ void esk_exponents(const EskArgs& a, int S[3]) {
+    const double wmax = a.per_w ? (double)__uint_as_float(a.bound[0]) : fabs(a.w_uniform);
+    const double wuz = a.per_w ? (double)__uint_as_float(a.bound[1])
+                               : fabs(a.w_uniform) * (double)__uint_as_float(a.bound[1]);
+    const double b[3] = {fabs(a.qm) * wmax / (a.dt * a.dy), fabs(a.qm) * wmax / (a.dt * a.dx),
+                         fabs(a.qm) * wuz / (a.dx * a.dy)};
+    for (int q = 0; q < 3; ++q) {
+        int e = 0;
+        (void)frexp(b[q], &e);  // b < 2^e
+        S[q] = kFixBits - e;
+    }
+}
+
+__device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
+    int S[3];
+    esk_exponents(a, S);
+    for (int q = 0; q < 3; ++q) scale[q] = ldexp(1.0, S[q]);
+}
+
+// One particle's move: the shared stencil geometry plus its fixed-point
+// scaled coefficients.
+struct EskMove {
+    dev::EskShape sh;
+    double cJx, cJy, cJz;
+};
+
+// the move (x0, y0) -> (x1, y1) with deposit weight w and velocity uz
+__device__ __forceinline__ void esk_prepare_v(const EskArgs& a, float x0, float y0, float x1,
+                                              float y1, float uz, double w,
+                                              const double scale[3], EskMove& m) {
+    dev::esk_shape(x0, y0, x1, y1, a.nx, a.ny, m.sh);
+    const double qw = a.qm * w;
+    m.cJx = -qw / (a.dt * a.dy) * scale[0];
+    m.cJy = -qw / (a.dt * a.dx) * scale[1];
+    m.cJz = qw * (double)uz / (a.dx * a.dy) * scale[2];
+}
+
+__device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double scale[3],
+                                            EskMove& m) {
+    esk_prepare_v(a, a.x0[i], a.y0[i], a.x1[i], a.y1[i], a.uz[i],
+                  a.w ? (double)a.w[i] : a.w_uniform, scale, m);
+}
+
+// The contributions of one move.  On a one-row grid (ny = 1) the three
+// stencil rows are the same row: each (component, column) sum of the
+// fixed-point contributions is formed in registers (exact, int64) and added
+// once, at row 1, instead of once per row.
+template <bool ONE_ROW, class Add>
+__device__ __forceinline__ void esk_emit(const EskMove& m, Add&& add) {
+    if constexpr (!ONE_ROW) {
+        dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz, add);
+    } else {
+        unsigned long long col[3][4] = {};
+        dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz,
+                      [&](int q, int k, int, unsigned long long v) { col[q][k] += v; });
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 1; k < 4; ++k)
+                if (col[q][k]) add(q, k, 1, col[q][k]);
+    }
+}
+
+__device__ __forceinline__ int wrap_local(int l, int n) {
+    if (l < 0) l += n;
+    else if (l >= n) l -= n;
+    return l;
+}
+
+// Host side.  setup clears the accumulators, fixes the scale bound (from a
+// bound pass, the caller's per-workgroup maxima, or the caller's fixed
+// values) and returns the kernels' arguments; convert adds the accumulated
+// current to jx, jy, jz.
+EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
+              double dt, hipStream_t s, const EskExtra& ex);
+void convert(aw_ctx& c, const EskArgs& a, float* jx, float* jy, float* jz, hipStream_t s);
+
+}  // namespace aw::esk

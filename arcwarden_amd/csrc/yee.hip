@@ -26,7 +26,7 @@
 // this engine never changes.  Every FP32 field and push operation is written
 // out (the build disables contraction; fused multiply-adds are explicit), and
 // the oracle restates the same sequence.
-#include "aw_internal.hpp"
+#include "esk_device.hpp"
 
 #include <cmath>
 #include <memory>
@@ -160,9 +160,19 @@ struct PushArgs {
     float *x0, *y0;      // out: positions before the move
     float* vz;           // out: deposit velocity uz / gamma (rel), else NULL
     float* wdep;         // out: deposit weight w * wd (delta-f), else NULL
-    float* bpart;        // out: per-workgroup (max |w|, max |w uz|) of the deposit, or NULL
+    // the current's fixed scale holds while |uz/gamma| <= ucap and |w| <= wmax;
+    // a particle past either sets *guard
+    float ucap, wmax;
+    unsigned int* guard;
     int n;
     double tnow;
+};
+
+// what the current deposit needs of one pushed particle
+struct PushOut {
+    float x0, y0, x1, y1;  // the move (x1 before the periodic wrap)
+    float uz;              // deposit velocity: uz / gamma (rel, walls) or uz
+    float w;               // deposit weight (w wd for delta-f, else w)
 };
 
 // yee_advance_particle (include/pic/yee2d.hpp:145-302): staggered gather of
@@ -252,11 +262,11 @@ struct WindowGather {
     }
 };
 
-// One particle; (bw, bwu) take the max of |w| and |w uz| of what the current
-// deposit reads (max |uz| for a uniform weight).
+// One particle: the push updates u (and the delta-f weight) in place and
+// returns the move; the caller stores the positions.
 template <class Gather>
 __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const aw_run_params& rp,
-                                         int t, float& bw, float& bwu, const Gather& gat) {
+                                         int t, PushOut& o, const Gather& gat) {
     const float x0 = a.x[t], y0 = a.y[t];
     float Ex, Ey, dBx, dBy;
     gat.two(0, x0, y0, Ex, dBy);
@@ -372,55 +382,68 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     a.ux[t] = uxs;
     a.uy[t] = uys;
     a.uz[t] = uzs;
-    const float uzd = a.vz ? uz * gnew : uzs;
-    if (a.vz) a.vz[t] = uzd;
-    a.x0[t] = x0;
-    a.y0[t] = y0;
-    a.x[t] = x1;
-    a.y[t] = y1;
-    // the same float operations as the deposit's own bound pass
-    const float uzb = fabsf(uzd);
-    if (rp.deltaf || a.w) {
-        const float wb = fabsf(rp.deltaf ? wdv : a.w[t]);
-        bw = fmaxf(bw, wb);
-        bwu = fmaxf(bwu, wb * uzb);
-    } else {
-        bwu = fmaxf(bwu, uzb);
-    }
+    // the current's uz/gamma is taken before the walls damp u
+    const float uzd = (rp.rel || rp.bnd_x) ? uz * gnew : uzs;
+    const float wp = a.w ? a.w[t] : a.w_uniform;
+    if (!(fabsf(uzd) <= a.ucap) || (a.w && !(fabsf(wp) <= a.wmax))) atomicOr(a.guard, 1u);
+    o.x0 = x0;
+    o.y0 = y0;
+    o.x1 = x1;
+    o.y1 = y1;
+    o.uz = uzd;
+    o.w = rp.deltaf ? wdv : wp;
 }
 
 // MaxwellSimulation's particle push (include/pic/yee2d.hpp:140-300), grid-
-// stride, with the per-workgroup bound of the current deposit
-__global__ void __launch_bounds__(kThreads) yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
-    float bw = 0.f, bwu = 0.f;
+// stride; stores the move for the current deposit that follows.  4 waves per
+// SIMD fit without spills.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
     const MeshGather gat{m};
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < a.n;
-         t += (int64_t)kThreads * gridDim.x)
-        push_one(a, m, rp, (int)t, bw, bwu, gat);
-    if (a.bpart) {
-        dev::block_max2(bw, bwu);
-        if (threadIdx.x == 0) {
-            a.bpart[2 * blockIdx.x] = bw;
-            a.bpart[2 * blockIdx.x + 1] = bwu;
-        }
+         t += (int64_t)kThreads * gridDim.x) {
+        PushOut o;
+        push_one(a, m, rp, (int)t, o, gat);
+        a.x0[t] = o.x0;
+        a.y0[t] = o.y0;
+        a.x[t] = o.x1;
+        a.y[t] = o.y1;
+        if (a.vz) a.vz[t] = o.uz;
+        if (a.wdep) a.wdep[t] = o.w;
     }
 }
 
 // Particles in the tile order of an earlier sort: a workgroup per (tile,
-// part) pushes its part of the tile's slice, gathering from an LDS copy of
-// the tile's fields with a D-cell apron for the drift since the sort.
+// part) pushes its part of the tile's slice and deposits the Esirkepov
+// current of each move on the spot (k_push_esirkepov_tiled,
+// include/pic/yee2d.hpp:401-469).  The six fields sit in LDS over the tile
+// with a D-cell apron for the drift since the sort (the mesh for what leaves
+// it); the current goes to an LDS copy of the tile with a 4-node apron
+// (global atomics for what leaves it), flushed at the end.  The move never
+// touches HBM: the positions are stored wrapped, as the migrate does.
 struct PushTiles {
     const int32_t* off;  // ntiles + 1
-    int ntx, tx, ty, D, W, H, bpt;  // W = tx + 2 D + 2, H = ty + 2 D + 2
+    int ntx, tx, ty, D, W, H, bpt;  // field window: W = tx + 2 D + 2, H = ty + 2 D + 2
+    int JW, JH;                     // current window: tx + 2 A, ty + 2 A
 };
 
-__global__ void __launch_bounds__(kThreads)
-yee_push_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t) {
-    extern __shared__ float s_f[];
+// Register budget: 4 waves per SIMD (128 VGPRs, a few spilled).  The
+// compiler's own choice, 152 VGPRs (3 waves), measured 3.81 ms on the
+// 156 M-particle eaw_case7 step against 3.32 ms here; 5 waves (96 VGPRs, 39
+// spilled) measured 4.08 ms.
+template <bool ONE_ROW>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk::EskArgs e,
+                          unsigned long long* __restrict__ acc) {
+    extern __shared__ unsigned long long s_mem[];
+    const int RJ = t.JW * t.JH, R = t.W * t.H;
+    unsigned long long* s_j = s_mem;                       // [3][JH][JW]
+    float* s_f = reinterpret_cast<float*>(s_mem + 3 * RJ);  // field planes
     const int lb = dev::xcd_block(blockIdx.x, gridDim.x);
     const int tile = lb / t.bpt, part = lb % t.bpt;
-    const int orgx = (tile % t.ntx) * t.tx - t.D - 1, orgy = (tile / t.ntx) * t.ty - t.D - 1;
-    const int R = t.W * t.H;
+    const int tx0 = (tile % t.ntx) * t.tx, ty0 = (tile / t.ntx) * t.ty;
+    const int orgx = tx0 - t.D - 1, orgy = ty0 - t.D - 1;
+    const int jorgx = tx0 - esk::kSortedApron, jorgy = ty0 - esk::kSortedApron;
     float2* s2 = reinterpret_cast<float2*>(s_f);
     for (int l = threadIdx.x; l < R; l += kThreads) {
         const int gx = dev::wrapfar(orgx + l % t.W, m.nx), gy = dev::wrapfar(orgy + l / t.W, m.ny);
@@ -430,18 +453,59 @@ yee_push_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t) {
         s_f[4 * R + l] = m.ez[g];
         s_f[5 * R + l] = m.bz[g];
     }
+    for (int c = threadIdx.x; c < 3 * RJ; c += kThreads) s_j[c] = 0ull;
     __syncthreads();
     const WindowGather gat{s_f, m, orgx, orgy, t.W, t.H};
+    double scale[3];
+    esk::esk_scales(e, scale);
+    const int64_t nc = (int64_t)m.nx * m.ny;
     const int beg = t.off[tile], end = t.off[tile + 1];
     const int chunk = (end - beg + t.bpt - 1) / t.bpt;
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
-    float bw = 0.f, bwu = 0.f;
-    for (int r = r0 + (int)threadIdx.x; r < r1; r += kThreads) push_one(a, m, rp, r, bw, bwu, gat);
-    dev::block_max2(bw, bwu);
-    if (threadIdx.x == 0) {
-        a.bpart[2 * blockIdx.x] = bw;
-        a.bpart[2 * blockIdx.x + 1] = bwu;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += kThreads) {
+        PushOut o;
+        push_one(a, m, rp, r, o, gat);
+        a.x[r] = dev::wrap_coord(o.x1, (float)m.nx);
+        a.y[r] = dev::wrap_coord(o.y1, (float)m.ny);
+        esk::EskMove mv;
+        esk::esk_prepare_v(e, o.x0, o.y0, o.x1, o.y1, o.uz, e.per_w ? (double)o.w : e.w_uniform,
+                           scale, mv);
+        const int li = esk::wrap_local(mv.sh.ib - 1 - jorgx, m.nx);
+        const int lj = esk::wrap_local(mv.sh.jb - 1 - jorgy, m.ny);
+        if (dev::in_tile(li, t.JW - 4) && dev::in_tile(lj, t.JH - 4)) {
+            const int base = lj * t.JW + li;
+            esk::esk_emit<ONE_ROW>(mv, [&](int q, int k, int b, unsigned long long v) {
+                atomicAdd(&s_j[q * RJ + base + b * t.JW + k], v);
+            });
+        } else {
+            esk::esk_emit<ONE_ROW>(mv, [&](int q, int k, int b, unsigned long long v) {
+                atomicAdd(&acc[q * nc + (int64_t)dev::wrapfar(mv.sh.jb - 1 + b, m.ny) * m.nx +
+                               dev::wrapfar(mv.sh.ib - 1 + k, m.nx)],
+                          v);
+            });
+        }
     }
+    __syncthreads();
+    for (int c = threadIdx.x; c < 3 * RJ; c += kThreads) {
+        const unsigned long long v = s_j[c];
+        if (v) {
+            const int q = c / RJ, l = c % RJ;
+            const int gx = dev::wrapfar(jorgx + l % t.JW, m.nx);
+            const int gy = dev::wrapfar(jorgy + l / t.JW, m.ny);
+            atomicAdd(&acc[q * nc + (int64_t)gy * m.nx + gx], v);
+        }
+    }
+}
+
+// per-workgroup max |a[i]|
+__global__ void __launch_bounds__(kThreads) max_abs_kernel(const float* __restrict__ a, int64_t n,
+                                                           float* __restrict__ part) {
+    float v = 0.f, u = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (int64_t)kThreads * gridDim.x)
+        v = fmaxf(v, fabsf(a[i]));
+    dev::block_max2(v, u);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
 __global__ void gather_kernel(const float* __restrict__ src, const int32_t* __restrict__ perm,
@@ -806,7 +870,11 @@ struct aw_maxwell {
     DevBuf<float> mask;           // [node | half-node] x damping masks
     DevBuf<float> vc;             // [vcy | vcz] cold fluid
     DevBuf<float> x0, y0, vz, wdep;
-    DevBuf<float> bpart;          // the push's per-workgroup deposit bounds
+    DevBuf<unsigned int> jguard;  // set when a particle leaves the current's fixed scale
+    DevBuf<float> wpart;          // per-workgroup max |w|
+    float wmax_val = 0.f;         // max |w| of the weights at (wmax_ptr, wmax_n)
+    const float* wmax_ptr = nullptr;
+    int64_t wmax_n = -1;
     DevBuf<float> rho, rres, r0, dfields;
     DevBuf<double> part;
     bool have_ref = false;
@@ -901,6 +969,43 @@ struct aw_maxwell {
         }
     }
 
+    // max |w| over the particle weights (|w_uniform| without them), taken
+    // once per weight array; a later change of a weight past it sets the guard
+    float weight_max(const aw_particles& p) {
+        if (!p.w) return std::fabs((float)p.w_uniform);
+        if (p.w != wmax_ptr || p.n != wmax_n) {
+            const int nbk = std::min(blocks_for(p.n), kRedBlocks);
+            wpart.reserve((size_t)nbk);
+            max_abs_kernel<<<nbk, kThreads, 0, stream>>>(p.w, p.n, wpart.get());
+            AW_LAUNCH_CHECK();
+            std::vector<float> h((size_t)nbk);
+            AW_HIP(hipMemcpyAsync(h.data(), wpart.get(), h.size() * sizeof(float),
+                                  hipMemcpyDeviceToHost, stream));
+            AW_HIP(hipStreamSynchronize(stream));
+            float v = 0.f;
+            for (float x : h) v = std::max(v, x);
+            AW_REQUIRE(std::isfinite(v), "a particle weight is not finite");
+            wmax_val = v;
+            wmax_ptr = p.w;
+            wmax_n = p.n;
+        }
+        return wmax_val;
+    }
+
+    // Reports (once) a step whose current left its fixed scale.
+    void check_guard() {
+        unsigned int f = 0;
+        AW_HIP(hipStreamSynchronize(stream));
+        AW_HIP(hipMemcpy(&f, jguard.get(), sizeof(f), hipMemcpyDeviceToHost));
+        if (f) {
+            AW_HIP(hipMemset(jguard.get(), 0, sizeof(unsigned int)));
+            fail(AW_ERR_INVALID,
+                 "a particle's |uz/gamma| exceeded 2 max(1, c), or a weight exceeded the "
+                 "run's largest: the current of that step left its fixed-point scale",
+                 __FILE__, __LINE__);
+        }
+    }
+
     // Stable physical sort of the particles into tile order, in place in the
     // caller's arrays (Particles::sort_by_tile, include/pic/particles.hpp:677-726)
     void tile_sort(aw_particles& p, const aw_deltaf_state* df) {
@@ -962,12 +1067,15 @@ struct aw_maxwell {
             AW_REQUIRE(!rp.deltaf || rp.df_wprec != 2 || df->wdd,
                        "df_wprec = 2 needs the FP64 weight array");
             const size_t n = (size_t)p.n;
-            x0.reserve(n);
-            y0.reserve(n);
             // the current's uz/gamma is taken before the walls damp u (bnd_x = 2)
             const bool own_vz = rp.rel || rp.bnd_x;
-            if (own_vz) vz.reserve(n);
-            if (rp.deltaf) wdep.reserve(n);
+            const bool fused = rp.tile_sort > 0;
+            if (!fused) {
+                x0.reserve(n);
+                y0.reserve(n);
+                if (own_vz) vz.reserve(n);
+                if (rp.deltaf) wdep.reserve(n);
+            }
             PushArgs a{};
             a.x = p.x;
             a.y = p.y;
@@ -979,22 +1087,34 @@ struct aw_maxwell {
             a.wd = rp.deltaf ? df->wd : nullptr;
             a.wc = rp.deltaf ? df->wc : nullptr;
             a.wdd = rp.deltaf ? df->wdd : nullptr;
-            a.x0 = x0.get();
-            a.y0 = y0.get();
-            a.vz = own_vz ? vz.get() : nullptr;
-            a.wdep = rp.deltaf ? wdep.get() : nullptr;
+            a.x0 = fused ? nullptr : x0.get();
+            a.y0 = fused ? nullptr : y0.get();
+            a.vz = !fused && own_vz ? vz.get() : nullptr;
+            a.wdep = !fused && rp.deltaf ? wdep.get() : nullptr;
             a.n = (int)p.n;
             a.tnow = tnow;
+            // The current's fixed-point scale comes from bounds fixed before the
+            // push: |uz/gamma| <= 2 max(1, c) (below c for the relativistic
+            // push), |w| <= the run's largest weight, and |w wd| <= 8 |w|
+            // (wd is clamped to [-8, 1)).  Particles past them set the guard,
+            // which synchronize reports.
+            const float wm = weight_max(p);
+            a.ucap = (float)(2.0 * std::max(1.0, rp.c));
+            a.wmax = wm;
+            a.guard = jguard.get();
             EskExtra ex;
-            if (rp.tile_sort > 0) {
+            ex.fixed = true;
+            ex.fixed_per_w = rp.deltaf || p.w ? 1 : 0;
+            const float wb = rp.deltaf ? 8.f * wm : wm;
+            ex.fixed_bound[0] = ex.fixed_per_w ? wb : 0.f;
+            ex.fixed_bound[1] = ex.fixed_per_w
+                                    ? std::nextafter((float)((double)wb * a.ucap), INFINITY)
+                                    : a.ucap;
+            if (fused) {
                 if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
                     tile_sort(p, df);
                     next_sort = nstep + rp.tile_sort;
                 }
-                ex.slice_off = soff.get();
-            }
-            int pb = 0;
-            if (rp.tile_sort > 0) {
                 PushTiles t{};
                 t.off = soff.get();
                 t.ntx = ctx->ntx;
@@ -1003,29 +1123,33 @@ struct aw_maxwell {
                 t.D = kPushApron;
                 t.W = t.tx + 2 * t.D + 2;
                 t.H = t.ty + 2 * t.D + 2;
+                t.JW = t.tx + 2 * esk::kSortedApron;
+                t.JH = t.ty + 2 * esk::kSortedApron;
                 t.bpt = tile_bpt(*ctx, p.n);
-                pb = ctx->ntiles * t.bpt;
-                bpart.reserve((size_t)(2 * pb));
-                a.bpart = bpart.get();
-                const size_t sh = (size_t)6 * t.W * t.H * sizeof(float);
-                AW_REQUIRE(sh <= ctx->lds_per_block, "push field window does not fit LDS");
-                yee_push_tiles_kernel<<<pb, kThreads, sh, stream>>>(a, m, rp, t);
+                const size_t sh = (size_t)3 * t.JW * t.JH * sizeof(unsigned long long) +
+                                  (size_t)6 * t.W * t.H * sizeof(float);
+                AW_REQUIRE(sh <= ctx->lds_per_block, "the tile windows do not fit LDS");
+                const esk::EskArgs e = esk::setup(*ctx, nullptr, nullptr, p, rp.qm, rp.dt, stream,
+                                                  ex);
+                hipLaunchKernelGGL(g.ny == 1 ? yee_push_esk_tiles_kernel<true>
+                                             : yee_push_esk_tiles_kernel<false>,
+                                   dim3(ctx->ntiles * t.bpt), dim3(kThreads), sh, stream, a, m, rp,
+                                   t, e, ctx->jacc.get());
+                AW_LAUNCH_CHECK();
+                esk::convert(*ctx, e, m.jx, m.jy, m.jz, stream);
             } else {
-                pb = std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
-                bpart.reserve((size_t)(2 * pb));
-                a.bpart = bpart.get();
+                const int pb =
+                    std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
                 yee_push_kernel<<<pb, kThreads, 0, stream>>>(a, m, rp);
+                AW_LAUNCH_CHECK();
+                aw_particles dep = p;
+                if (own_vz) dep.uz = vz.get();
+                if (rp.deltaf) dep.w = wdep.get();
+                ex.xw = p.x;  // the periodic wrap of the move
+                ex.yw = p.y;
+                deposit_esirkepov(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, AW_DEPOSIT_AUTO,
+                                  m.jx, m.jy, m.jz, stream, ex);
             }
-            AW_LAUNCH_CHECK();
-            aw_particles dep = p;
-            if (own_vz) dep.uz = vz.get();
-            if (rp.deltaf) dep.w = wdep.get();
-            ex.bound_part = bpart.get();
-            ex.npart = pb;
-            ex.xw = p.x;  // the periodic wrap of the move
-            ex.yw = p.y;
-            deposit_esirkepov(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, AW_DEPOSIT_AUTO,
-                              m.jx, m.jy, m.jz, stream, ex);
             smooth(m.jx, 3);
         }
         // without particles the antenna and the cold fluid start from J = 0
@@ -1148,6 +1272,8 @@ aw_status aw_maxwell_create(const aw_grid* g, const aw_run_params* rp, aw_maxwel
         const int64_t nc = m->nc();
         m->F.resize((size_t)(9 * nc));
         m->zero(m->F.get(), 9 * nc);
+        m->jguard.resize(1);
+        AW_HIP(hipMemsetAsync(m->jguard.get(), 0, sizeof(unsigned int), m->stream));
         if (rp->jfilter > 0 && nc > kSmoothLdsCells) m->jtmp.resize((size_t)(6 * nc));
         if (rp->bnd_x) m->build_masks();
         if (rp->cold_nc > 0.0) {
@@ -1173,7 +1299,7 @@ aw_status aw_maxwell_stream(aw_maxwell* m, aw_stream* out) {
 aw_status aw_maxwell_synchronize(aw_maxwell* m) {
     return guard("aw_maxwell_synchronize", [&] {
         AW_REQUIRE(m, "NULL argument");
-        AW_HIP(hipStreamSynchronize(m->stream));
+        m->check_guard();
     });
 }
 

@@ -509,3 +509,30 @@ def test_antenna_cold_fluid_walls_match_oracle(gpu, oracle, nx):
             print(f"nx={nx} step {k} grid {q}: rel {rel_l2(a, b):.2e}", flush=True)
             assert rel_l2(a, b) < 2e-6, (nx, k, q)
     sim.close()
+
+
+@pytest.mark.parametrize("tile_sort", [0, 4])
+def test_current_scale_guard_reports_a_runaway_particle(gpu, tile_sort):
+    """The current's fixed-point scale assumes |uz/gamma| <= 2 max(1, c) and
+    weights no larger than the run's largest: a particle past the first is
+    reported by the next synchronize (once), and the run goes on."""
+    nx = ny = 32
+    g = L.grid(nx, ny, 16.0, 16.0)
+    rp = L.run_params(dt=0.04, qm=-1.0, eps0=1.0, c=5.0)
+    rp.B0[0], rp.tile_sort = 0.3, tile_sort
+    rng = np.random.default_rng(4)
+    n = 4000
+    st = {"x": rng.uniform(0, nx, n), "y": rng.uniform(0, ny, n),
+          "ux": rng.normal(0, 0.1, n), "uy": rng.normal(0, 0.1, n), "uz": rng.normal(0, 0.1, n)}
+    sim = MaxwellSimulation(g, rp)
+    sim.set_particles(st, w=np.full(n, 0.25, np.float32))
+    sim.step()
+    sim.synchronize()  # within the scale
+    st = sim.particles()
+    st["uz"][17] = 11.0  # past 2 c = 10
+    sim.set_particles(st, w=np.full(n, 0.25, np.float32))
+    sim.step()
+    with pytest.raises(L.AwError, match="fixed-point scale"):
+        sim.synchronize()
+    sim.synchronize()  # reported once
+    sim.close()
