@@ -591,6 +591,10 @@ typedef struct aw_deltaf_state {
 aw_status aw_maxwell_create(const aw_grid* g, const aw_run_params* rp, aw_maxwell** out);
 aw_status aw_maxwell_destroy(aw_maxwell* m);
 aw_status aw_maxwell_stream(aw_maxwell* m, aw_stream* out);
+/* Waits for the stream; AW_ERR_INVALID (once) when a step since the last call
+ * had a particle outside the current's fixed-point scale: |uz/gamma| above
+ * 2 max(1, c), or a weight above the largest the run's weights had when the
+ * array was first stepped. */
 aw_status aw_maxwell_synchronize(aw_maxwell* m);
 aw_status aw_maxwell_mesh(aw_maxwell* m, aw_yee_mesh* out);
 /* MaxwellSimulation::step / step_at (:51-110): B half step; staggered gather
@@ -599,8 +603,12 @@ aw_status aw_maxwell_mesh(aw_maxwell* m, aw_yee_mesh* out);
  * (rp.bnd_x); the charge-conserving current of the move (int64 fixed point,
  * independent of thread and particle order); periodic wrap; rp.jfilter
  * binomial passes; antenna and cold-fluid currents; B half step; E full
- * step; x damping.  The caller's particle arrays are updated in place and
- * never reordered.  step uses t = (steps so far) * dt. */
+ * step; x damping.  The caller's particle arrays are updated in place; with
+ * rp.tile_sort = N > 0 they are stably re-sorted into 16x16 tiles every N
+ * steps (with the delta-f weight; df_wprec must be 0) and the push and the
+ * current run as one kernel per tile (k_push_esirkepov_tiled,
+ * include/pic/yee2d.hpp:401-469), else they are never reordered.  step uses
+ * t = (steps so far) * dt. */
 aw_status aw_maxwell_step(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df);
 aw_status aw_maxwell_step_at(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df,
                              double tnow);
