@@ -373,9 +373,11 @@ def yee_bench(steps: int = 300, warmup: int = 5):
     wsum, wrms, wmax = sim.wd_stats()
     sim.close()
     # per particle-step: the push reads x, y, ux, uy, uz, w, wd (28 B) and
-    # writes x, y, ux, uy, uz, wd, the old x, y and the deposit weight (36 B);
-    # the current reads the old and new x, y, uz and that weight (24 B)
-    bpp = 88
+    # writes ux, uy, uz, wd, the old and new x, y, the deposit velocity
+    # uz/gamma (the walls damp u after it is taken) and weight (40 B); the
+    # current reads the move, that velocity and weight (24 B) and writes the
+    # wrapped x, y (8 B)
+    bpp = 100
     out = {"workload": "Yee delta-f whistler line, chen2026_case1_df deck: 5000x1 cells, "
                        "dipole background, loss-cone hot electrons 800 ppc, walls, "
                        "jfilter 3, wd and By/Bz noise seeds", "particles": n, "steps": steps,
