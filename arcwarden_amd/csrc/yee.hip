@@ -264,16 +264,21 @@ struct WindowGather {
 
 // One particle: the push updates u (and the delta-f weight) in place and
 // returns the move; the caller stores the positions.
-template <class Gather>
+template <bool PLAIN, class Gather>
 __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const aw_run_params& rp,
                                          int t, PushOut& o, const Gather& gat) {
+    // PLAIN: none of the pump, relativistic push, B0 profile, delta-f weights
+    // or walls (a compile-time instance without their registers)
+    const bool pump = !PLAIN && rp.pump, rel = !PLAIN && rp.rel;
+    const bool b0_prof = !PLAIN && rp.b0_prof, deltaf = !PLAIN && rp.deltaf;
+    const int bnd_x = PLAIN ? 0 : rp.bnd_x;
     const float x0 = a.x[t], y0 = a.y[t];
     float Ex, Ey, dBx, dBy;
     gat.two(0, x0, y0, Ex, dBy);
     gat.two(1, x0, y0, Ey, dBx);
     float Ez = gat.one(2, x0, y0);
     const float dBz = gat.one(3, x0, y0);
-    if (rp.pump && a.tnow < rp.pump_toff) {
+    if (pump && a.tnow < rp.pump_toff) {
         const double th = rp.pump_k0 * (double)x0 * (double)m.dx - rp.pump_w0 * a.tnow;
         const double r = ramp_of(a.tnow, rp.pump_trmp, rp.pump_toff);
         Ex += (float)(rp.pump_ex * cos(th) * r);
@@ -287,8 +292,8 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     uy = fmaf(qmh, Ey, uy);
     uz = fmaf(qmh, Ez, uz);
     const float gmid =
-        rp.rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
-    if (rp.b0_prof) {
+        rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
+    if (b0_prof) {
         // B0 along x varying with x; the mirror force enters as the effective
         // transverse field (include/pic/yee2d.hpp:177-186)
         const float xph = x0 * m.dx;
@@ -301,21 +306,21 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     ux = fmaf(qmh, Ex, ux);
     uy = fmaf(qmh, Ey, uy);
     uz = fmaf(qmh, Ez, uz);
-    const float gnew = rp.rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
+    const float gnew = rel ? 1.f / sqrtf(1.f + fmaf(uz, uz, fmaf(uy, uy, ux * ux))) : 1.f;
     float wdv = 0.f;
-    if (rp.deltaf) {
+    if (deltaf) {
         // dwd/dt = -(1 - wd) (q/m) F . dln f0/du at u^n = (u^{n-1/2} + u^{n+1/2})/2,
         // F = E + v x dB (wave fields and pump), f0 the bi-Maxwellian with B0
         // along x (include/pic/yee2d.hpp:191-271)
         const float uxc = 0.5f * (ux + uxo), uyc = 0.5f * (uy + uyo), uzc = 0.5f * (uz + uzo);
         const float gc =
-            rp.rel ? 1.f / sqrtf(1.f + fmaf(uzc, uzc, fmaf(uyc, uyc, uxc * uxc))) : 1.f;
+            rel ? 1.f / sqrtf(1.f + fmaf(uzc, uzc, fmaf(uyc, uyc, uxc * uxc))) : 1.f;
         const float Fx = Ex + gc * (uyc * dBz - uzc * dBy);
         const float Fy = Ey + gc * (uzc * dBx - uxc * dBz);
         const float Fz = Ez + gc * (uxc * dBy - uyc * dBx);
         float tperp = (float)rp.df_tperp;
         float t2 = (float)(rp.df_kappa * rp.df_tperp);
-        if (rp.b0_prof) {
+        if (b0_prof) {
             const float b = b0_profile(rp, x0 * m.dx) / rp.B0[0];
             const float par = (1.f - 1.f / b) / (float)rp.df_tpar;
             tperp = 1.f / (par + (1.f / b) / (float)rp.df_tperp);
@@ -357,7 +362,7 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     float x1 = x0 + (float)((double)(ux * gnew) * rp.dt / (double)m.dx);
     const float y1 = y0 + (float)((double)(uy * gnew) * rp.dt / (double)m.dy);
     float uxs = ux, uys = uy, uzs = uz;
-    if (rp.bnd_x) {  // specular walls at the domain ends (include/pic/yee2d.hpp:276-297)
+    if (bnd_x) {  // specular walls at the domain ends (include/pic/yee2d.hpp:276-297)
         const float fnx = (float)m.nx;
         if (x1 < 0.f) {
             x1 = -x1;
@@ -367,7 +372,7 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
             uxs = -ux;
         }
         if (x1 >= fnx) x1 = nextafterf(fnx, 0.f);
-        if (rp.bnd_x == 2) {  // transverse momentum damped inside the layers
+        if (bnd_x == 2) {  // transverse momentum damped inside the layers
             const float nd = (float)rp.bnd_nd;
             float d = 0.f;
             if (x1 < nd) d = (nd - x1) / nd;
@@ -383,7 +388,7 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     a.uy[t] = uys;
     a.uz[t] = uzs;
     // the current's uz/gamma is taken before the walls damp u
-    const float uzd = (rp.rel || rp.bnd_x) ? uz * gnew : uzs;
+    const float uzd = (rel || bnd_x) ? uz * gnew : uzs;
     const float wp = a.w ? a.w[t] : a.w_uniform;
     if (!(fabsf(uzd) <= a.ucap) || (a.w && !(fabsf(wp) <= a.wmax))) atomicOr(a.guard, 1u);
     o.x0 = x0;
@@ -391,19 +396,20 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
     o.x1 = x1;
     o.y1 = y1;
     o.uz = uzd;
-    o.w = rp.deltaf ? wdv : wp;
+    o.w = deltaf ? wdv : wp;
 }
 
 // MaxwellSimulation's particle push (include/pic/yee2d.hpp:140-300), grid-
 // stride; stores the move for the current deposit that follows.  4 waves per
 // SIMD fit without spills.
+template <bool PLAIN>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
 yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
     const MeshGather gat{m};
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < a.n;
          t += (int64_t)kThreads * gridDim.x) {
         PushOut o;
-        push_one(a, m, rp, (int)t, o, gat);
+        push_one<PLAIN>(a, m, rp, (int)t, o, gat);
         a.x0[t] = o.x0;
         a.y0[t] = o.y0;
         a.x[t] = o.x1;
@@ -427,12 +433,15 @@ struct PushTiles {
     int JW, JH;                     // current window: tx + 2 A, ty + 2 A
 };
 
-// Register budget: 4 waves per SIMD (128 VGPRs, a few spilled).  The
-// compiler's own choice, 152 VGPRs (3 waves), measured 3.81 ms on the
-// 156 M-particle eaw_case7 step against 3.32 ms here; 5 waves (96 VGPRs, 39
-// spilled) measured 4.08 ms.
-template <bool ONE_ROW>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+// Register budgets, measured on the 156 M-particle eaw_case7 step: the
+// general instance at 4 waves per SIMD (128 VGPRs, 12 spilled) took 3.32 ms
+// against 3.81 ms at the compiler's 152 VGPRs (3 waves) and 4.08 ms at 5
+// waves (39 spilled); the plain instance at 6 waves (80 VGPRs, 4 spilled),
+// which is also as many as its 25 KB of LDS lets a CU hold, took 2.70 ms
+// against 3.04 ms at its own 91 VGPRs (5 waves).
+template <bool ONE_ROW, bool PLAIN>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(PLAIN ? 6 : 4, PLAIN ? 6 : 4)))
 yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk::EskArgs e,
                           unsigned long long* __restrict__ acc) {
     extern __shared__ unsigned long long s_mem[];
@@ -464,7 +473,7 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
     for (int r = r0 + (int)threadIdx.x; r < r1; r += kThreads) {
         PushOut o;
-        push_one(a, m, rp, r, o, gat);
+        push_one<PLAIN>(a, m, rp, r, o, gat);
         a.x[r] = dev::wrap_coord(o.x1, (float)m.nx);
         a.y[r] = dev::wrap_coord(o.y1, (float)m.ny);
         esk::EskMove mv;
@@ -1131,8 +1140,13 @@ struct aw_maxwell {
                 AW_REQUIRE(sh <= ctx->lds_per_block, "the tile windows do not fit LDS");
                 const esk::EskArgs e = esk::setup(*ctx, nullptr, nullptr, p, rp.qm, rp.dt, stream,
                                                   ex);
-                hipLaunchKernelGGL(g.ny == 1 ? yee_push_esk_tiles_kernel<true>
-                                             : yee_push_esk_tiles_kernel<false>,
+                const bool plain =
+                    !(rp.pump || rp.rel || rp.b0_prof || rp.deltaf || rp.bnd_x);
+                auto kern = g.ny == 1 ? (plain ? yee_push_esk_tiles_kernel<true, true>
+                                               : yee_push_esk_tiles_kernel<true, false>)
+                                      : (plain ? yee_push_esk_tiles_kernel<false, true>
+                                               : yee_push_esk_tiles_kernel<false, false>);
+                hipLaunchKernelGGL(kern,
                                    dim3(ctx->ntiles * t.bpt), dim3(kThreads), sh, stream, a, m, rp,
                                    t, e, ctx->jacc.get());
                 AW_LAUNCH_CHECK();
@@ -1140,7 +1154,10 @@ struct aw_maxwell {
             } else {
                 const int pb =
                     std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
-                yee_push_kernel<<<pb, kThreads, 0, stream>>>(a, m, rp);
+                if (!(rp.pump || rp.rel || rp.b0_prof || rp.deltaf || rp.bnd_x))
+                    yee_push_kernel<true><<<pb, kThreads, 0, stream>>>(a, m, rp);
+                else
+                    yee_push_kernel<false><<<pb, kThreads, 0, stream>>>(a, m, rp);
                 AW_LAUNCH_CHECK();
                 aw_particles dep = p;
                 if (own_vz) dep.uz = vz.get();
