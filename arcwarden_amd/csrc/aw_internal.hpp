@@ -274,6 +274,8 @@ struct EskExtra {
     float fixed_bound[2] = {0.f, 0.f};
     const int32_t* slice_off = nullptr;
     float *xw = nullptr, *yw = nullptr;
+    // leave the accumulated current in c.jacc for the caller to convert
+    bool no_convert = false;
 };
 void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p,
                        double qm, double dt, int policy, float* jx, float* jy, float* jz,

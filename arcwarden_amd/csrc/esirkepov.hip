@@ -75,6 +75,19 @@ __global__ void __launch_bounds__(kFinalThreads) esk_bound_final_kernel(const fl
     }
 }
 
+// the accumulators cleared and the fixed bounds written, in one launch
+__global__ void __launch_bounds__(kThreads) esk_begin_kernel(unsigned long long* __restrict__ acc,
+                                                             int64_t n, float f0, float f1,
+                                                             unsigned int* __restrict__ bound) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (int64_t)kThreads * gridDim.x)
+        acc[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        bound[0] = __float_as_uint(f0);
+        bound[1] = __float_as_uint(f1);
+    }
+}
+
 __device__ __forceinline__ void write_wrapped(const EskArgs& a, int i) {
     if (a.xw) {
         a.xw[i] = dev::wrap_coord(a.x1[i], (float)a.nx);
@@ -197,8 +210,7 @@ __global__ void __launch_bounds__(kThreads) esk_convert_kernel(EskArgs a,
     esk_exponents(a, S);
     float* out[3] = {jx, jy, jz};
     for (int q = 0; q < 3; ++q) {
-        const long long v = (long long)acc[q * nc + c];
-        out[q][c] += (float)ldexp((double)v, -S[q]);
+        out[q][c] += esk_value((long long)acc[q * nc + c], S[q]);
     }
 }
 
@@ -288,19 +300,13 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                  __LINE__);
         }
     }
-    esk::convert(c, a, jx, jy, jz, s);
+    if (!ex.no_convert) esk::convert(c, a, jx, jy, jz, s);
 }
 
 namespace esk {
 
-EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
-              double dt, hipStream_t s, const EskExtra& ex) {
-    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
-    AW_REQUIRE(dt > 0, "dt must be positive");
-    AW_REQUIRE(!ex.xw == !ex.yw, "wrapped x and y outputs go together");
-    const int64_t nc = ncell(c);
-    c.jacc.reserve((size_t)(3 * nc));
-    AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
+EskArgs args(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
+             double dt, const EskExtra& ex) {
     c.jbound.reserve(2);
     EskArgs a{};
     a.x0 = x0;
@@ -321,14 +327,31 @@ EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p
     a.dx = c.g.dx;
     a.dy = c.g.dy;
     a.bound = c.jbound.get();
+    return a;
+}
+
+EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
+              double dt, hipStream_t s, const EskExtra& ex) {
+    AW_REQUIRE(p.n < (int64_t)INT32_MAX, "particle count must fit int32");
+    AW_REQUIRE(dt > 0, "dt must be positive");
+    AW_REQUIRE(!ex.xw == !ex.yw, "wrapped x and y outputs go together");
+    const int64_t nc = ncell(c);
+    c.jacc.reserve((size_t)(3 * nc));
+    if (!ex.fixed)
+        AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
+    const EskArgs a = args(c, x0, y0, p, qm, dt, ex);
     const float* part = ex.bound_part;
     int npart = ex.npart;
     if (ex.fixed) {
         AW_REQUIRE(ex.fixed_bound[0] >= 0.f && ex.fixed_bound[1] >= 0.f &&
                        std::isfinite(ex.fixed_bound[0]) && std::isfinite(ex.fixed_bound[1]),
                    "the current's fixed scale bounds must be finite and non-negative");
-        part = nullptr;
-        npart = 0;
+        const int nb = (int)std::min<int64_t>((3 * nc + kThreads - 1) / kThreads,
+                                              8 * std::max(1, c.num_cu));
+        esk_begin_kernel<<<nb, kThreads, 0, s>>>(c.jacc.get(), 3 * nc, ex.fixed_bound[0],
+                                                 ex.fixed_bound[1], c.jbound.get());
+        AW_LAUNCH_CHECK();
+        return a;
     } else if (!part) {
         npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
         c.jpart.reserve((size_t)std::max(2, 2 * npart));

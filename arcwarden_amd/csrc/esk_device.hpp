@@ -36,6 +36,11 @@ __device__ __forceinline__ void esk_exponents(const EskArgs& a, int S[3]) {
     }
 }
 
+// the current of accumulated quanta v of component q: v 2^-S_q
+__device__ __forceinline__ float esk_value(long long v, int Sq) {
+    return (float)ldexp((double)v, -Sq);
+}
+
 __device__ __forceinline__ void esk_scales(const EskArgs& a, double scale[3]) {
     int S[3];
     esk_exponents(a, S);
@@ -96,6 +101,8 @@ __device__ __forceinline__ int wrap_local(int l, int n) {
 // bound pass, the caller's per-workgroup maxima, or the caller's fixed
 // values) and returns the kernels' arguments; convert adds the accumulated
 // current to jx, jy, jz.
+EskArgs args(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
+             double dt, const EskExtra& ex);
 EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
               double dt, hipStream_t s, const EskExtra& ex);
 void convert(aw_ctx& c, const EskArgs& a, float* jx, float* jy, float* jz, hipStream_t s);

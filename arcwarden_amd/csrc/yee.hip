@@ -550,6 +550,30 @@ binomial_kernel(const float* __restrict__ src, float* __restrict__ dst, Mesh m) 
     dst[blockIdx.y * nc + c] = binomial_at(src + blockIdx.y * nc, m, c);
 }
 
+// the current's conversion (J += acc 2^-S) and every smoothing pass of
+// component blockIdx.x in LDS, for grids of up to kSmoothLdsCells
+__global__ void __launch_bounds__(1024)
+convert_smooth_kernel(float* __restrict__ j, Mesh m, int passes,
+                      const unsigned long long* __restrict__ acc, esk::EskArgs e) {
+    extern __shared__ float s_c[];
+    const int nc = m.nx * m.ny, q = blockIdx.x;
+    int S[3];
+    esk::esk_exponents(e, S);
+    float* g = j + (int64_t)q * nc;
+    float *cur = s_c, *nxt = s_c + nc;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x)
+        cur[c] = g[c] + esk::esk_value((long long)acc[(int64_t)q * nc + c], S[q]);
+    __syncthreads();
+    for (int k = 0; k < passes; ++k) {
+        for (int c = threadIdx.x; c < nc; c += blockDim.x) nxt[c] = binomial_at(cur, m, c);
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) g[c] = cur[c];
+}
+
 // every pass of grid blockIdx.x in LDS, for grids of up to kSmoothLdsCells
 constexpr int kSmoothThreads = 1024;
 constexpr int kSmoothLdsCells = 8192;
@@ -590,12 +614,6 @@ __device__ __forceinline__ void antenna_cell(const Mesh& m, const aw_run_params&
     m.jz[c] += (float)(g * sin(ph));
 }
 
-__global__ void antenna_kernel(Mesh m, aw_run_params rp, double t) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
-    antenna_cell(m, rp, t, c);
-}
-
 // linearised cold electron fluid on the nodes, ny = 1 (include/pic/yee2d.hpp:554-578), one cell
 __device__ __forceinline__ void cold_cell(const Mesh& m, const aw_run_params& rp,
                                           float* __restrict__ vcy, float* __restrict__ vcz,
@@ -619,13 +637,6 @@ __device__ __forceinline__ void cold_cell(const Mesh& m, const aw_run_params& rp
     m.jz[c] += jc * uz;
 }
 
-__global__ void cold_fluid_kernel(Mesh m, aw_run_params rp, float* __restrict__ vcy,
-                                  float* __restrict__ vcz) {
-    const int c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= m.nx * m.ny) return;
-    cold_cell(m, rp, vcy, vcz, c);
-}
-
 // x damping layers: node mask on Ey, Ez, Bx (and the cold fluid), half-node
 // mask on Ex, By, Bz (include/pic/yee2d.hpp:531-545), one cell
 __device__ __forceinline__ void damp_cell(const Mesh& m, const float* __restrict__ mn,
@@ -644,6 +655,20 @@ __device__ __forceinline__ void damp_cell(const Mesh& m, const float* __restrict
         vcy[c] *= a;
         vcz[c] *= a;
     }
+}
+
+// the field side between the particles' current and Ampere, per cell:
+// [J cleared], antenna, cold fluid, the second Faraday half step.  None of
+// them reads what another cell's update writes (Faraday reads E only).
+__global__ void __launch_bounds__(kThreads)
+field_mid_kernel(Mesh m, aw_run_params rp, double t, float h, int clear, int ant,
+                 float* __restrict__ vcy, float* __restrict__ vcz) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= m.nx * m.ny) return;
+    if (clear) m.jx[c] = m.jy[c] = m.jz[c] = 0.f;
+    if (ant) antenna_cell(m, rp, t, c);
+    if (vcy) cold_cell(m, rp, vcy, vcz, c);
+    faraday_cell(m, h, c);
 }
 
 __global__ void damp_kernel(Mesh m, const float* __restrict__ mn, const float* __restrict__ mh,
@@ -1015,6 +1040,22 @@ struct aw_maxwell {
         }
     }
 
+    // J += the accumulated current, then rp.jfilter smoothing passes: one LDS
+    // launch for the three components of a small grid
+    void convert_and_smooth(const esk::EskArgs& e) {
+        const Mesh m = mesh();
+        const int64_t n = nc();
+        const size_t lds = (size_t)(2 * n) * sizeof(float);
+        if (rp.jfilter > 0 && n <= kSmoothLdsCells && lds <= ctx->lds_per_block) {
+            convert_smooth_kernel<<<3, kSmoothThreads, lds, stream>>>(m.jx, m, rp.jfilter,
+                                                                     ctx->jacc.get(), e);
+            AW_LAUNCH_CHECK();
+            return;
+        }
+        esk::convert(*ctx, e, m.jx, m.jy, m.jz, stream);
+        smooth(m.jx, 3);
+    }
+
     // Stable physical sort of the particles into tile order, in place in the
     // caller's arrays (Particles::sort_by_tile, include/pic/particles.hpp:677-726)
     void tile_sort(aw_particles& p, const aw_deltaf_state* df) {
@@ -1150,7 +1191,7 @@ struct aw_maxwell {
                                    dim3(ctx->ntiles * t.bpt), dim3(kThreads), sh, stream, a, m, rp,
                                    t, e, ctx->jacc.get());
                 AW_LAUNCH_CHECK();
-                esk::convert(*ctx, e, m.jx, m.jy, m.jz, stream);
+                convert_and_smooth(e);
             } else {
                 const int pb =
                     std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
@@ -1164,26 +1205,19 @@ struct aw_maxwell {
                 if (rp.deltaf) dep.w = wdep.get();
                 ex.xw = p.x;  // the periodic wrap of the move
                 ex.yw = p.y;
+                ex.no_convert = true;
                 deposit_esirkepov(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, AW_DEPOSIT_AUTO,
                                   m.jx, m.jy, m.jz, stream, ex);
+                convert_and_smooth(esk::args(*ctx, x0.get(), y0.get(), dep, rp.qm, rp.dt, ex));
             }
-            smooth(m.jx, 3);
         }
         // without particles the antenna and the cold fluid start from J = 0
         const bool ant = rp.ant_amp != 0.0, cold = rp.cold_nc > 0.0;
         const bool clear = p.n == 0 && (ant || cold);
         float* vcy = cold ? vc.get() : nullptr;
         float* vcz = cold ? vc.get() + nc() : nullptr;
-        if (clear) zero(m.jx, 3 * nc());
-        if (ant) {
-            antenna_kernel<<<nb, kThreads, 0, stream>>>(m, rp, tnow);
-            AW_LAUNCH_CHECK();
-        }
-        if (cold) {
-            cold_fluid_kernel<<<nb, kThreads, 0, stream>>>(m, rp, vcy, vcz);
-            AW_LAUNCH_CHECK();
-        }
-        faraday_kernel<<<nb, kThreads, 0, stream>>>(m, h, 0);
+        field_mid_kernel<<<nb, kThreads, 0, stream>>>(m, rp, tnow, h, clear ? 1 : 0, ant ? 1 : 0,
+                                                      vcy, vcz);
         AW_LAUNCH_CHECK();
         ampere_kernel<<<nb, kThreads, 0, stream>>>(m);
         AW_LAUNCH_CHECK();
