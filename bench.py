@@ -21,7 +21,8 @@ on this host's cores on the same particle state.  ``ensemble`` compares a
 times the field-energy gradient of a 128^2 two-stream run against its plain
 forward run; ``darwin`` times the three-run Darwin whistler-pump scan with
 the Esirkepov current deposit (``darwin_cic_current``: with the reference's
-CIC current); ``yee_eaw_case7`` runs the reference's own Yee benchmark deck
+CIC current); ``yee_eaw_case7`` runs the reference's own Yee benchmark deck (and
+``yee_eaw_case7_flat`` the same run without its tile sort)
 and ``yee`` its delta-f whistler line; ``checkpoint`` times a restart file
 write and restore of the headline run, outside the timed region.
 """
@@ -389,17 +390,21 @@ def yee_bench(steps: int = 300, warmup: int = 5):
     return out
 
 
-def yee_eaw_bench(steps: int = 40, warmup: int = 5):
+def yee_eaw_bench(steps: int = 40, warmup: int = 5, tile_sort: bool = True):
     """The reference's published Yee benchmark, the eaw_case7_tiled deck
     (tests/golden/decks): 625x625 periodic cells, 400 ppc (156.25 M
     electrons, T_perp/T_par = 5), B0 = 0.25 along x, c = 1, dt 0.0145,
     3-pass binomial current smoothing, tile re-sort every 20 steps, no
     diagnostics inside the timed steps.  The reference quotes 1.51e10
-    particle-steps/s for this run without diagnostics on an RTX 5090
-    (docs/PROFILE_BASELINE.md:39-50)."""
+    particle-steps/s for this run without diagnostics on an RTX 5090, and
+    4.5e9 for the same run without the tile sort (eaw_case7.ini, the flat
+    path) (docs/PROFILE_BASELINE.md:39-50).  tile_sort=False runs the flat
+    configuration."""
     from arcwarden_amd.deck import load_deck
     from arcwarden_amd.maxwell import MaxwellSimulation
     d = load_deck(ROOT / "tests" / "golden" / "decks" / "eaw_case7_tiled.ini")
+    if not tile_sort:
+        d.rp.tile_sort = 0
     sim = MaxwellSimulation(d.grid(), d.rp)
     sim.load_species(d.species)
     n = sim.n
@@ -413,14 +418,15 @@ def yee_eaw_bench(steps: int = 40, warmup: int = 5):
     el = time.perf_counter() - t0
     we, wb = sim.field_energy()
     sim.close()
-    ref = 1.51e10
-    out = {"workload": "Yee eaw_case7_tiled deck: 625x625 cells, 400 ppc, B0 0.25 x, "
-                       "jfilter 3, tile_sort 20", "particles": n, "steps": steps,
+    ref = 1.51e10 if tile_sort else 4.5e9
+    out = {"workload": "Yee eaw_case7 deck: 625x625 cells, 400 ppc, B0 0.25 x, jfilter 3, " +
+                       ("tile_sort 20" if tile_sort else "no tile sort"),
+           "particles": n, "steps": steps,
            "value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3,
            "reference_rtx5090": ref, "vs_reference": n * steps / el / ref,
            "field_energy": [we, wb]}
-    print(f"# yee eaw_case7: {n} particles, {out['ms_per_step']:.4f} ms/step", file=sys.stderr,
-          flush=True)
+    print(f"# yee eaw_case7 (tile_sort {int(tile_sort)}): {n} particles, "
+          f"{out['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
     return out
 
 
@@ -770,6 +776,8 @@ def main():
         dar_cic = darwin_bench(args.darwin_steps, L.AW_CURRENT_CIC, **kw)
     yee = yee_bench(args.yee_steps) if args.yee_steps > 0 else None
     yee_eaw = yee_eaw_bench(args.eaw_steps) if args.eaw_steps > 0 else None
+    yee_eaw_flat = (yee_eaw_bench(max(1, args.eaw_steps // 2), tile_sort=False)
+                    if args.eaw_steps > 0 else None)
     pol = None if args.no_policies else policy_bench(nx, args.ppc)
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -811,6 +819,7 @@ def main():
         "darwin_cic_current": dar_cic,
         "yee": yee,
         "yee_eaw_case7": yee_eaw,
+        "yee_eaw_case7_flat": yee_eaw_flat,
         "checkpoint": ck_out or None,
         "policies": pol,
         "checks": {"charge_drift": charge_drift, "sorts": sorts,
