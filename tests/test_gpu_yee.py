@@ -392,8 +392,8 @@ def test_mirror_load_counts_and_moments(gpu, oracle):
     assert np.allclose(per_cell, 2 * tp, rtol=0.15)
 
 
-def _tile_sort_run(tile_sort, deltaf, steps=15, seed=11):
-    nx, ny, dx = 64, 48, 0.5
+def _tile_sort_run(tile_sort, deltaf, steps=15, seed=11, nx=64, ny=48):
+    dx = 0.5
     g = L.grid(nx, ny, nx * dx, ny * dx)
     rp = L.run_params(dt=0.06, qm=-1.0, eps0=1.0, c=5.0)
     rp.B0[0], rp.B0[2] = 0.4, 0.1
@@ -402,7 +402,7 @@ def _tile_sort_run(tile_sort, deltaf, steps=15, seed=11):
         rp.B0[2] = 0.0
         rp.deltaf, rp.df_tpar, rp.df_tperp = 1, 1.5, 2.0
     rng = np.random.default_rng(seed)
-    n = 200_000
+    n = 200_000 if ny > 1 else 60_000
     st = {"x": rng.uniform(0, nx, n).astype(np.float32),
           "y": rng.uniform(0, ny, n).astype(np.float32)}
     for k in ("ux", "uy", "uz"):  # up to ~0.8 cells a step, ~2 cells between sorts
@@ -430,20 +430,28 @@ def _as_set(P, keys):
     return cols[:, order]
 
 
-@pytest.mark.parametrize("deltaf", [False, True])
-def test_tile_sort_matches_unsorted(gpu, deltaf):
+@pytest.mark.parametrize("deltaf,nx,ny", [(False, 64, 48), (True, 64, 48), (False, 512, 1),
+                                          (True, 512, 1)])
+def test_tile_sort_matches_unsorted(gpu, deltaf, nx, ny):
     """MaxwellSimulation::step_at with rp.tile_sort (include/pic/simulation_maxwell.hpp:
     60-72): re-sorting the particles every 5 steps only reorders them -- the
-    fields after 15 steps are the same bits, and so is the particle set."""
-    M0, P0 = _tile_sort_run(0, deltaf)
-    M1, P1 = _tile_sort_run(5, deltaf)
-    print(f"tile_sort deltaf={deltaf}: max |J| {np.abs(M0[-3 * 3072:]).max():.3e}", flush=True)
+    fields after 15 steps are the same bits, and so is the particle set.  The
+    tile-sorted run pushes and deposits in one kernel per tile (the plain
+    instance without delta-f, the general one with it; one-row grids sum
+    their aliased stencil rows), the unsorted one in a push and a deposit."""
+    M0, P0 = _tile_sort_run(0, deltaf, nx=nx, ny=ny)
+    M1, P1 = _tile_sort_run(5, deltaf, nx=nx, ny=ny)
+    nc = nx * ny
+    print(f"tile_sort deltaf={deltaf} {nx}x{ny}: max |J| {np.abs(M0[-3 * nc:]).max():.3e}",
+          flush=True)
+    assert np.abs(M0[-3 * nc:]).max() > 0
     assert np.array_equal(M0, M1)
     keys = ("x", "y", "ux", "uy", "uz", "w") + (("wd",) if deltaf else ())
     assert np.array_equal(_as_set(P0, keys), _as_set(P1, keys))
     # the sorted run keeps the particles near tile order (the unsorted one
     # steps back to a lower tile at about every other particle)
-    t = (P1["y"] // 16).astype(np.int64) * 4 + (P1["x"] // 16).astype(np.int64)
+    ntx = (nx + 15) // 16
+    t = (P1["y"] // 16).astype(np.int64) * ntx + (P1["x"] // 16).astype(np.int64)
     assert np.mean(np.diff(t) < 0) < 0.25
 
 
