@@ -146,6 +146,98 @@ __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_kernel(EskArgs a,
     }
 }
 
+// The whole one-row grid in LDS (ny = 1), for particles that may arrive in
+// cell order: when every particle of a wavefront starts its stencil at the
+// same column, the wavefront sums its eight column quanta across the lanes
+// (exact int64 butterflies) and one lane adds them, instead of 64 lanes
+// adding to the same LDS words one after another.  Otherwise each lane adds
+// its own.
+__global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_row_kernel(EskArgs a,
+                                                                   unsigned long long* __restrict__ acc) {
+    extern __shared__ unsigned long long s_j[];
+    const int nc = a.nx;
+    for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) s_j[c] = 0ull;
+    __syncthreads();
+    double scale[3];
+    esk_scales(a, scale);
+    const int stride = blockDim.x * gridDim.x;
+    // one iteration's particle loads are in flight while the previous one is
+    // deposited
+    float p0[6] = {};
+    auto load = [&](int i, float* v) {
+        if (i < a.n) {
+            v[0] = a.x0[i];
+            v[1] = a.y0[i];
+            v[2] = a.x1[i];
+            v[3] = a.y1[i];
+            v[4] = a.uz[i];
+            v[5] = a.w ? a.w[i] : 0.f;
+        }
+    };
+    load(blockIdx.x * blockDim.x + (int)threadIdx.x, p0);
+    for (int base = blockIdx.x * blockDim.x; base < a.n; base += stride) {
+        const int i = base + (int)threadIdx.x;
+        const bool valid = i < a.n;
+        float p[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) p[k] = p0[k];
+        load(i + stride, p0);
+        unsigned long long col[3][4] = {};
+        int key = 0;
+        if (valid) {
+            EskMove m;
+            esk_prepare_v(a, p[0], p[1], p[2], p[3], p[4], a.w ? (double)p[5] : a.w_uniform,
+                          scale, m);
+            dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz,
+                          [&](int q, int k, int, unsigned long long v) { col[q][k] += v; });
+            key = dev::wrapfar(m.sh.ib - 1, a.nx);
+            if (a.xw) {
+                a.xw[i] = dev::wrap_coord(p[2], (float)a.nx);
+                a.yw[i] = dev::wrap_coord(p[3], (float)a.ny);
+            }
+        }
+        // the first valid lane's column base; invalid lanes carry zeros
+        const unsigned long long vmask = __ballot(valid);
+        const int lead = __ffsll((long long)vmask) - 1;
+        const int k0 = __shfl(key, lead, 64);
+        const bool uniform = __ballot(valid && key != k0) == 0ull;
+        if (uniform) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 1; k < 4; ++k) {
+                    unsigned long long v = col[q][k];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                    col[q][k] = v;
+                }
+            if ((int)(threadIdx.x & 63) == lead) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int k = 1; k < 4; ++k)
+                        if (col[q][k]) {
+                            atomicAdd(&s_j[q * nc + dev::wrapfar(k0 + k, a.nx)], col[q][k]);
+                        }
+            }
+        } else if (valid) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 1; k < 4; ++k)
+                    if (col[q][k]) {
+                        const int gi = dev::wrapfar(key + k, a.nx);
+                        atomicAdd(&s_j[q * nc + gi], col[q][k]);
+                    }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) {
+        const unsigned long long v = s_j[c];
+        if (v) atomicAdd(&acc[c], v);
+    }
+}
+
 struct EskTiles {
     const int32_t* off;  // ntiles+1, tile bins of the old positions (or of the last sort)
     const int32_t* idx;  // bin permutation (NULL: the particles are in tile order)
@@ -272,7 +364,10 @@ void deposit_esirkepov(aw_ctx& c, const float* x0, const float* y0, const aw_par
                 &per_cu, reinterpret_cast<const void*>(esk_ldsgrid_kernel<false>), kGridThreads, sh));
             const int nb = (int)std::min<int64_t>(std::max(1, per_cu) * std::max(1, c.num_cu),
                                                   (p.n + kGridThreads - 1) / kGridThreads);
-            launch_one_row(esk_ldsgrid_kernel<true>, esk_ldsgrid_kernel<false>, one_row, nb, kGridThreads, sh, s, a, acc);
+            if (one_row)
+                esk_ldsgrid_row_kernel<<<nb, kGridThreads, sh, s>>>(a, acc);
+            else
+                esk_ldsgrid_kernel<false><<<nb, kGridThreads, sh, s>>>(a, acc);
             AW_LAUNCH_CHECK();
         } else if (policy == AW_DEPOSIT_BINNED) {
             // bin by the old position (stable), as deposit_rho_fixed does

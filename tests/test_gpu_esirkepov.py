@@ -91,17 +91,25 @@ def test_esirkepov_continuity_reference_setup(gpu, oracle):
     assert res_gpu < 5e-5  # the reference's own gate, with FP32 rho storage
 
 
-@pytest.mark.parametrize("nx,ny,tile,n", [(128, 96, 16, 1_000_000), (40, 24, 8, 300_000),
-                                          (16, 4, 16, 5_000), (600, 1, 16, 200_000),
-                                          (3000, 1, 16, 60_000)])
-def test_esirkepov_wrapped_positions_and_policies(gpu, oracle, nx, ny, tile, n):
+@pytest.mark.parametrize("nx,ny,tile,n,by_x", [(128, 96, 16, 1_000_000, False),
+                                               (40, 24, 8, 300_000, False),
+                                               (16, 4, 16, 5_000, False),
+                                               (600, 1, 16, 200_000, False),
+                                               (600, 1, 16, 200_000, True),
+                                               (3000, 1, 16, 60_000, False)])
+def test_esirkepov_wrapped_positions_and_policies(gpu, oracle, nx, ny, tile, n, by_x):
     """Uniform weight; new positions wrapped with the migrate semantics, so the
     kernel must take the minimal image.  Policies agree bit for bit and with
     the FP64 oracle on the same moves; the one-row grids (whose stencil
-    rows coincide) take the LDS grid (600 x 1) and the binned tiles."""
+    rows coincide) take the LDS grid (600 x 1, also with the particles in
+    cell order, where wavefronts sum their quanta before adding) and the
+    binned tiles."""
     dx, dy, dt, qm = 0.5, 0.5, 0.2, -1.0
     rng = np.random.default_rng(nx * 7 + n)
     x0, y0, x1, y1, uz, _ = _moves(rng, nx, ny, n, dx, dy, dt, uniform_w=True)
+    if by_x:  # particles in cell order: whole wavefronts share a stencil column
+        o = np.argsort(x0, kind="stable")
+        x0, y0, x1, y1, uz = x0[o], y0[o], x1[o], y1[o], uz[o]
     wu = 1.0 / 64
     xw = np.mod(x1, np.float32(nx)).astype(np.float32)
     yw = np.mod(y1, np.float32(ny)).astype(np.float32)
