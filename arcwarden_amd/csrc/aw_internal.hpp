@@ -257,18 +257,14 @@ bool tile_sort_onepass(aw_ctx& c, const aw_particles& src, const aw_particles& d
 // counters fit LDS, else the radix sort of tile keys).
 void bin_tiles(aw_ctx& c, const aw_particles& p, int32_t* perm, int32_t* off, hipStream_t s);
 
-// Esirkepov current (esirkepov.hip).  Optional inputs: the per-workgroup
-// maxima pairs (max |w|, max |w uz|, or max |uz| for a uniform weight) of the
-// deposit's particles, already formed by the caller's push (or fixed bounds
-// the caller guarantees); the tile offsets
-// of particles the caller keeps in tile order (ntiles + 1, from an earlier
-// stable tile sort of the same n particles); outputs for the wrapped new
-// positions (may alias p.x, p.y).
+// Esirkepov current (esirkepov.hip).  Optional inputs: fixed scale bounds the
+// caller guarantees, in place of a bound pass over the particles; the tile
+// offsets of particles the caller keeps in tile order (ntiles + 1, from an
+// earlier stable tile sort of the same n particles); outputs for the wrapped
+// new positions (may alias p.x, p.y).
 struct EskExtra {
-    const float* bound_part = nullptr;
-    int npart = 0;
-    // or fixed bounds in place of the particles' maxima: (max |w|, max |w uz|)
-    // with per-particle deposit weights, (unused, max |uz|) with w_uniform
+    // fixed bounds: (max |w|, max |w uz|) with per-particle deposit weights,
+    // (unused, max |uz|) with w_uniform
     bool fixed = false;
     int fixed_per_w = 0;
     float fixed_bound[2] = {0.f, 0.f};

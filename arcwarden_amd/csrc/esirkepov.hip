@@ -435,8 +435,6 @@ EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p
     if (!ex.fixed)
         AW_HIP(hipMemsetAsync(c.jacc.get(), 0, (size_t)(3 * nc) * sizeof(unsigned long long), s));
     const EskArgs a = args(c, x0, y0, p, qm, dt, ex);
-    const float* part = ex.bound_part;
-    int npart = ex.npart;
     if (ex.fixed) {
         AW_REQUIRE(ex.fixed_bound[0] >= 0.f && ex.fixed_bound[1] >= 0.f &&
                        std::isfinite(ex.fixed_bound[0]) && std::isfinite(ex.fixed_bound[1]),
@@ -447,17 +445,15 @@ EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p
                                                  ex.fixed_bound[1], c.jbound.get());
         AW_LAUNCH_CHECK();
         return a;
-    } else if (!part) {
-        npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
-        c.jpart.reserve((size_t)std::max(2, 2 * npart));
-        if (npart > 0) {
-            esk_bound_kernel<<<npart, kThreads, 0, s>>>(a, c.jpart.get());
-            AW_LAUNCH_CHECK();
-        }
-        part = c.jpart.get();
     }
-    esk_bound_final_kernel<<<1, kFinalThreads, 0, s>>>(part, npart, ex.fixed_bound[0],
-                                                       ex.fixed_bound[1], c.jbound.get());
+    const int npart = p.n > 0 ? std::min(blocks_for(p.n), 4 * std::max(1, c.num_cu)) : 0;
+    c.jpart.reserve((size_t)std::max(2, 2 * npart));
+    if (npart > 0) {
+        esk_bound_kernel<<<npart, kThreads, 0, s>>>(a, c.jpart.get());
+        AW_LAUNCH_CHECK();
+    }
+    esk_bound_final_kernel<<<1, kFinalThreads, 0, s>>>(c.jpart.get(), npart, 0.f, 0.f,
+                                                       c.jbound.get());
     AW_LAUNCH_CHECK();
     return a;
 }

@@ -97,10 +97,9 @@ __device__ __forceinline__ int wrap_local(int l, int n) {
     return l;
 }
 
-// Host side.  setup clears the accumulators, fixes the scale bound (from a
-// bound pass, the caller's per-workgroup maxima, or the caller's fixed
-// values) and returns the kernels' arguments; convert adds the accumulated
-// current to jx, jy, jz.
+// Host side.  args fills the kernels' arguments; setup also clears the
+// accumulators and fixes the scale bound (from a bound pass, or the caller's
+// fixed values); convert adds the accumulated current to jx, jy, jz.
 EskArgs args(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
              double dt, const EskExtra& ex);
 EskArgs setup(aw_ctx& c, const float* x0, const float* y0, const aw_particles& p, double qm,
