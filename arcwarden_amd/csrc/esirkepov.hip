@@ -105,11 +105,11 @@ __global__ void __launch_bounds__(kThreads) esk_global_kernel(EskArgs a,
                                                               unsigned long long* __restrict__ acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    double scale[3];
-    esk_scales(a, scale);
+    double kc[3];
+    esk_coefs(a, kc);
     const int64_t nc = (int64_t)a.nx * a.ny;
     EskMove m;
-    esk_prepare(a, i, scale, m);
+    esk_prepare(a, i, kc, m);
     esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
         global_add(acc, a.nx, a.ny, nc, q, m.sh.ib - 1 + k, m.sh.jb - 1 + b, v);
     });
@@ -127,11 +127,11 @@ __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_kernel(EskArgs a,
     const int nc = a.nx * a.ny;
     for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) s_j[c] = 0ull;
     __syncthreads();
-    double scale[3];
-    esk_scales(a, scale);
+    double kc[3];
+    esk_coefs(a, kc);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += blockDim.x * gridDim.x) {
         EskMove m;
-        esk_prepare(a, i, scale, m);
+        esk_prepare(a, i, kc, m);
         esk_emit<ONE_ROW>(m, [&](int q, int k, int b, unsigned long long v) {
             const int gi = dev::wrapfar(m.sh.ib - 1 + k, a.nx);
             const int gj = dev::wrapfar(m.sh.jb - 1 + b, a.ny);
@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_row_kernel(EskArgs a
     const int nc = a.nx;
     for (int c = threadIdx.x; c < 3 * nc; c += blockDim.x) s_j[c] = 0ull;
     __syncthreads();
-    double scale[3];
-    esk_scales(a, scale);
+    double kc[3];
+    esk_coefs(a, kc);
     const int stride = blockDim.x * gridDim.x;
     // one iteration's particle loads are in flight while the previous one is
     // deposited
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(kGridThreads) esk_ldsgrid_row_kernel(EskArgs a
         if (valid) {
             EskMove m;
             esk_prepare_v(a, p[0], p[1], p[2], p[3], p[4], a.w ? (double)p[5] : a.w_uniform,
-                          scale, m);
+                          kc, m);
             dev::esk_emit(m.sh, m.cJx, m.cJy, m.cJz,
                           [&](int q, int k, int, unsigned long long v) { col[q][k] += v; });
             key = dev::wrapfar(m.sh.ib - 1, a.nx);
@@ -256,8 +256,8 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
     const int orgx = (tile % t.ntx) * t.tx - t.A, orgy = (tile / t.ntx) * t.ty - t.A;
     for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) s_j[c] = 0ull;
     __syncthreads();
-    double scale[3];
-    esk_scales(a, scale);
+    double kc[3];
+    esk_coefs(a, kc);
     const int64_t nc = (int64_t)a.nx * a.ny;
     const int beg = t.off[tile], end = t.off[tile + 1];
     const int chunk = (end - beg + t.bpt - 1) / t.bpt;
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
     for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
         const int i = t.idx ? t.idx[r] : r;
         EskMove m;
-        esk_prepare(a, i, scale, m);
+        esk_prepare(a, i, kc, m);
         const int li = wrap_local(m.sh.ib - 1 - orgx, a.nx), lj = wrap_local(m.sh.jb - 1 - orgy, a.ny);
         if (dev::in_tile(li, t.W - 4) && dev::in_tile(lj, t.H - 4)) {
             const int base = lj * t.W + li;

@@ -54,21 +54,32 @@ struct EskMove {
     double cJx, cJy, cJz;
 };
 
-// the move (x0, y0) -> (x1, y1) with deposit weight w and velocity uz
-__device__ __forceinline__ void esk_prepare_v(const EskArgs& a, float x0, float y0, float x1,
-                                              float y1, float uz, double w,
-                                              const double scale[3], EskMove& m) {
-    dev::esk_shape(x0, y0, x1, y1, a.nx, a.ny, m.sh);
-    const double qw = a.qm * w;
-    m.cJx = -qw / (a.dt * a.dy) * scale[0];
-    m.cJy = -qw / (a.dt * a.dx) * scale[1];
-    m.cJz = qw * (double)uz / (a.dx * a.dy) * scale[2];
+// The coefficients' constant factors, once per thread: kc = (2^Sx/(dt dy),
+// 2^Sy/(dt dx), 2^Sz/(dx dy)), so a particle's coefficients are products.
+__device__ __forceinline__ void esk_coefs(const EskArgs& a, double kc[3]) {
+    double scale[3];
+    esk_scales(a, scale);
+    kc[0] = scale[0] / (a.dt * a.dy);
+    kc[1] = scale[1] / (a.dt * a.dx);
+    kc[2] = scale[2] / (a.dx * a.dy);
 }
 
-__device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double scale[3],
+// the move (x0, y0) -> (x1, y1) with deposit weight w and velocity uz:
+// cJx = -q w kc0, cJy = -q w kc1, cJz = q w uz kc2
+__device__ __forceinline__ void esk_prepare_v(const EskArgs& a, float x0, float y0, float x1,
+                                              float y1, float uz, double w, const double kc[3],
+                                              EskMove& m) {
+    dev::esk_shape(x0, y0, x1, y1, a.nx, a.ny, m.sh);
+    const double qw = a.qm * w;
+    m.cJx = -qw * kc[0];
+    m.cJy = -qw * kc[1];
+    m.cJz = qw * (double)uz * kc[2];
+}
+
+__device__ __forceinline__ void esk_prepare(const EskArgs& a, int i, const double kc[3],
                                             EskMove& m) {
     esk_prepare_v(a, a.x0[i], a.y0[i], a.x1[i], a.y1[i], a.uz[i],
-                  a.w ? (double)a.w[i] : a.w_uniform, scale, m);
+                  a.w ? (double)a.w[i] : a.w_uniform, kc, m);
 }
 
 // The contributions of one move.  On a one-row grid (ny = 1) the three

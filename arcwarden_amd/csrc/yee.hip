@@ -465,8 +465,8 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
     for (int c = threadIdx.x; c < 3 * RJ; c += kThreads) s_j[c] = 0ull;
     __syncthreads();
     const WindowGather gat{s_f, m, orgx, orgy, t.W, t.H};
-    double scale[3];
-    esk::esk_scales(e, scale);
+    double kc[3];
+    esk::esk_coefs(e, kc);
     const int64_t nc = (int64_t)m.nx * m.ny;
     const int beg = t.off[tile], end = t.off[tile + 1];
     const int chunk = (end - beg + t.bpt - 1) / t.bpt;
@@ -478,7 +478,7 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
         a.y[r] = dev::wrap_coord(o.y1, (float)m.ny);
         esk::EskMove mv;
         esk::esk_prepare_v(e, o.x0, o.y0, o.x1, o.y1, o.uz, e.per_w ? (double)o.w : e.w_uniform,
-                           scale, mv);
+                           kc, mv);
         const int li = esk::wrap_local(mv.sh.ib - 1 - jorgx, m.nx);
         const int lj = esk::wrap_local(mv.sh.jb - 1 - jorgy, m.ny);
         if (dev::in_tile(li, t.JW - 4) && dev::in_tile(lj, t.JH - 4)) {
