@@ -419,11 +419,17 @@ def yee_eaw_bench(steps: int = 40, warmup: int = 5, tile_sort: bool = True):
     we, wb = sim.field_energy()
     sim.close()
     ref = 1.51e10 if tile_sort else 4.5e9
+    # particle bytes per step: with the tile sort one kernel reads x, y, u, w
+    # (24 B) and writes x, y, u (20 B); without it the push reads 24 B and
+    # writes u and the old and new x, y (28 B), and the current reads the move
+    # and uz (20 B) and writes the wrapped x, y (8 B)
+    bpp = 44 if tile_sort else 80
     out = {"workload": "Yee eaw_case7 deck: 625x625 cells, 400 ppc, B0 0.25 x, jfilter 3, " +
                        ("tile_sort 20" if tile_sort else "no tile sort"),
            "particles": n, "steps": steps,
            "value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el / steps * 1e3,
            "reference_rtx5090": ref, "vs_reference": n * steps / el / ref,
+           "step_bytes_per_particle": bpp, "step_GBs": n * bpp * steps / el / 1e9,
            "field_energy": [we, wb]}
     print(f"# yee eaw_case7 (tile_sort {int(tile_sort)}): {n} particles, "
           f"{out['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
