@@ -280,13 +280,23 @@ __global__ void __launch_bounds__(kThreads) esk_tiles_kernel(EskArgs a, EskTiles
         write_wrapped(a, i);
     }
     __syncthreads();
+    // (row, column) over the three components' 3 H rows, advanced by the
+    // stride without dividing
+    const int dr = (int)blockDim.x / t.W, dc = (int)blockDim.x % t.W;
+    int row = (int)threadIdx.x / t.W, col = (int)threadIdx.x % t.W;
     for (int c = threadIdx.x; c < 3 * R; c += blockDim.x) {
         const unsigned long long v = s_j[c];
         if (v) {
-            const int q = c / R, l = c % R;
-            const int gx = dev::wrapfar(orgx + l % t.W, a.nx);
-            const int gy = dev::wrapfar(orgy + l / t.W, a.ny);
+            const int q = row >= 2 * t.H ? 2 : (row >= t.H ? 1 : 0);
+            const int gx = dev::wrapfar(orgx + col, a.nx);
+            const int gy = dev::wrapfar(orgy + row - q * t.H, a.ny);
             atomicAdd(&acc[q * nc + (int64_t)gy * a.nx + gx], v);
+        }
+        row += dr;
+        col += dc;
+        if (col >= t.W) {
+            col -= t.W;
+            ++row;
         }
     }
 }

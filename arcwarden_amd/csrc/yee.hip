@@ -454,9 +454,17 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
     const int orgx = tx0 - t.D - 1, orgy = ty0 - t.D - 1;
     const int jorgx = tx0 - esk::kSortedApron, jorgy = ty0 - esk::kSortedApron;
     float2* s2 = reinterpret_cast<float2*>(s_f);
+    // (row, column) of a thread's cells advance by the stride without dividing
+    const int dr = kThreads / t.W, dc = kThreads % t.W;
+    int row = threadIdx.x / t.W, col = threadIdx.x % t.W;
     for (int l = threadIdx.x; l < R; l += kThreads) {
-        const int gx = dev::wrapfar(orgx + l % t.W, m.nx), gy = dev::wrapfar(orgy + l / t.W, m.ny);
-        const int g = gy * m.nx + gx;
+        const int g = dev::wrapfar(orgy + row, m.ny) * m.nx + dev::wrapfar(orgx + col, m.nx);
+        row += dr;
+        col += dc;
+        if (col >= t.W) {
+            col -= t.W;
+            ++row;
+        }
         s2[l] = make_float2(m.ex[g], m.by[g]);
         s2[R + l] = make_float2(m.ey[g], m.bx[g]);
         s_f[4 * R + l] = m.ez[g];
@@ -495,13 +503,23 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
         }
     }
     __syncthreads();
+    // rows of the three components follow each other: (row, column) over
+    // 3 JH rows, advanced by the stride without dividing
+    const int jdr = kThreads / t.JW, jdc = kThreads % t.JW;
+    int jrow = threadIdx.x / t.JW, jcol = threadIdx.x % t.JW;
     for (int c = threadIdx.x; c < 3 * RJ; c += kThreads) {
         const unsigned long long v = s_j[c];
         if (v) {
-            const int q = c / RJ, l = c % RJ;
-            const int gx = dev::wrapfar(jorgx + l % t.JW, m.nx);
-            const int gy = dev::wrapfar(jorgy + l / t.JW, m.ny);
+            const int q = jrow >= 2 * t.JH ? 2 : (jrow >= t.JH ? 1 : 0);
+            const int gx = dev::wrapfar(jorgx + jcol, m.nx);
+            const int gy = dev::wrapfar(jorgy + jrow - q * t.JH, m.ny);
             atomicAdd(&acc[q * nc + (int64_t)gy * m.nx + gx], v);
+        }
+        jrow += jdr;
+        jcol += jdc;
+        if (jcol >= t.JW) {
+            jcol -= t.JW;
+            ++jrow;
         }
     }
 }
