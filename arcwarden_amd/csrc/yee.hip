@@ -429,6 +429,7 @@ yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
 // touches HBM: the positions are stored wrapped, as the migrate does.
 struct PushTiles {
     const int32_t* off;  // ntiles + 1
+    const int32_t* idx;  // bin permutation (NULL: the particles are in tile order)
     int ntx, tx, ty, D, W, H, bpt;  // field window: W = tx + 2 D + 2, H = ty + 2 D + 2
     int JW, JH;                     // current window: tx + 2 A, ty + 2 A
 };
@@ -480,10 +481,11 @@ yee_push_esk_tiles_kernel(PushArgs a, Mesh m, aw_run_params rp, PushTiles t, esk
     const int chunk = (end - beg + t.bpt - 1) / t.bpt;
     const int r0 = beg + part * chunk, r1 = min(end, r0 + chunk);
     for (int r = r0 + (int)threadIdx.x; r < r1; r += kThreads) {
+        const int i = t.idx ? t.idx[r] : r;
         PushOut o;
-        push_one<PLAIN>(a, m, rp, r, o, gat);
-        a.x[r] = dev::wrap_coord(o.x1, (float)m.nx);
-        a.y[r] = dev::wrap_coord(o.y1, (float)m.ny);
+        push_one<PLAIN>(a, m, rp, i, o, gat);
+        a.x[i] = dev::wrap_coord(o.x1, (float)m.nx);
+        a.y[i] = dev::wrap_coord(o.y1, (float)m.ny);
         esk::EskMove mv;
         esk::esk_prepare_v(e, o.x0, o.y0, o.x1, o.y1, o.uz, e.per_w ? (double)o.w : e.w_uniform,
                            kc, mv);
@@ -1137,7 +1139,17 @@ struct aw_maxwell {
             const size_t n = (size_t)p.n;
             // the current's uz/gamma is taken before the walls damp u (bnd_x = 2)
             const bool own_vz = rp.rel || rp.bnd_x;
-            const bool fused = rp.tile_sort > 0;
+            // With tile_sort the particles are kept in tile order; without it,
+            // where the current would bin the moves by tile anyway, the
+            // particles are binned by tile before the push (a permutation,
+            // the arrays keep their order) and the same one-kernel push +
+            // current walks the bins.
+            const int64_t ncl = nc();
+            const bool binned = rp.tile_sort <= 0 && ncl > 0 &&
+                                !((size_t)(3 * ncl) * 8 <= ctx->lds_per_block &&
+                                  p.n >= 16 * ncl) &&
+                                (double)p.n / (double)ncl >= 8.0;
+            const bool fused = rp.tile_sort > 0 || binned;
             if (!fused) {
                 x0.reserve(n);
                 y0.reserve(n);
@@ -1179,11 +1191,17 @@ struct aw_maxwell {
                                     ? std::nextafter((float)((double)wb * a.ucap), INFINITY)
                                     : a.ucap;
             if (fused) {
-                if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
+                PushTiles t{};
+                if (binned) {
+                    soff.reserve((size_t)ctx->ntiles + 1);
+                    sperm.reserve(n);
+                    bin_tiles(*ctx, p, sperm.get(), soff.get(), stream);
+                    t.idx = sperm.get();
+                    sorted_n = -1;  // the tile order of an earlier sort is gone
+                } else if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
                     tile_sort(p, df);
                     next_sort = nstep + rp.tile_sort;
                 }
-                PushTiles t{};
                 t.off = soff.get();
                 t.ntx = ctx->ntx;
                 t.tx = ctx->tx;

@@ -402,7 +402,7 @@ def _tile_sort_run(tile_sort, deltaf, steps=15, seed=11, nx=64, ny=48):
         rp.B0[2] = 0.0
         rp.deltaf, rp.df_tpar, rp.df_tperp = 1, 1.5, 2.0
     rng = np.random.default_rng(seed)
-    n = 200_000 if ny > 1 else 60_000
+    n = {48: 200_000, 1: 60_000}.get(ny, 10 * nx * ny)
     st = {"x": rng.uniform(0, nx, n).astype(np.float32),
           "y": rng.uniform(0, ny, n).astype(np.float32)}
     for k in ("ux", "uy", "uz"):  # up to ~0.8 cells a step, ~2 cells between sorts
@@ -431,14 +431,17 @@ def _as_set(P, keys):
 
 
 @pytest.mark.parametrize("deltaf,nx,ny", [(False, 64, 48), (True, 64, 48), (False, 512, 1),
-                                          (True, 512, 1)])
+                                          (True, 512, 1), (False, 256, 160), (True, 256, 160)])
 def test_tile_sort_matches_unsorted(gpu, deltaf, nx, ny):
     """MaxwellSimulation::step_at with rp.tile_sort (include/pic/simulation_maxwell.hpp:
     60-72): re-sorting the particles every 5 steps only reorders them -- the
     fields after 15 steps are the same bits, and so is the particle set.  The
     tile-sorted run pushes and deposits in one kernel per tile (the plain
     instance without delta-f, the general one with it; one-row grids sum
-    their aliased stencil rows), the unsorted one in a push and a deposit."""
+    their aliased stencil rows); the unsorted one in a push and a deposit
+    (64 x 48, 512 x 1: the current's whole grid fits LDS), or with the same
+    kernel over tile bins of the unmoved arrays (256 x 160, 10 particles per
+    cell)."""
     M0, P0 = _tile_sort_run(0, deltaf, nx=nx, ny=ny)
     M1, P1 = _tile_sort_run(5, deltaf, nx=nx, ny=ny)
     nc = nx * ny
