@@ -42,6 +42,7 @@ constexpr int kThreads = 256;
 constexpr int kRedBlocks = 256;
 constexpr int kPushBlocksPerCu = 8;  // grid-stride push: 8 workgroups of 256 per CU
 constexpr int kPushApron = 2;        // tile push: cells of drift since the sort in LDS
+constexpr int kRebinEvery = 8;       // steps between tile binnings without tile_sort
 
 inline int blocks_for(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
@@ -1193,11 +1194,18 @@ struct aw_maxwell {
             if (fused) {
                 PushTiles t{};
                 if (binned) {
-                    soff.reserve((size_t)ctx->ntiles + 1);
-                    sperm.reserve(n);
-                    bin_tiles(*ctx, p, sperm.get(), soff.get(), stream);
+                    // the bins are rebuilt every kRebinEvery steps: between
+                    // them the particles drift, and the windows' aprons (the
+                    // mesh and global atomics beyond them) take the drift
+                    if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
+                        soff.reserve((size_t)ctx->ntiles + 1);
+                        sperm.reserve(n);
+                        bin_tiles(*ctx, p, sperm.get(), soff.get(), stream);
+                        sorted_n = p.n;
+                        sorted_x = p.x;
+                        next_sort = nstep + kRebinEvery;
+                    }
                     t.idx = sperm.get();
-                    sorted_n = -1;  // the tile order of an earlier sort is gone
                 } else if (nstep >= next_sort || sorted_n != p.n || sorted_x != p.x) {
                     tile_sort(p, df);
                     next_sort = nstep + rp.tile_sort;
