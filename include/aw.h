@@ -608,8 +608,8 @@ aw_status aw_maxwell_mesh(aw_maxwell* m, aw_yee_mesh* out);
  * steps (with the delta-f weight; df_wprec must be 0) and the push and the
  * current run as one kernel per tile (k_push_esirkepov_tiled,
  * include/pic/yee2d.hpp:401-469), else they are never reordered (large
- * grids with >= 8 particles per cell run the same kernel over per-step tile
- * bins).  step uses t = (steps so far) * dt. */
+ * grids with >= 8 particles per cell run the same kernel over tile bins
+ * rebuilt every 8 steps).  step uses t = (steps so far) * dt. */
 aw_status aw_maxwell_step(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df);
 aw_status aw_maxwell_step_at(aw_maxwell* m, aw_particles* p, const aw_deltaf_state* df,
                              double tnow);
