@@ -419,13 +419,11 @@ def yee_eaw_bench(steps: int = 40, warmup: int = 5, tile_sort: bool = True):
     we, wb = sim.field_energy()
     sim.close()
     ref = 1.51e10 if tile_sort else 4.5e9
-    # particle bytes per step: with the tile sort one kernel reads x, y, u, w
-    # (24 B) and writes x, y, u (20 B); without it the push reads 24 B and
-    # writes u and the old and new x, y (28 B), the binning by the old
-    # position reads it twice and writes and reads the permutation (24 B), and
-    # the current reads the move, uz and w (24 B) and writes the wrapped x, y
-    # (8 B)
-    bpp = 44 if tile_sort else 108
+    # particle bytes per step: one kernel reads x, y, u, w (24 B) and writes
+    # x, y, u (20 B); without the tile sort it also reads the bin permutation
+    # (4 B), and the binning every 8 steps (x, y read twice, the permutation
+    # written: 20 B) adds 2.5 B a step
+    bpp = 44 if tile_sort else 50.5
     out = {"workload": "Yee eaw_case7 deck: 625x625 cells, 400 ppc, B0 0.25 x, jfilter 3, " +
                        ("tile_sort 20" if tile_sort else "no tile sort"),
            "particles": n, "steps": steps,
