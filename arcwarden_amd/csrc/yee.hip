@@ -265,12 +265,13 @@ struct WindowGather {
 
 // One particle: the push updates u (and the delta-f weight) in place and
 // returns the move; the caller stores the positions.
-template <bool PLAIN, class Gather>
+template <bool PLAIN, class Gather, bool NOPR = false>
 __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const aw_run_params& rp,
                                          int t, PushOut& o, const Gather& gat) {
     // PLAIN: none of the pump, relativistic push, B0 profile, delta-f weights
-    // or walls (a compile-time instance without their registers)
-    const bool pump = !PLAIN && rp.pump, rel = !PLAIN && rp.rel;
+    // or walls; NOPR: no pump or relativistic push (compile-time instances
+    // without their registers)
+    const bool pump = !PLAIN && !NOPR && rp.pump, rel = !PLAIN && !NOPR && rp.rel;
     const bool b0_prof = !PLAIN && rp.b0_prof, deltaf = !PLAIN && rp.deltaf;
     const int bnd_x = PLAIN ? 0 : rp.bnd_x;
     const float x0 = a.x[t], y0 = a.y[t];
@@ -402,15 +403,18 @@ __device__ __forceinline__ void push_one(const PushArgs& a, const Mesh& m, const
 
 // MaxwellSimulation's particle push (include/pic/yee2d.hpp:140-300), grid-
 // stride; stores the move for the current deposit that follows.  4 waves per
-// SIMD fit without spills.
-template <bool PLAIN>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4)))
+// SIMD fit without spills; without the pump and the relativistic push
+// (NOPR: delta-f, profiles and walls stay) 5 waves do, which took the
+// chen2026 delta-f line from 0.173 to 0.170 ms a step.
+template <bool PLAIN, bool NOPR = false>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(NOPR ? 5 : 4, NOPR ? 5 : 4)))
 yee_push_kernel(PushArgs a, Mesh m, aw_run_params rp) {
     const MeshGather gat{m};
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < a.n;
          t += (int64_t)kThreads * gridDim.x) {
         PushOut o;
-        push_one<PLAIN>(a, m, rp, (int)t, o, gat);
+        push_one<PLAIN, MeshGather, NOPR>(a, m, rp, (int)t, o, gat);
         a.x0[t] = o.x0;
         a.y0[t] = o.y0;
         a.x[t] = o.x1;
@@ -1241,6 +1245,8 @@ struct aw_maxwell {
                     std::min(blocks_for(p.n), kPushBlocksPerCu * std::max(1, ctx->num_cu));
                 if (!(rp.pump || rp.rel || rp.b0_prof || rp.deltaf || rp.bnd_x))
                     yee_push_kernel<true><<<pb, kThreads, 0, stream>>>(a, m, rp);
+                else if (!(rp.pump || rp.rel))
+                    yee_push_kernel<false, true><<<pb, kThreads, 0, stream>>>(a, m, rp);
                 else
                     yee_push_kernel<false><<<pb, kThreads, 0, stream>>>(a, m, rp);
                 AW_LAUNCH_CHECK();
